@@ -1,0 +1,595 @@
+// C ABI of the stage pipeline (include/chasqui_amd.h): context, device arena, transfers,
+// the per-kernel entry points with the reference's semantics, and the device-resident
+// time loop.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/chasqui_amd.h"
+#include "kernels.hpp"
+
+namespace chq {
+hipError_t launch_relayout(void *ref, void *dev, const Geom &g, int nc, int nk, int nj, int ni, int to_dev,
+                           int elem_bytes, hipStream_t s);
+}
+
+using namespace chq;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(CHQ_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// RK coefficient table (g0, g1, beta, cur, out), main.cpp:250-268
+struct Coef { double g0, g1, beta; int cur, out; };
+Coef rk_coef(int nstage, int s) {
+  if (nstage == 1) return {1, 0, 1, 0, 0};
+  if (nstage == 2) return s == 1 ? Coef{1, 0, 1, 0, 1} : Coef{0.5, 0.5, 0.5, 1, 0};
+  if (nstage == 3) {
+    if (s == 1) return {1, 0, 1, 0, 1};
+    if (s == 2) return {0.75, 0.25, 0.25, 1, 1};
+    return {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 1, 0};
+  }
+  if (s == 1) return {1, 0, 0.5, 0, 1};
+  if (s == 2) return {0, 1, 0.5, 1, 1};
+  if (s == 3) return {2.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 1, 1};
+  return {0, 1, 0.5, 1, 0};
+}
+}  // namespace
+
+struct chq_ctx {
+  Geom g{};
+  Eos eos{};
+  chq_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+  double *u[2]{}, *bf[2][3]{}, *w[2]{}, *bcc[2]{}, *ecc[2]{};
+  double *flx[3]{}, *ef[6]{}, *ee[3]{};
+  int *flag = nullptr;
+  Ctrl *ctrl = nullptr;      // time loop
+  Ctrl *ctrl_api = nullptr;  // per-kernel entry points (always active)
+  void *staging = nullptr;   // device staging buffer for layout conversion
+  size_t staging_bytes = 0;
+  int wcur = 0;              // which primitive set (w, bcc, ecc) is current
+  bool entropy = false;      // transport IENT in the time loop (only needed with dual energy)
+
+  // face-flux launch description for direction d from buffer set `cur`
+  FluxArgs flux_args(int d, int cur, int wset, const Ctrl *c) const {
+    FluxArgs a{};
+    a.g = g;
+    a.eos = eos;
+    a.flatten = cfg.flatten;
+    a.low_mach = cfg.low_mach;
+    a.entropy = 1;
+    a.lhlld_phi = cfg.lhlld_phi;
+    // physics.hpp:85-88
+    int il = g.is, iu = g.ie, jl = g.js, ju = g.je, kl = g.ks, ku = g.ke;
+    if (d == 0) iu = g.ie + 1; else if (g.nx1 > 1) { il = g.is - 1; iu = g.ie + 1; }
+    if (d == 1) ju = g.je + 1; else if (g.two_d) { jl = g.js - 1; ju = g.je + 1; }
+    if (d == 2) ku = g.ke + 1; else if (g.three_d) { kl = g.ks - 1; ku = g.ke + 1; }
+    a.il = il; a.iu = iu; a.jl = jl; a.ju = ju; a.kl = kl; a.ku = ku;
+    a.w = w[wset];
+    a.bcc = bcc[wset];
+    a.bn = bf[cur][d];
+    a.flx = flx[d];
+    // et2 = e3x1 / e1x2 / e2x3 ; et1 = e2x1 / e3x2 / e1x3 (physics.hpp:93-94)
+    static const int et2_id[3] = {1, 2, 5}, et1_id[3] = {0, 3, 4};   // ef = e2x1 e3x1 e1x2 e3x2 e1x3 e2x3
+    a.et2 = ef[et2_id[d]];
+    a.et1 = ef[et1_id[d]];
+    a.flag = flag;
+    a.ctrl = c;
+    return a;
+  }
+  CornerArgs corner_args(int wset, const Ctrl *c, int only_if_bad) const {
+    CornerArgs a{};
+    a.g = g;
+    a.f1 = flx[0]; a.f2 = flx[1]; a.f3 = flx[2];
+    a.e2x1 = ef[0]; a.e3x1 = ef[1]; a.e1x2 = ef[2]; a.e3x2 = ef[3]; a.e1x3 = ef[4]; a.e2x3 = ef[5];
+    a.ecc = ecc[wset];
+    a.e1 = ee[0]; a.e2 = ee[1]; a.e3 = ee[2];
+    a.ctrl = c;
+    a.only_if_bad = only_if_bad;
+    return a;
+  }
+  UpdateArgs update_args(int out, double g0, double g1, int nvar, const Ctrl *c, int only_if_bad) const {
+    UpdateArgs a{};
+    a.g = g;
+    a.g0 = g0; a.g1 = g1;
+    a.nvar = nvar;
+    a.f1 = flx[0]; a.f2 = flx[1]; a.f3 = flx[2];
+    a.ua = u[0]; a.ub = u[1]; a.uo = u[out];
+    a.b1a = bf[0][0]; a.b2a = bf[0][1]; a.b3a = bf[0][2];
+    a.b1b = bf[1][0]; a.b2b = bf[1][1]; a.b3b = bf[1][2];
+    a.b1o = bf[out][0]; a.b2o = bf[out][1]; a.b3o = bf[out][2];
+    a.e1 = ee[0]; a.e2 = ee[1]; a.e3 = ee[2];
+    a.ctrl = c;
+    a.only_if_bad = only_if_bad;
+    return a;
+  }
+  C2pArgs c2p_args(int buf, int wset, Ctrl *c) const {
+    C2pArgs a{};
+    a.g = g;
+    a.eos = eos;
+    a.b1 = bf[buf][0]; a.b2 = bf[buf][1]; a.b3 = bf[buf][2];
+    a.u = u[buf];
+    a.w = w[wset]; a.bcc = bcc[wset]; a.ecc = ecc[wset];
+    a.write_went = 1;
+    a.flag = flag;
+    a.ctrl = c;
+    return a;
+  }
+  hipError_t boundaries(int buf, int nvar, const Ctrl *c, int only_if_bad) {
+    BcArgs a{};
+    a.g = g;
+    a.nvar = nvar;
+    a.u = u[buf];
+    a.b1 = bf[buf][0]; a.b2 = bf[buf][1]; a.b3 = bf[buf][2];
+    a.ctrl = c;
+    a.only_if_bad = only_if_bad;
+    a.bc = cfg.bc[0];
+    hipError_t e = launch_bc(a, 0, stream);
+    if (e != hipSuccess) return e;
+    if (g.two_d) { a.bc = cfg.bc[1]; e = launch_bc(a, 1, stream); if (e != hipSuccess) return e; }
+    if (g.three_d) { a.bc = cfg.bc[2]; e = launch_bc(a, 2, stream); }
+    return e;
+  }
+  int ndir() const { return 1 + (g.two_d ? 1 : 0) + (g.three_d ? 1 : 0); }
+};
+
+namespace {
+
+bool array_desc(const chq_ctx *c, int id, void **dev, int *nc, int *face, int *ebytes) {
+  *face = 1; *ebytes = 8; *nc = 1; *dev = nullptr;
+  switch (id) {
+    case CHQ_U0: *dev = c->u[0]; *nc = 6; *face = 0; break;
+    case CHQ_U1: *dev = c->u[1]; *nc = 6; *face = 0; break;
+    case CHQ_W: *dev = c->w[c->wcur]; *nc = 6; *face = 0; break;
+    case CHQ_BCC: *dev = c->bcc[c->wcur]; *nc = 3; *face = 0; break;
+    case CHQ_ECC: *dev = c->ecc[c->wcur]; *nc = 3; *face = 0; break;
+    case CHQ_B1F0: *dev = c->bf[0][0]; break;
+    case CHQ_B2F0: *dev = c->bf[0][1]; break;
+    case CHQ_B3F0: *dev = c->bf[0][2]; break;
+    case CHQ_B1F1: *dev = c->bf[1][0]; break;
+    case CHQ_B2F1: *dev = c->bf[1][1]; break;
+    case CHQ_B3F1: *dev = c->bf[1][2]; break;
+    case CHQ_F1: *dev = c->flx[0]; *nc = 6; break;
+    case CHQ_F2: *dev = c->flx[1]; *nc = 6; break;
+    case CHQ_F3: *dev = c->flx[2]; *nc = 6; break;
+    case CHQ_E2X1: case CHQ_E3X1: case CHQ_E1X2: case CHQ_E3X2: case CHQ_E1X3: case CHQ_E2X3:
+      *dev = c->ef[id - CHQ_E2X1]; break;
+    case CHQ_E1: case CHQ_E2: case CHQ_E3: *dev = c->ee[id - CHQ_E1]; break;
+    case CHQ_FLAG: *dev = c->flag; *face = 0; *ebytes = 4; break;
+    default: return false;
+  }
+  return true;
+}
+
+int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
+  void *dev;
+  int nc, face, eb;
+  if (!array_desc(c, id, &dev, &nc, &face, &eb)) return fail(CHQ_EINVAL, "unknown array id");
+  const Geom &g = c->g;
+  const int nk = g.nk + face, nj = g.nj + face, ni = g.ni + face;
+  const size_t n = size_t(nc) * nk * nj * ni;
+  if (count != n) return fail(CHQ_EINVAL, "host buffer has " + std::to_string(count) + " elements, expected " + std::to_string(n));
+  if (!host) return fail(CHQ_EINVAL, "null host buffer");
+  const size_t bytes = n * eb;
+  if (bytes > c->staging_bytes) {
+    if (c->staging) HIPCHK(hipFree(c->staging));
+    c->staging = nullptr;
+    c->staging_bytes = 0;
+    HIPCHK(hipMalloc(&c->staging, bytes));
+    c->staging_bytes = bytes;
+  }
+  if (to_dev) {
+    HIPCHK(hipMemcpyAsync(c->staging, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_relayout(c->staging, dev, g, nc, nk, nj, ni, 1, eb, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    HIPCHK(launch_relayout(c->staging, dev, g, nc, nk, nj, ni, 0, eb, c->stream));
+    HIPCHK(hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return CHQ_OK;
+}
+
+int check_cfg(const chq_mesh_desc *m, const chq_config *c) {
+  if (!m || !c) return fail(CHQ_EINVAL, "null mesh or config");
+  if (m->nx1 < 1 || m->nx2 < 1 || m->nx3 < 1) return fail(CHQ_EINVAL, "mesh sizes must be >= 1");
+  if (m->nx3 > 1 && m->nx2 <= 1) return fail(CHQ_EINVAL, "a 3D mesh needs nx2 > 1");
+  if (c->recon < 0 || c->recon > 4) return fail(CHQ_EINVAL, "unknown reconstruction");
+  if (c->rsolver < 0 || c->rsolver > 2) return fail(CHQ_EINVAL, "unknown Riemann solver");
+  if (c->nstage < 1 || c->nstage > 4) return fail(CHQ_EINVAL, "nstage must be 1..4");
+  const int ngmin = (c->recon == 2 || c->recon == 3 || c->recon == 4) ? 3 : 2;   // main.cpp:169-171
+  if (m->ng < ngmin) return fail(CHQ_EINVAL, "nghost too small for the reconstruction method");
+  for (int d = 0; d < 3; ++d)
+    if (c->bc[d] < 0 || c->bc[d] > 2) return fail(CHQ_EINVAL, "unknown boundary condition");
+  // a periodic ghost fill reads nx cells back from the ghost: it needs nx >= ng
+  if (m->nx1 < m->ng || (m->nx2 > 1 && m->nx2 < m->ng) || (m->nx3 > 1 && m->nx3 < m->ng))
+    return fail(CHQ_EINVAL, "every multi-cell dimension needs at least nghost active cells");
+  return CHQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *chq_last_error(void) { return g_err.c_str(); }
+const char *chq_version(void) { return "chasqui_amd 0.1 (gfx950)"; }
+
+int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
+  if (!out) return fail(CHQ_EINVAL, "null output pointer");
+  *out = nullptr;
+  int rc = check_cfg(m, cfg);
+  if (rc) return rc;
+  chq_ctx *c = new (std::nothrow) chq_ctx();
+  if (!c) return fail(CHQ_ENOMEM, "host allocation failed");
+  c->cfg = *cfg;
+  Geom &g = c->g;
+  // Mesh::Allocate, mesh.hpp:37-48
+  g.nx1 = m->nx1; g.nx2 = m->nx2; g.nx3 = m->nx3; g.ng = m->ng;
+  g.ni = g.nx1 + 2 * g.ng;
+  g.nj = (g.nx2 > 1) ? g.nx2 + 2 * g.ng : 1;
+  g.nk = (g.nx3 > 1) ? g.nx3 + 2 * g.ng : 1;
+  g.is = g.ng; g.ie = g.ng + g.nx1 - 1;
+  g.js = (g.nx2 > 1) ? g.ng : 0; g.je = (g.nx2 > 1) ? g.ng + g.nx2 - 1 : 0;
+  g.ks = (g.nx3 > 1) ? g.ng : 0; g.ke = (g.nx3 > 1) ? g.ng + g.nx3 - 1 : 0;
+  g.two_d = g.nx2 > 1; g.three_d = g.nx3 > 1;
+  g.dx1 = (m->x1max - m->x1min) / g.nx1;
+  g.dx2 = (m->x2max - m->x2min) / g.nx2;
+  g.dx3 = (m->x3max - m->x3min) / g.nx3;
+  g.idx1 = 1.0 / g.dx1; g.idx2 = 1.0 / g.dx2; g.idx3 = 1.0 / g.dx3;   // physics.hpp:309
+  g.ioff = (16 - g.is % 16) % 16;
+  g.pitch = ((g.ioff + g.ni + 1 + 15) / 16) * 16;
+  g.plane = (long long)g.pitch * (g.nj + 1);
+  g.cstride = g.plane * (g.nk + 1);
+  c->eos.gamma = cfg->gamma;
+  c->eos.gm1 = cfg->gamma - 1.0;
+  c->eos.dfloor = cfg->dfloor;
+  c->eos.pfloor = cfg->pfloor;
+  c->eos.dual_energy = cfg->dual_energy;
+  c->eos.de_eta = cfg->de_eta;
+  c->entropy = cfg->dual_energy != 0;
+
+  auto cleanup = [&](int code) { chq_destroy(c); return code; };
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) return cleanup(fail(CHQ_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
+    return cleanup(fail(CHQ_EHIP, std::string("stream/event: ") + hipGetErrorString(e)));
+  // arena: 70 component arrays of cstride doubles + two control blocks
+  const size_t comp = size_t(g.cstride) * sizeof(double);
+  const size_t ncomp = 12 + 6 + 12 + 6 + 6 + 18 + 6 + 3 + 1;
+  c->arena_bytes = ncomp * comp + 2 * 256;
+  e = hipMalloc((void **)&c->arena, c->arena_bytes);
+  if (e != hipSuccess) { c->arena = nullptr; return cleanup(fail(CHQ_ENOMEM, "device arena of " + std::to_string(c->arena_bytes) + " bytes: " + hipGetErrorString(e))); }
+  e = hipMemsetAsync(c->arena, 0, c->arena_bytes, c->stream);
+  if (e != hipSuccess) return cleanup(fail(CHQ_EHIP, "arena clear"));
+  char *p = c->arena;
+  auto take = [&](int n) { double *r = (double *)p; p += size_t(n) * comp; return r; };
+  for (int s = 0; s < 2; ++s) c->u[s] = take(6);
+  for (int s = 0; s < 2; ++s) for (int d = 0; d < 3; ++d) c->bf[s][d] = take(1);
+  for (int s = 0; s < 2; ++s) c->w[s] = take(6);
+  for (int s = 0; s < 2; ++s) c->bcc[s] = take(3);
+  for (int s = 0; s < 2; ++s) c->ecc[s] = take(3);
+  for (int d = 0; d < 3; ++d) c->flx[d] = take(6);
+  for (int q = 0; q < 6; ++q) c->ef[q] = take(1);
+  for (int q = 0; q < 3; ++q) c->ee[q] = take(1);
+  c->flag = (int *)take(1);
+  c->ctrl = (Ctrl *)p;
+  c->ctrl_api = (Ctrl *)(p + 256);
+  Ctrl h{};
+  h.active = 1;
+  h.tlim = 1e300;
+  h.nlim = -1;
+  h.cfl = cfg->cfl;
+  e = hipMemcpyAsync(c->ctrl_api, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return cleanup(fail(CHQ_EHIP, std::string("control block: ") + hipGetErrorString(e)));
+  *out = c;
+  return CHQ_OK;
+}
+
+int chq_destroy(chq_ctx *c) {
+  if (!c) return CHQ_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->arena) hipFree(c->arena);
+  if (c->staging) hipFree(c->staging);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return CHQ_OK;
+}
+
+int chq_geometry(const chq_ctx *c, int *iv, double *dx) {
+  if (!c) return fail(CHQ_EINVAL, "null context");
+  const Geom &g = c->g;
+  if (iv) {
+    const int a[16] = {g.nx1, g.nx2, g.nx3, g.ng, g.ni, g.nj, g.nk, g.is, g.ie, g.js, g.je, g.ks, g.ke,
+                       g.two_d, g.three_d, g.pitch};
+    std::memcpy(iv, a, sizeof(a));
+  }
+  if (dx) { dx[0] = g.dx1; dx[1] = g.dx2; dx[2] = g.dx3; }
+  return CHQ_OK;
+}
+
+size_t chq_device_bytes(const chq_ctx *c) { return c ? c->arena_bytes + c->staging_bytes : 0; }
+
+size_t chq_array_count(const chq_ctx *c, int id) {
+  void *dev;
+  int nc, face, eb;
+  if (!c || !array_desc(c, id, &dev, &nc, &face, &eb)) return 0;
+  return size_t(nc) * (c->g.nk + face) * (c->g.nj + face) * (c->g.ni + face);
+}
+
+int chq_upload(chq_ctx *c, int id, const void *host, size_t count) {
+  if (!c) return fail(CHQ_EINVAL, "null context");
+  return relayout(c, id, const_cast<void *>(host), count, true);
+}
+
+int chq_download(chq_ctx *c, int id, void *host, size_t count) {
+  if (!c) return fail(CHQ_EINVAL, "null context");
+  return relayout(c, id, host, count, false);
+}
+
+// ---------------------------------------------------------------- per-kernel entry points
+int chq_fluxes(chq_ctx *c, int dir, int cur) {
+  if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
+  FluxArgs a = c->flux_args(dir, cur, c->wcur, c->ctrl_api);
+  HIPCHK(launch_flux(a, dir, Recon(c->cfg.recon), RSolver(c->cfg.rsolver), true, c->stream));
+  return CHQ_OK;
+}
+
+int chq_corner_emf(chq_ctx *c) {
+  if (!c) return fail(CHQ_EINVAL, "null context");
+  HIPCHK(launch_corner(c->corner_args(c->wcur, c->ctrl_api, 0), c->stream));
+  return CHQ_OK;
+}
+
+int chq_update(chq_ctx *c, int out, double g0, double g1, double beta_dt) {
+  if (!c || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  UpdateArgs a = c->update_args(out, g0, g1, 6, c->ctrl_api, 0);
+  a.use_ctrl_dt = 0;
+  a.beta_dt = beta_dt;
+  HIPCHK(launch_update(a, c->stream));
+  return CHQ_OK;
+}
+
+int chq_boundaries(chq_ctx *c, int buf) {
+  if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  HIPCHK(c->boundaries(buf, 6, c->ctrl_api, 0));
+  return CHQ_OK;
+}
+
+int chq_cons_to_prim(chq_ctx *c, int buf, int compute_dt, double *min_dt) {
+  if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  C2pArgs a = c->c2p_args(buf, c->wcur, c->ctrl_api);
+  a.compute_dt = compute_dt ? 1 : 0;
+  a.slot = 0;
+  HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 1, c->stream));
+  HIPCHK(launch_c2p(a, c->stream));
+  if (min_dt) {
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, &c->ctrl_api->mdt_bits[0], sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *min_dt = compute_dt ? v : 1e300;
+  }
+  return CHQ_OK;
+}
+
+int chq_fofc_flag(chq_ctx *c, int buf, int *count) {
+  if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  C2pArgs a = c->c2p_args(buf, c->wcur, c->ctrl_api);
+  HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 0, c->stream));
+  HIPCHK(launch_flag(a, c->stream));
+  if (count) {
+    HIPCHK(hipMemcpyAsync(count, &c->ctrl_api->nbad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return CHQ_OK;
+}
+
+int chq_fofc_repair(chq_ctx *c, int dir, int cur) {
+  if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
+  FluxArgs a = c->flux_args(dir, cur, c->wcur, c->ctrl_api);
+  HIPCHK(launch_repair(a, dir, false, c->stream));
+  return CHQ_OK;
+}
+
+// Stage(), main.cpp:46-93, reference order (flag before boundaries; fofc branch taken on the
+// device when the flag kernel counted cells)
+int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt, long *nbad) {
+  if (!c || (cur & ~1) || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  const Ctrl *k = c->ctrl_api;
+  const Recon rc = Recon(c->cfg.recon);
+  const RSolver rs = RSolver(c->cfg.rsolver);
+  for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_flux(c->flux_args(d, cur, c->wcur, k), d, rc, rs, true, c->stream));
+  HIPCHK(launch_corner(c->corner_args(c->wcur, k, 0), c->stream));
+  UpdateArgs ua = c->update_args(out, g0, g1, 6, k, 0);
+  ua.beta_dt = beta_dt;
+  HIPCHK(launch_update(ua, c->stream));
+  HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 0, c->stream));
+  if (c->cfg.fofc) {
+    HIPCHK(launch_flag(c->c2p_args(out, c->wcur, c->ctrl_api), c->stream));
+    for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_repair(c->flux_args(d, cur, c->wcur, k), d, true, c->stream));
+    HIPCHK(launch_corner(c->corner_args(c->wcur, k, 1), c->stream));
+    UpdateArgs ub = c->update_args(out, g0, g1, 6, k, 1);
+    ub.beta_dt = beta_dt;
+    HIPCHK(launch_update(ub, c->stream));
+  }
+  HIPCHK(c->boundaries(out, 6, k, 0));
+  if (nbad) {
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, &c->ctrl_api->nbad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *nbad = n;
+  }
+  return CHQ_OK;
+}
+
+// ---------------------------------------------------------------- time loop
+int chq_init(chq_ctx *c, double t, double tlim, int nlim) {
+  if (!c) return fail(CHQ_EINVAL, "null context");
+  HIPCHK(c->boundaries(0, 6, c->ctrl_api, 0));
+  double mdt = 0.0;
+  int rc = chq_cons_to_prim(c, 0, 1, &mdt);
+  if (rc) return rc;
+  Ctrl h{};
+  h.t = t;
+  h.tlim = tlim;
+  h.nlim = nlim;
+  h.cfl = c->cfg.cfl;
+  h.min_dt = std::fmin(mdt, 1e300);   // main.cpp:201-202
+  h.active = 0;
+  HIPCHK(hipMemcpyAsync(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CHQ_OK;
+}
+
+// One RK step on the device. Fast order (dual energy off): the fofc flag is evaluated inside
+// con2prim of the tentative state, whose primitives go to the second (w, bcc, ecc) set, so
+// the stage-input primitives the repair needs stay intact; the repair branch (repair,
+// corner, update, boundaries, con2prim) always launches and exits at once on the device
+// when nothing was flagged. Entropy (IENT) is not transported: with dual energy off
+// con2prim recomputes u[IENT] from rho and p in every stage, so its transport cannot
+// change any result.
+static int enqueue_step(chq_ctx *c) {
+  const Recon rc = Recon(c->cfg.recon);
+  const RSolver rs = RSolver(c->cfg.rsolver);
+  Ctrl *k = c->ctrl;
+  const int nst = c->cfg.nstage;
+  const bool fused = !c->eos.dual_energy;
+  const int nvar = c->entropy ? 6 : 5;
+  HIPCHK(launch_ctrl_begin_step(k, c->stream));
+  for (int s = 1; s <= nst; ++s) {
+    const Coef cf = rk_coef(nst, s);
+    const int last = s == nst;
+    const int wc = c->wcur, wn = fused ? (c->wcur ^ 1) : c->wcur;
+    HIPCHK(launch_ctrl_begin_stage(k, last, c->stream));
+    for (int d = 0; d < c->ndir(); ++d) {
+      FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
+      fa.entropy = c->entropy;
+      HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
+    }
+    HIPCHK(launch_corner(c->corner_args(wc, k, 0), c->stream));
+    UpdateArgs ua = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 0);
+    ua.use_ctrl_dt = 1;
+    ua.beta = cf.beta;
+    HIPCHK(launch_update(ua, c->stream));
+    if (fused) {
+      HIPCHK(c->boundaries(cf.out, nvar, k, 0));
+      C2pArgs ca = c->c2p_args(cf.out, wn, k);
+      ca.write_went = c->entropy;
+      ca.compute_dt = last;
+      ca.slot = 0;
+      ca.flag_mode = c->cfg.fofc ? 1 : 0;
+      HIPCHK(launch_c2p(ca, c->stream));
+      if (c->cfg.fofc) {
+        for (int d = 0; d < c->ndir(); ++d) {
+          FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
+          fa.entropy = c->entropy;
+          HIPCHK(launch_repair(fa, d, true, c->stream));
+        }
+        HIPCHK(launch_corner(c->corner_args(wc, k, 1), c->stream));
+        UpdateArgs ub = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 1);
+        ub.use_ctrl_dt = 1;
+        ub.beta = cf.beta;
+        HIPCHK(launch_update(ub, c->stream));
+        HIPCHK(c->boundaries(cf.out, nvar, k, 1));
+        C2pArgs cb = ca;
+        cb.flag_mode = 0;
+        cb.slot = 1;
+        cb.only_if_bad = 1;
+        HIPCHK(launch_c2p(cb, c->stream));
+      }
+    } else {
+      if (c->cfg.fofc) {
+        HIPCHK(launch_flag(c->c2p_args(cf.out, wc, k), c->stream));
+        for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_repair(c->flux_args(d, cf.cur, wc, k), d, true, c->stream));
+        HIPCHK(launch_corner(c->corner_args(wc, k, 1), c->stream));
+        UpdateArgs ub = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 1);
+        ub.use_ctrl_dt = 1;
+        ub.beta = cf.beta;
+        HIPCHK(launch_update(ub, c->stream));
+      }
+      HIPCHK(c->boundaries(cf.out, nvar, k, 0));
+      C2pArgs ca = c->c2p_args(cf.out, wc, k);
+      ca.compute_dt = last;
+      ca.slot = 0;
+      HIPCHK(launch_c2p(ca, c->stream));
+    }
+    HIPCHK(launch_ctrl_end_stage(k, last, fused ? 1 : 0, c->stream));
+    c->wcur = wn;
+  }
+  HIPCHK(launch_ctrl_end_step(k, c->stream));
+  return CHQ_OK;
+}
+
+int chq_step(chq_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) return fail(CHQ_EINVAL, "bad argument");
+  for (int n = 0; n < nsteps; ++n) {
+    const int rc = enqueue_step(c);
+    if (rc) return rc;
+  }
+  return CHQ_OK;
+}
+
+int chq_step_timed(chq_ctx *c, int nsteps, double *ms) {
+  if (!c || nsteps < 0 || !ms) return fail(CHQ_EINVAL, "bad argument");
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  const int rc = chq_step(c, nsteps);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float f = 0.0f;
+  HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+  *ms = f;
+  return CHQ_OK;
+}
+
+int chq_get_status(chq_ctx *c, chq_status *st) {
+  if (!c || !st) return fail(CHQ_EINVAL, "bad argument");
+  Ctrl h{};
+  HIPCHK(hipMemcpyAsync(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  st->t = h.t;
+  st->dt = h.dt;
+  st->tlim = h.tlim;
+  st->min_dt = h.min_dt;
+  st->cycle = h.cycle;
+  st->nlim = h.nlim;
+  st->active = h.active;
+  st->nbad_last = h.nbad_last;
+  st->fofc_total = h.fofc_total;
+  return CHQ_OK;
+}
+
+int chq_sync(chq_ctx *c) {
+  if (!c) return fail(CHQ_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CHQ_OK;
+}
+
+void *chq_stream(chq_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,485 @@
+// Memory-bound stage kernels for gfx950: upwind corner EMF, RK combine + flux divergence +
+// CT update, boundaries, con2prim (+ cell-centred B/EMF, CFL minimum and, in the fused step,
+// the fofc flag), the standalone fofc flag, and the device-resident time-loop control.
+//
+// Reference: kernels::CornerEMF / Update / Boundaries / ConsToPrim (physics.hpp:242-494),
+// fofc::FlagBadCells (fofc.hpp:17-44) and the driver's dt logic (main.cpp:246-292). For
+// FP64 every expression keeps the reference's operation order (bitwise parity).
+//
+// Launch shape: blocks of 64 x 4 threads (i fastest), one wave64 per row segment, so all
+// loads and stores of a wave are contiguous 512 B segments.
+#include "kernels.hpp"
+#include "numerics.hpp"
+
+namespace chq {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+__device__ __forceinline__ bool gate(const Ctrl *c, int only_if_bad) {
+  if (!c->active) return false;
+  if (only_if_bad && c->nbad == 0) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------- corner EMF
+__device__ __forceinline__ double upw(double flux, double left, double right) {   // physics.hpp:233-236
+  const double tol = Lit<double>::upwind_tol;
+  return (flux > tol) ? left : (flux < -tol) ? right : 0.5 * (left + right);
+}
+
+// 3D: one thread per (k, j, i) of the union box [ks..ke+1] x [js..je+1] x [is..ie+1];
+// e1 for i <= ie, e2 for j <= je, e3 for k <= ke (physics.hpp:265-296).
+__global__ void __launch_bounds__(BX * BY) k_corner3(const CornerArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int k = g.ks + int(blockIdx.z);
+  if (i > g.ie + 1 || j > g.je + 1) return;
+  const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
+  const double *ecc = a.ecc;
+  if (i <= g.ie) {   // E1 at (k-1/2, j-1/2, i)
+    const double *ex = ecc;
+    const double l3 = upw(a.f2[c - sk], a.e1x3[c - sj] - ex[c - sk - sj], a.e1x3[c] - ex[c - sk]);
+    const double r3 = upw(a.f2[c], a.e1x3[c - sj] - ex[c - sj], a.e1x3[c] - ex[c]);
+    const double l2 = upw(a.f3[c - sj], a.e1x2[c - sk] - ex[c - sk - sj], a.e1x2[c] - ex[c - sj]);
+    const double r2 = upw(a.f3[c], a.e1x2[c - sk] - ex[c - sk], a.e1x2[c] - ex[c]);
+    a.e1[c] = 0.25 * (((l3 + r3) + (l2 + r2)) + ((a.e1x2[c - sk] + a.e1x2[c]) + (a.e1x3[c - sj] + a.e1x3[c])));
+  }
+  if (j <= g.je) {   // E2 at (k-1/2, j, i-1/2)
+    const double *ey = ecc + cs;
+    const double l3 = upw(a.f1[c - sk], a.e2x3[c - 1] - ey[c - sk - 1], a.e2x3[c] - ey[c - sk]);
+    const double r3 = upw(a.f1[c], a.e2x3[c - 1] - ey[c - 1], a.e2x3[c] - ey[c]);
+    const double l1 = upw(a.f3[c - 1], a.e2x1[c - sk] - ey[c - sk - 1], a.e2x1[c] - ey[c - 1]);
+    const double r1 = upw(a.f3[c], a.e2x1[c - sk] - ey[c - sk], a.e2x1[c] - ey[c]);
+    a.e2[c] = 0.25 * (((l3 + r3) + (l1 + r1)) + ((a.e2x3[c - 1] + a.e2x3[c]) + (a.e2x1[c - sk] + a.e2x1[c])));
+  }
+  if (k <= g.ke) {   // E3 at (k, j-1/2, i-1/2)
+    const double *ez = ecc + 2 * cs;
+    const double l2 = upw(a.f1[c - sj], a.e3x2[c - 1] - ez[c - sj - 1], a.e3x2[c] - ez[c - sj]);
+    const double r2 = upw(a.f1[c], a.e3x2[c - 1] - ez[c - 1], a.e3x2[c] - ez[c]);
+    const double l1 = upw(a.f2[c - 1], a.e3x1[c - sj] - ez[c - sj - 1], a.e3x1[c] - ez[c - 1]);
+    const double r1 = upw(a.f2[c], a.e3x1[c - sj] - ez[c - sj], a.e3x1[c] - ez[c]);
+    a.e3[c] = 0.25 * (((l2 + r2) + (l1 + r1)) + ((a.e3x2[c - 1] + a.e3x2[c]) + (a.e3x1[c - sj] + a.e3x1[c])));
+  }
+}
+
+// 2D and 1D: E1/E2 are copies of the face EMFs on the two k-planes (physics.hpp:250-256);
+// E3 is the Gardiner-Stone average in 2D (288-296) or a copy in 1D (257-261).
+__global__ void __launch_bounds__(BX * BY) k_corner12(const CornerArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int k = g.ks;
+  if (i > g.ie + 1 || j > g.je + 1) return;
+  const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
+  a.e1[c] = a.e1x2[c];
+  a.e1[c + sk] = a.e1x2[c];
+  a.e2[c] = a.e2x1[c];
+  a.e2[c + sk] = a.e2x1[c];
+  if (g.two_d) {
+    const double *ez = a.ecc + 2 * cs;
+    const double l2 = upw(a.f1[c - sj], a.e3x2[c - 1] - ez[c - sj - 1], a.e3x2[c] - ez[c - sj]);
+    const double r2 = upw(a.f1[c], a.e3x2[c - 1] - ez[c - 1], a.e3x2[c] - ez[c]);
+    const double l1 = upw(a.f2[c - 1], a.e3x1[c - sj] - ez[c - sj - 1], a.e3x1[c] - ez[c - 1]);
+    const double r1 = upw(a.f2[c], a.e3x1[c - sj] - ez[c - sj], a.e3x1[c] - ez[c]);
+    a.e3[c] = 0.25 * (((l2 + r2) + (l1 + r1)) + ((a.e3x2[c - 1] + a.e3x2[c]) + (a.e3x1[c - sj] + a.e3x1[c])));
+  } else if (j == g.js) {
+    a.e3[c] = a.e3x1[c];
+    a.e3[c + sj] = a.e3x1[c];
+  }
+}
+
+// ---------------------------------------------------------------- update
+// One thread per (k, j, i) of [ks..ke+1] x [js..je+1] x [is..ie+1]: the conserved update of
+// the active cell (physics.hpp:314-322) and the CT update of its three lower faces
+// (physics.hpp:327-347). In-place (uo aliasing ua or ub) is pointwise safe.
+__global__ void __launch_bounds__(BX * BY) k_update(const UpdateArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int k = g.ks + int(blockIdx.z);
+  if (i > g.ie + 1 || j > g.je + 1) return;
+  const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
+  const double g0 = a.g0, g1 = a.g1;
+  const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
+  const double idx1 = g.idx1, idx2 = g.idx2, idx3 = g.idx3;
+  if (i <= g.ie && j <= g.je && k <= g.ke) {
+    for (int n = 0; n < a.nvar; ++n) {
+      const long long o = n * cs + c;
+      double dv = (a.f1[o + 1] - a.f1[o]) * idx1;
+      if (g.two_d) dv += (a.f2[o + sj] - a.f2[o]) * idx2;
+      if (g.three_d) dv += (a.f3[o + sk] - a.f3[o]) * idx3;
+      a.uo[o] = g0 * a.ua[o] + g1 * a.ub[o] - bdt * dv;
+    }
+  }
+  if (j <= g.je && k <= g.ke) {   // B1 on x1-faces
+    double cr = 0.0;
+    if (g.two_d) cr += (a.e3[c + sj] - a.e3[c]) * idx2;
+    if (g.three_d) cr -= (a.e2[c + sk] - a.e2[c]) * idx3;
+    a.b1o[c] = g0 * a.b1a[c] + g1 * a.b1b[c] - bdt * cr;
+  }
+  if (i <= g.ie && k <= g.ke && (g.two_d || j == g.js)) {   // B2 on x2-faces
+    double cr = -(a.e3[c + 1] - a.e3[c]) * idx1;
+    if (g.three_d) cr += (a.e1[c + sk] - a.e1[c]) * idx3;
+    const double v = g0 * a.b2a[c] + g1 * a.b2b[c] - bdt * cr;
+    a.b2o[c] = v;
+    if (!g.two_d) a.b2o[c + sj] = v;
+  }
+  if (i <= g.ie && j <= g.je && (g.three_d || k == g.ks)) {   // B3 on x3-faces
+    double cr = (a.e2[c + 1] - a.e2[c]) * idx1;
+    if (g.two_d) cr -= (a.e1[c + sj] - a.e1[c]) * idx2;
+    const double v = g0 * a.b3a[c] + g1 * a.b3b[c] - bdt * cr;
+    a.b3o[c] = v;
+    if (!g.three_d) a.b3o[c + sk] = v;
+  }
+}
+
+// ---------------------------------------------------------------- boundaries
+// physics.hpp:354-432. x1: threads over (k, j, g) with g fastest; x2: (k, g, i); x3: (g, j, i).
+__global__ void __launch_bounds__(256) k_bc1(const BcArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const int ng = g.ng;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n_all = (long long)(g.nk + 1) * (g.nj + 1) * ng;
+  if (t >= n_all) return;
+  const int gg = int(t % ng), j = int((t / ng) % (g.nj + 1)), k = int(t / ((long long)ng * (g.nj + 1)));
+  const int bc = a.bc, nx = g.nx1, is = g.is, ie = g.ie;
+  const int il = gg, ir = ie + 1 + gg;
+  const int sl = bc == 0 ? il + nx : bc == 1 ? is : 2 * is - 1 - il;
+  const int sr = bc == 0 ? ir - nx : bc == 1 ? ie : 2 * ie + 1 - ir;
+  const double sg = bc == 2 ? -1.0 : 1.0;
+  const long long row = g.at(k, j, 0), cs = g.cstride;
+  if (k < g.nk && j < g.nj) {
+    for (int n = 0; n < a.nvar; ++n) {
+      const double f = n == IM1 ? sg : 1.0;
+      double *u = a.u + n * cs + row;
+      const double vl = u[sl], vr = u[sr];
+      u[il] = f * vl;
+      u[ir] = f * vr;
+    }
+    a.b2[row + il] = a.b2[row + sl];
+    a.b2[row + ir] = a.b2[row + sr];
+    a.b3[row + il] = a.b3[row + sl];
+    a.b3[row + ir] = a.b3[row + sr];
+  }
+  double *b1 = a.b1 + row;
+  const int fl = gg, fr = ie + 2 + gg;
+  if (bc == 0) { b1[fl] = b1[fl + nx]; b1[fr] = b1[fr - nx]; }
+  else if (bc == 1) { b1[fl] = b1[is]; b1[fr] = b1[ie + 1]; }
+  else { b1[fl] = -b1[2 * is - fl]; b1[fr] = -b1[2 * (ie + 1) - fr]; }
+  if (bc == 0 && gg == 0) b1[ie + 1] = b1[is];
+}
+
+__global__ void __launch_bounds__(256) k_bc2(const BcArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const int ng = g.ng;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n_all = (long long)(g.nk + 1) * ng * (g.ni + 1);
+  if (t >= n_all) return;
+  const int i = int(t % (g.ni + 1)), gg = int((t / (g.ni + 1)) % ng), k = int(t / ((long long)ng * (g.ni + 1)));
+  const int bc = a.bc, ny = g.nx2, js = g.js, je = g.je;
+  const int jl = gg, jr = je + 1 + gg;
+  const int sl = bc == 0 ? jl + ny : bc == 1 ? js : 2 * js - 1 - jl;
+  const int sr = bc == 0 ? jr - ny : bc == 1 ? je : 2 * je + 1 - jr;
+  const double sg = bc == 2 ? -1.0 : 1.0;
+  const long long sj = g.pitch, cs = g.cstride;
+  const long long col = g.at(k, 0, i);
+  if (k < g.nk && i < g.ni) {
+    for (int n = 0; n < a.nvar; ++n) {
+      const double f = n == IM2 ? sg : 1.0;
+      double *u = a.u + n * cs + col;
+      const double vl = u[sl * sj], vr = u[sr * sj];
+      u[jl * sj] = f * vl;
+      u[jr * sj] = f * vr;
+    }
+    a.b3[col + jl * sj] = a.b3[col + sl * sj];
+    a.b3[col + jr * sj] = a.b3[col + sr * sj];
+  }
+  if (k < g.nk) {
+    a.b1[col + jl * sj] = a.b1[col + sl * sj];
+    a.b1[col + jr * sj] = a.b1[col + sr * sj];
+  }
+  double *b2 = a.b2 + col;
+  const int fl = gg, fr = je + 2 + gg;
+  if (bc == 0) { b2[fl * sj] = b2[(fl + ny) * sj]; b2[fr * sj] = b2[(fr - ny) * sj]; }
+  else if (bc == 1) { b2[fl * sj] = b2[js * sj]; b2[fr * sj] = b2[(je + 1) * sj]; }
+  else { b2[fl * sj] = -b2[(2 * js - fl) * sj]; b2[fr * sj] = -b2[(2 * (je + 1) - fr) * sj]; }
+  if (bc == 0 && gg == 0) b2[(je + 1) * sj] = b2[js * sj];
+}
+
+__global__ void __launch_bounds__(256) k_bc3(const BcArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const int ng = g.ng;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n_all = (long long)ng * (g.nj + 1) * (g.ni + 1);
+  if (t >= n_all) return;
+  const int i = int(t % (g.ni + 1)), j = int((t / (g.ni + 1)) % (g.nj + 1)), gg = int(t / ((long long)(g.nj + 1) * (g.ni + 1)));
+  const int bc = a.bc, nz = g.nx3, ks = g.ks, ke = g.ke;
+  const int kl = gg, kr = ke + 1 + gg;
+  const int sl = bc == 0 ? kl + nz : bc == 1 ? ks : 2 * ks - 1 - kl;
+  const int sr = bc == 0 ? kr - nz : bc == 1 ? ke : 2 * ke + 1 - kr;
+  const double sg = bc == 2 ? -1.0 : 1.0;
+  const long long sk = g.plane, cs = g.cstride;
+  const long long col = g.at(0, j, i);
+  if (j < g.nj && i < g.ni) {
+    for (int n = 0; n < a.nvar; ++n) {
+      const double f = n == IM3 ? sg : 1.0;
+      double *u = a.u + n * cs + col;
+      const double vl = u[sl * sk], vr = u[sr * sk];
+      u[kl * sk] = f * vl;
+      u[kr * sk] = f * vr;
+    }
+  }
+  a.b1[col + kl * sk] = a.b1[col + sl * sk];
+  a.b1[col + kr * sk] = a.b1[col + sr * sk];
+  a.b2[col + kl * sk] = a.b2[col + sl * sk];
+  a.b2[col + kr * sk] = a.b2[col + sr * sk];
+  double *b3 = a.b3 + col;
+  const int fl = gg, fr = ke + 2 + gg;
+  if (bc == 0) { b3[fl * sk] = b3[(fl + nz) * sk]; b3[fr * sk] = b3[(fr - nz) * sk]; }
+  else if (bc == 1) { b3[fl * sk] = b3[ks * sk]; b3[fr * sk] = b3[(ke + 1) * sk]; }
+  else { b3[fl * sk] = -b3[(2 * ks - fl) * sk]; b3[fr * sk] = -b3[(2 * (ke + 1) - fr) * sk]; }
+  if (bc == 0 && gg == 0) b3[(ke + 1) * sk] = b3[ks * sk];
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// cons -> prim over all cells, physics.hpp:437-494, with the CFL minimum reduced by wave64
+// shuffles, one LDS pass and one 64-bit atomic per block. flag_mode 1 adds the fofc flag of
+// the (tentative) active cells with the expressions of fofc.hpp:24-40.
+__global__ void __launch_bounds__(BX * BY) k_c2p(const C2pArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const int i = int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = int(blockIdx.y) * BY + int(threadIdx.y);
+  const int k = int(blockIdx.z);
+  const Eos e = a.eos;
+  double mdt = 1e300;
+  int bad_count = 0;
+  if (i < g.ni && j < g.nj) {
+    const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
+    const double bx = 0.5 * (a.b1[c] + a.b1[c + 1]);
+    const double by = g.two_d ? 0.5 * (a.b2[c] + a.b2[c + sj]) : a.b2[c];
+    const double bz = g.three_d ? 0.5 * (a.b3[c] + a.b3[c + sk]) : a.b3[c];
+    double *u = a.u + c;
+    const double u0 = u[IDN * cs], u1 = u[IM1 * cs], u2 = u[IM2 * cs], u3 = u[IM3 * cs], u4 = u[IEN * cs];
+    const bool active = i >= g.is && i <= g.ie && j >= g.js && j <= g.je && k >= g.ks && k <= g.ke;
+    if (a.flag_mode && active) {   // fofc.hpp:24-40 on the tentative state
+      bool bad = u0 <= e.dfloor;
+      if (!bad) {
+        const double kin = 0.5 * (sq(u1) + sq(u2) + sq(u3)) / u0;
+        const double ei = u4 - kin - 0.5 * (bx * bx + by * by + bz * bz);
+        double p = e.gm1 * ei;
+        if (e.dual_energy && ei < e.de_eta * fabs(u4)) p = (u[IENT * cs] / u0) * pow(u0, e.gamma);
+        bad = p <= e.pfloor;
+      }
+      a.flag[c] = bad ? 1 : 0;
+      bad_count = bad ? 1 : 0;
+    }
+    const double d = fmax(u0, e.dfloor);
+    const double id = 1.0 / d;
+    const double vx = u1 * id, vy = u2 * id, vz = u3 * id;
+    const double bsq = bx * bx + by * by + bz * bz;
+    const double kin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+    const double ei = u4 - kin - 0.5 * bsq;
+    double p = e.gm1 * ei;
+    double se;
+    if (e.dual_energy && ei < e.de_eta * fabs(u4)) {
+      se = u[IENT * cs] * id;
+      p = se * pow(d, e.gamma);
+      p = fmax(p, e.pfloor);
+      u[IEN * cs] = kin + 0.5 * bsq + p / e.gm1;
+    } else {
+      p = fmax(p, e.pfloor);
+      se = p / pow(d, e.gamma);
+      u[IENT * cs] = d * se;
+    }
+    double *w = a.w + c;
+    w[IDN * cs] = d;
+    w[IVX * cs] = vx;
+    w[IVY * cs] = vy;
+    w[IVZ * cs] = vz;
+    w[IPR * cs] = p;
+    if (a.write_went) w[IENT * cs] = se;
+    a.bcc[c] = bx;
+    a.bcc[c + cs] = by;
+    a.bcc[c + 2 * cs] = bz;
+    a.ecc[c] = vz * by - vy * bz;
+    a.ecc[c + cs] = vx * bz - vz * bx;
+    a.ecc[c + 2 * cs] = vy * bx - vx * by;
+    if (a.compute_dt && active) {
+      const double gp = e.gamma * p;
+      const double aa = (gp + bsq) * id;
+      auto cf = [&](double b) {
+        const double disc = aa * aa - 4.0 * gp * b * b * id * id;
+        return sqrt(0.5 * (aa + sqrt(fmax(disc, 0.0))));
+      };
+      double dt = g.dx1 / (fabs(vx) + cf(bx));
+      if (g.two_d) dt = fmin(dt, g.dx2 / (fabs(vy) + cf(by)));
+      if (g.three_d) dt = fmin(dt, g.dx3 / (fabs(vz) + cf(bz)));
+      mdt = dt;
+    }
+  }
+  if (!a.compute_dt && !a.flag_mode) return;
+  __shared__ double smin[BX * BY / 64];
+  __shared__ int ssum[BX * BY / 64];
+  const int lane = int(threadIdx.x) & 63, wid = (int(threadIdx.y) * BX + int(threadIdx.x)) >> 6;
+  const double wm = wave_min(mdt);
+  const int ws = wave_sum(bad_count);
+  if (lane == 0) { smin[wid] = wm; ssum[wid] = ws; }
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    double m = smin[0];
+    int s = ssum[0];
+    for (int q = 1; q < BX * BY / 64; ++q) { m = fmin(m, smin[q]); s += ssum[q]; }
+    if (a.compute_dt && m < 1e300) atomicMin(&a.ctrl->mdt_bits[a.slot], (unsigned long long)__double_as_longlong(m));
+    if (a.flag_mode && s) atomicAdd(&a.ctrl->nbad, s);
+  }
+}
+
+// standalone FlagBadCells over the active cells (fofc.hpp:17-44); count into ctrl->nbad
+__global__ void __launch_bounds__(BX * BY) k_flag(const C2pArgs a) {
+  if (!gate(a.ctrl, 0)) return;
+  const Geom &g = a.g;
+  const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int k = g.ks + int(blockIdx.z);
+  int bad_count = 0;
+  if (i <= g.ie && j <= g.je) {
+    const Eos e = a.eos;
+    const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
+    const double bx = 0.5 * (a.b1[c] + a.b1[c + 1]);
+    const double by = g.two_d ? 0.5 * (a.b2[c] + a.b2[c + sj]) : a.b2[c];
+    const double bz = g.three_d ? 0.5 * (a.b3[c] + a.b3[c + sk]) : a.b3[c];
+    const double *u = a.u + c;
+    const double d = u[IDN * cs];
+    bool bad = d <= e.dfloor;
+    if (!bad) {
+      const double kin = 0.5 * (sq(u[IM1 * cs]) + sq(u[IM2 * cs]) + sq(u[IM3 * cs])) / d;
+      const double ei = u[IEN * cs] - kin - 0.5 * (bx * bx + by * by + bz * bz);
+      double p = e.gm1 * ei;
+      if (e.dual_energy && ei < e.de_eta * fabs(u[IEN * cs])) p = (u[IENT * cs] / d) * pow(d, e.gamma);
+      bad = p <= e.pfloor;
+    }
+    a.flag[c] = bad ? 1 : 0;
+    bad_count = bad;
+  }
+  const int ws = wave_sum(bad_count);
+  if ((int(threadIdx.x) & 63) == 0 && ws) atomicAdd(&a.ctrl->nbad, ws);
+}
+
+// ---------------------------------------------------------------- control
+__device__ __forceinline__ unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ double from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// main.cpp:246-248
+__global__ void k_begin_step(Ctrl *c) {
+  if (!(c->t < c->tlim && (c->nlim < 0 || c->cycle < c->nlim))) { c->active = 0; return; }
+  c->active = 1;
+  double dt = c->cfl * c->min_dt;
+  if (c->t + dt > c->tlim) dt = c->tlim - c->t;
+  c->dt = dt;
+}
+__global__ void k_begin_stage(Ctrl *c, int reset_slots) {
+  c->nbad = 0;
+  if (reset_slots) { c->mdt_bits[0] = bits_of(1e300); c->mdt_bits[1] = bits_of(1e300); }
+}
+// the CFL minimum of the final (possibly fofc-repaired) con2prim of the last stage
+// (fused order: slot 1 holds the minimum of the repaired state when cells were flagged)
+__global__ void k_end_stage(Ctrl *c, int last, int fused) {
+  if (!c->active) return;
+  c->fofc_total += c->nbad;
+  c->nbad_last = c->nbad;
+  if (last) c->min_dt = fmin(from_bits(c->mdt_bits[(fused && c->nbad) ? 1 : 0]), 1e300);
+}
+// main.cpp:291-292
+__global__ void k_end_step(Ctrl *c) {
+  if (!c->active) return;
+  c->t += c->dt;
+  c->cycle += 1;
+}
+
+}  // namespace
+
+hipError_t launch_corner(const CornerArgs &a, hipStream_t s) {
+  const Geom &g = a.g;
+  dim3 block(BX, BY, 1);
+  if (g.three_d) {
+    dim3 grid(div_up(g.ie - g.is + 2, BX), div_up(g.je - g.js + 2, BY), g.ke - g.ks + 2);
+    hipLaunchKernelGGL(k_corner3, grid, block, 0, s, a);
+  } else {
+    dim3 grid(div_up(g.ie - g.is + 2, BX), div_up(g.je - g.js + 2, BY), 1);
+    hipLaunchKernelGGL(k_corner12, grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const UpdateArgs &a, hipStream_t s) {
+  const Geom &g = a.g;
+  dim3 block(BX, BY, 1);
+  dim3 grid(div_up(g.ie - g.is + 2, BX), div_up(g.je - g.js + 2, BY), g.ke - g.ks + 2);
+  hipLaunchKernelGGL(k_update, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s) {
+  const Geom &g = a.g;
+  long long n = 0;
+  if (dir == 0) n = (long long)(g.nk + 1) * (g.nj + 1) * g.ng;
+  else if (dir == 1) n = (long long)(g.nk + 1) * g.ng * (g.ni + 1);
+  else n = (long long)g.ng * (g.nj + 1) * (g.ni + 1);
+  const int nb = int((n + 255) / 256);
+  if (dir == 0) hipLaunchKernelGGL(k_bc1, dim3(nb), dim3(256), 0, s, a);
+  else if (dir == 1) hipLaunchKernelGGL(k_bc2, dim3(nb), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_bc3, dim3(nb), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_c2p(const C2pArgs &a, hipStream_t s) {
+  const Geom &g = a.g;
+  dim3 grid(div_up(g.ni, BX), div_up(g.nj, BY), g.nk), block(BX, BY, 1);
+  hipLaunchKernelGGL(k_c2p, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_flag(const C2pArgs &a, hipStream_t s) {
+  const Geom &g = a.g;
+  dim3 grid(div_up(g.ie - g.is + 1, BX), div_up(g.je - g.js + 1, BY), g.ke - g.ks + 1), block(BX, BY, 1);
+  hipLaunchKernelGGL(k_flag, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ctrl_begin_step(Ctrl *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, c);
+  return hipGetLastError();
+}
+hipError_t launch_ctrl_begin_stage(Ctrl *c, int reset_slots, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_stage, dim3(1), dim3(1), 0, s, c, reset_slots);
+  return hipGetLastError();
+}
+hipError_t launch_ctrl_end_stage(Ctrl *c, int last, int fused, hipStream_t s) {
+  hipLaunchKernelGGL(k_end_stage, dim3(1), dim3(1), 0, s, c, last, fused);
+  return hipGetLastError();
+}
+hipError_t launch_ctrl_end_step(Ctrl *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_end_step, dim3(1), dim3(1), 0, s, c);
+  return hipGetLastError();
+}
+
+}  // namespace chq

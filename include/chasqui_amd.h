@@ -1,0 +1,125 @@
+/* chasqui_amd — C ABI of the MI355X-native per-RK-stage MHD pipeline.
+ *
+ * Plain C types only (no torch, no Kokkos, no C++ types cross this boundary). Every call
+ * returns 0 on success or a negative CHQ_E* code; chq_last_error() gives the message of the
+ * last failure on the calling thread. Nothing throws across the ABI.
+ *
+ * A context (chq_ctx) owns one mesh: one HIP stream and one device arena holding every
+ * field of the reference's Mesh (mesh.hpp:13-63), in a padded device layout. Host buffers
+ * passed to chq_upload/chq_download use the reference layout (LayoutRight, [var][k][j][i],
+ * cell arrays nk*nj*ni, face/edge arrays (nk+1)*(nj+1)*(ni+1)), so they compare 1:1 with the
+ * reference's arrays. One context = one stream; a context is not thread-safe, distinct
+ * contexts may be used concurrently. All kernel entry points are asynchronous on the
+ * context's stream unless they return a host value.
+ *
+ * Buffer sets: `cur`/`out`/`buf` = 0 selects u0 and b?f0, 1 selects u1 and b?f1 (the
+ * reference's ping-pong pair, main.cpp:48-50, 69-74).
+ */
+#ifndef CHASQUI_AMD_H
+#define CHASQUI_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHQ_OK 0
+#define CHQ_EINVAL -1   /* bad argument or configuration */
+#define CHQ_EHIP -2     /* HIP runtime error */
+#define CHQ_ENOMEM -3   /* device allocation failed */
+#define CHQ_EUNSUP -4   /* option outside what this build implements */
+
+typedef struct chq_ctx chq_ctx;
+
+/* Mesh description: Mesh::nx1..3, ng, x?min/x?max (mesh.hpp:14-20; main.cpp:165-175). */
+typedef struct {
+  int nx1, nx2, nx3, ng;
+  double x1min, x1max, x2min, x2max, x3min, x3max;
+} chq_mesh_desc;
+
+/* Physics configuration: Config (physics.hpp:22-33) + the driver switches of main.cpp:112-163.
+ * recon: 0 dc, 1 plm, 2 ppm, 3 wenoz, 4 ppmx (defs.hpp:35). rsolver: 0 hlle, 1 hlld,
+ * 2 lhlld (physics.hpp:20). nstage: 1..4 = rk1, rk2, rk3, rk4/ssprk43 (defs.hpp:36).
+ * bc[d]: 0 periodic, 1 outflow, 2 reflect (defs.hpp:37). */
+typedef struct {
+  int recon, rsolver, nstage;
+  double gamma, dfloor, pfloor, cfl;
+  int flatten, low_mach, fofc, dual_energy;
+  double de_eta, lhlld_phi;
+  int bc[3];
+} chq_config;
+
+/* Array identifiers for chq_upload / chq_download / chq_array_count. */
+enum chq_array {
+  CHQ_U0 = 0, CHQ_U1, CHQ_W, CHQ_BCC, CHQ_ECC,
+  CHQ_B1F0, CHQ_B2F0, CHQ_B3F0, CHQ_B1F1, CHQ_B2F1, CHQ_B3F1,
+  CHQ_F1, CHQ_F2, CHQ_F3,
+  CHQ_E2X1, CHQ_E3X1, CHQ_E1X2, CHQ_E3X2, CHQ_E1X3, CHQ_E2X3,
+  CHQ_E1, CHQ_E2, CHQ_E3,
+  CHQ_FLAG, /* int32 */
+  CHQ_NARRAYS
+};
+
+/* Time-loop state kept on the device (read with chq_get_status). */
+typedef struct {
+  double t, dt, tlim, min_dt;
+  int cycle, nlim, active, nbad_last;
+  long long fofc_total;
+} chq_status;
+
+const char *chq_last_error(void);
+const char *chq_version(void);
+
+/* Mesh::Allocate (mesh.hpp:37-63) + Config; selects the current device. */
+int chq_create(const chq_mesh_desc *mesh, const chq_config *cfg, chq_ctx **out);
+int chq_destroy(chq_ctx *ctx);
+/* geometry: iv[16] = nx1 nx2 nx3 ng ni nj nk is ie js je ks ke two_d three_d pitch */
+int chq_geometry(const chq_ctx *ctx, int *iv, double *dx3);
+/* bytes of device memory held by the context */
+size_t chq_device_bytes(const chq_ctx *ctx);
+
+/* Host <-> device copies in the reference layout (pgen::Initialize's deep_copy,
+ * pgen.hpp:302-306; outputs, output.hpp:33-34; restart, restart.hpp:146-149). `count` is
+ * the number of elements of the host buffer and must equal chq_array_count(). */
+size_t chq_array_count(const chq_ctx *ctx, int array);
+int chq_upload(chq_ctx *ctx, int array, const void *host, size_t count);
+int chq_download(chq_ctx *ctx, int array, void *host, size_t count);
+
+/* ---- stage kernels with the reference's semantics (all six conserved variables) ---- */
+/* kernels::FluxesDispatch<DIR> (physics.hpp:505-514): writes f{dir+1} and its two face
+ * EMFs from w/bcc and the face-normal field of buffer set `cur`. */
+int chq_fluxes(chq_ctx *ctx, int dir, int cur);
+/* kernels::CornerEMF (physics.hpp:242-297): writes e1, e2, e3. */
+int chq_corner_emf(chq_ctx *ctx);
+/* kernels::Update (physics.hpp:307-348) with ua = u0/b?f0, ub = u1/b?f1, out = set `out`. */
+int chq_update(chq_ctx *ctx, int out, double g0, double g1, double beta_dt);
+/* kernels::Boundaries (physics.hpp:354-432) on buffer set `buf`. */
+int chq_boundaries(chq_ctx *ctx, int buf);
+/* kernels::ConsToPrim (physics.hpp:437-494) on set `buf`; when compute_dt, *min_dt receives
+ * the CFL minimum (synchronises); min_dt may be NULL otherwise. */
+int chq_cons_to_prim(chq_ctx *ctx, int buf, int compute_dt, double *min_dt);
+/* fofc::FlagBadCells (fofc.hpp:17-44): writes flag, *count receives the number (synchronises). */
+int chq_fofc_flag(chq_ctx *ctx, int buf, int *count);
+/* fofc::RepairFluxes<DIR> (fofc.hpp:46-96) from w/bcc and face B of set `cur`. */
+int chq_fofc_repair(chq_ctx *ctx, int dir, int cur);
+/* Stage() (main.cpp:46-93): fluxes, corner EMF, update, fofc, boundaries; *nbad (may be
+ * NULL) receives the fofc count (synchronises when non-NULL). */
+int chq_stage(chq_ctx *ctx, int cur, int out, double g0, double g1, double beta_dt, long *nbad);
+
+/* ---- the time loop with dt, t and the fofc count resident on the device ---- */
+/* main.cpp:192-201: boundaries(u0) + cons_to_prim(u0, dt) and sets t, tlim, nlim. */
+int chq_init(chq_ctx *ctx, double t, double tlim, int nlim);
+/* enqueue `nsteps` RK steps (main.cpp:246-292); asynchronous, no host round trip. Steps
+ * after t reaches tlim (or cycle reaches nlim) are no-ops on the device. */
+int chq_step(chq_ctx *ctx, int nsteps);
+/* enqueue nsteps and time them with HIP events on the context stream; *ms = elapsed */
+int chq_step_timed(chq_ctx *ctx, int nsteps, double *ms);
+int chq_get_status(chq_ctx *ctx, chq_status *st);
+int chq_sync(chq_ctx *ctx);
+void *chq_stream(chq_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
