@@ -1,0 +1,169 @@
+#!/usr/bin/env python
+"""Headline benchmark: zone-updates/s of the 3D MHD blast at 256^3 in FP64 on one MI355X.
+
+Workload: 256^3 periodic cube [-0.5, 0.5]^3, gamma = 5/3, rho = 1 with seeded 1e-3
+multiplicative noise, p = 0.1 ambient / 10 for r < 0.1, B = (1/sqrt2, 1/sqrt2, 0);
+PPM + HLLD + RK3 + CT + fofc + flattening, CFL 0.3 (synthetic seeded initial state from the
+library's own problem generator). A step is one full RK3 step (three stages) of the
+device-resident pipeline; the timed region is K steps between two events on the
+pipeline's stream, after W warm-up steps, bracketed by stream synchronisation.
+
+Output: one JSON line. `roofline` is for the dominant kernel (the face-flux kernel),
+timed with HIP events on its stream inside the timed region; `step_roofline` applies the
+reference's traffic model (1040 B per cell per RK stage, README.md:45-47) to the whole step;
+`cpu_baseline` times the CPU oracle on this host's cores on one step of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "zone-updates/s at 256³ FP64 & % HBM roofline; served runs/s + p99 latency"
+HBM_PEAK = 8.0e12                 # MI355X HBM3E, bytes/s
+MODEL_BYTES_PER_ZONE_STAGE = 1040  # README.md:45-47 (130 doubles per cell per RK stage)
+# algorithmic bytes per face of the FP64 flux kernel with the entropy slot not transported:
+# reads rho, v(3), p, B_t1, B_t2 (stencil elements counted once) and the face-normal B,
+# writes 5 fluxes and 2 face EMFs
+FLUX_BYTES_PER_FACE = (5 + 2 + 1 + 5 + 2) * 8
+
+# HBM bytes per flux-kernel launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (profiles/flux_blast256_pmc.md), scaled by the copy-kernel calibration factor stated there;
+# None until measured for the current kernel.
+FLUX_TRAFFIC_PER_LAUNCH = None
+
+DECK = """
+[mesh]
+nx1 = 256
+nx2 = 256
+nx3 = 256
+x1min = -0.5
+x1max = 0.5
+x2min = -0.5
+x2max = 0.5
+x3min = -0.5
+x3max = 0.5
+[time]
+integrator = rk3
+cfl = 0.3
+tlim = 1.0e9
+nlim = -1
+ndiag = 0
+[mhd]
+gamma = 1.6666666666666667
+reconstruct = ppm
+rsolver = hlld
+flatten = true
+fofc = true
+[problem]
+name = blast
+pres_in = 10.0
+pres_out = 0.1
+radius = 0.1
+b0 = 1.0
+angle = 45.0
+perturb = 1.0e-3
+seed = 42
+"""
+
+
+def faces_per_flux_launch(n):
+    return (n + 1) * (n + 2) * (n + 2)
+
+
+def cpu_baseline(n, steps):
+    """The CPU oracle (plain C++/OpenMP restatement of the reference) on this host."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    po.set_threads(threads)
+    o = po.OracleSim(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}"])
+    t0 = time.perf_counter()
+    taken = o.step(steps)
+    wall = time.perf_counter() - t0
+    o.close()
+    return {"value": n ** 3 * taken / wall, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{taken} RK3 step(s) of the same {n}^3 blast on {threads} OpenMP threads, "
+                      f"{wall:.1f} s wall (oracle/, -O3 -ffp-contract=off)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="cells per side (256 = the headline workload)")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one mesh per device; "
+              "sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from chasqui_amd.deck import Deck
+    n = args.n
+    deck = Deck(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}"])
+    mesh = deck.make_mesh()
+    deck.close()
+    mesh.step(args.warmup)
+    mesh.sync()
+    mesh.kernel_timing("flux")
+    ms = mesh.step_timed(args.steps)
+    flux_ms, flux_launches = mesh.kernel_timing_read()
+    mesh.kernel_timing(None)
+    st = mesh.status()
+
+    # per-kernel breakdown on two further (untimed) steps
+    breakdown = {}
+    for cls in ("flux", "corner", "update", "bc", "c2p", "repair"):
+        mesh.kernel_timing(cls)
+        mesh.step(2)
+        kms, kn = mesh.kernel_timing_read()
+        breakdown[cls] = {"ms_per_step": kms / 2, "launches_per_step": kn // 2}
+    mesh.kernel_timing(None)
+    mesh.close()
+
+    zone_updates = float(n) ** 3 * args.steps
+    value = zone_updates / (ms * 1e-3)
+    avg_flux_s = flux_ms * 1e-3 / max(flux_launches, 1)
+    flux_bytes = FLUX_BYTES_PER_FACE * faces_per_flux_launch(n)
+    achieved = flux_bytes / avg_flux_s
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK,
+                "traffic": FLUX_TRAFFIC_PER_LAUNCH,
+                "kernel": "k_flux (face flux, PPM + flattening + HLLD, one direction per launch)",
+                "bytes_per_launch": flux_bytes, "avg_launch_ms": avg_flux_s * 1e3, "launches": flux_launches}
+    model = value * 3 * MODEL_BYTES_PER_ZONE_STAGE
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded blast initial state from the library's problem generator)",
+        "config": {"workload": f"mhd_blast_3d_{n}^3_ppm_hlld_rk3_ct_fofc_flatten_cfl0.3",
+                   "cells": [n, n, n], "recon": "ppm", "rsolver": "hlld", "integrator": "rk3",
+                   "fofc": True, "flatten": True, "cfl": 0.3, "precision": "fp64",
+                   "fofc_corrections_in_run": st["fofc_total"], "cycles_run": st["cycle"]},
+        "roofline": roofline,
+        "step_roofline": {"model": "README.md:45-47, 1040 B/cell/RK stage", "achieved": model / 1e9,
+                          "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": model / HBM_PEAK},
+        "kernels": breakdown,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -23,6 +23,7 @@ RECON = {"dc": 0, "plm": 1, "ppm": 2, "wenoz": 3, "ppmx": 4}
 RSOLVER = {"hlle": 0, "hlld": 1, "lhlld": 2}
 INTEGRATOR = {"rk1": 1, "rk2": 2, "rk3": 3, "rk4": 4, "ssprk43": 4}
 BC = {"periodic": 0, "outflow": 1, "reflect": 2}
+KERNEL_CLASSES = {"flux": 0, "corner": 1, "update": 2, "bc": 3, "c2p": 4, "repair": 5}
 
 
 class MeshDesc(C.Structure):
@@ -88,6 +89,20 @@ def lib():
         "chq_get_status": (C.c_int, [vp, C.POINTER(Status)]),
         "chq_sync": (C.c_int, [vp]),
         "chq_stream": (vp, [vp]),
+        "chq_kernel_timing": (C.c_int, [vp, C.c_int]),
+        "chq_kernel_timing_read": (C.c_int, [vp, dp, ip]),
+        "chq_deck_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+        "chq_deck_close": (C.c_int, [vp]),
+        "chq_deck_last_error": (C.c_char_p, []),
+        "chq_deck_mesh": (C.c_int, [vp, C.POINTER(MeshDesc)]),
+        "chq_deck_config": (C.c_int, [vp, C.POINTER(Config)]),
+        "chq_deck_run": (C.c_int, [vp, dp, ip, ip]),
+        "chq_deck_get": (C.c_char_p, [vp, C.c_char_p, C.c_char_p]),
+        "chq_deck_initial": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "chq_deck_print": (C.c_size_t, [vp, C.c_char_p, C.c_size_t]),
+        "chq_deck_upload": (C.c_int, [vp, vp]),
+        "chq_write_vtk": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_char_p, C.c_double, C.c_int]),
+        "chq_history": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_int, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -103,7 +118,10 @@ def exported_symbols():
         "chq_device_bytes", "chq_array_count", "chq_upload", "chq_download", "chq_fluxes",
         "chq_corner_emf", "chq_update", "chq_boundaries", "chq_cons_to_prim", "chq_fofc_flag",
         "chq_fofc_repair", "chq_stage", "chq_init", "chq_step", "chq_step_timed",
-        "chq_get_status", "chq_sync", "chq_stream",
+        "chq_get_status", "chq_sync", "chq_stream", "chq_kernel_timing", "chq_kernel_timing_read",
+        "chq_deck_open", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
+        "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",
+        "chq_write_vtk", "chq_history",
     ]
 
 

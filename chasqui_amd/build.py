@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 HIP_SOURCES = ["flux.hip", "stage.hip", "layout.hip"]
-HOST_SOURCES = ["capi.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp", "host/sim.cpp"]
+HOST_SOURCES = ["capi.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]
 
 
 def _compile(src):

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/chasqui_amd.h"
 #include "kernels.hpp"
@@ -66,6 +67,21 @@ struct chq_ctx {
   size_t staging_bytes = 0;
   int wcur = 0;              // which primitive set (w, bcc, ecc) is current
   bool entropy = false;      // transport IENT in the time loop (only needed with dual energy)
+  // per-kernel-class timing: event pairs recorded around launches of one class
+  int timing_class = -1;
+  std::vector<hipEvent_t> tev;   // begin/end pairs
+  size_t tev_used = 0;
+
+  void tmark(int cls, bool begin) {
+    if (cls != timing_class) return;
+    if (tev_used == tev.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      tev.push_back(e);
+    }
+    if (!begin && (tev_used % 2) == 0) return;
+    hipEventRecord(tev[tev_used++], stream);
+  }
 
   // face-flux launch description for direction d from buffer set `cur`
   FluxArgs flux_args(int d, int cur, int wset, const Ctrl *c) const {
@@ -312,6 +328,7 @@ int chq_destroy(chq_ctx *c) {
   if (c->staging) hipFree(c->staging);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->tev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return CHQ_OK;
@@ -489,22 +506,33 @@ static int enqueue_step(chq_ctx *c) {
     for (int d = 0; d < c->ndir(); ++d) {
       FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
       fa.entropy = c->entropy;
+      c->tmark(CHQ_K_FLUX, true);
       HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
+      c->tmark(CHQ_K_FLUX, false);
     }
+    c->tmark(CHQ_K_CORNER, true);
     HIPCHK(launch_corner(c->corner_args(wc, k, 0), c->stream));
+    c->tmark(CHQ_K_CORNER, false);
     UpdateArgs ua = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 0);
     ua.use_ctrl_dt = 1;
     ua.beta = cf.beta;
+    c->tmark(CHQ_K_UPDATE, true);
     HIPCHK(launch_update(ua, c->stream));
+    c->tmark(CHQ_K_UPDATE, false);
     if (fused) {
+      c->tmark(CHQ_K_BC, true);
       HIPCHK(c->boundaries(cf.out, nvar, k, 0));
+      c->tmark(CHQ_K_BC, false);
       C2pArgs ca = c->c2p_args(cf.out, wn, k);
       ca.write_went = c->entropy;
       ca.compute_dt = last;
       ca.slot = 0;
       ca.flag_mode = c->cfg.fofc ? 1 : 0;
+      c->tmark(CHQ_K_C2P, true);
       HIPCHK(launch_c2p(ca, c->stream));
+      c->tmark(CHQ_K_C2P, false);
       if (c->cfg.fofc) {
+        c->tmark(CHQ_K_REPAIR, true);
         for (int d = 0; d < c->ndir(); ++d) {
           FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
           fa.entropy = c->entropy;
@@ -521,6 +549,7 @@ static int enqueue_step(chq_ctx *c) {
         cb.slot = 1;
         cb.only_if_bad = 1;
         HIPCHK(launch_c2p(cb, c->stream));
+        c->tmark(CHQ_K_REPAIR, false);
       }
     } else {
       if (c->cfg.fofc) {
@@ -564,6 +593,31 @@ int chq_step_timed(chq_ctx *c, int nsteps, double *ms) {
   float f = 0.0f;
   HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
   *ms = f;
+  return CHQ_OK;
+}
+
+int chq_kernel_timing(chq_ctx *c, int kernel_class) {
+  if (!c || kernel_class < -1 || kernel_class >= CHQ_K_NCLASS) return fail(CHQ_EINVAL, "bad kernel class");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->timing_class = kernel_class;
+  c->tev_used = 0;
+  return CHQ_OK;
+}
+
+int chq_kernel_timing_read(chq_ctx *c, double *total_ms, int *launches) {
+  if (!c || !total_ms || !launches) return fail(CHQ_EINVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  int n = 0;
+  for (size_t q = 0; q + 1 < c->tev_used; q += 2) {
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, c->tev[q], c->tev[q + 1]));
+    tot += f;
+    ++n;
+  }
+  *total_ms = tot;
+  *launches = n;
+  c->tev_used = 0;
   return CHQ_OK;
 }
 

@@ -1,0 +1,150 @@
+// Outputs: legacy-VTK dumps in the PLUTO layout (output.hpp:14-85) and the history sums of
+// the .hst file (output.hpp:87-119), from state downloaded off the device.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../../include/chasqui_amd.h"
+
+namespace {
+
+void put_be_floats(std::ofstream &f, const std::vector<float> &v) {
+  std::vector<unsigned char> b(v.size() * 4);
+  for (size_t n = 0; n < v.size(); ++n) {
+    uint32_t u;
+    std::memcpy(&u, &v[n], 4);
+    b[4 * n] = (u >> 24) & 0xff;
+    b[4 * n + 1] = (u >> 16) & 0xff;
+    b[4 * n + 2] = (u >> 8) & 0xff;
+    b[4 * n + 3] = u & 0xff;
+  }
+  f.write(reinterpret_cast<const char *>(b.data()), std::streamsize(b.size()));
+}
+
+void put_be_int(std::ofstream &f, int32_t v) {
+  const unsigned char b[4] = {static_cast<unsigned char>((v >> 24) & 0xff), static_cast<unsigned char>((v >> 16) & 0xff),
+                              static_cast<unsigned char>((v >> 8) & 0xff), static_cast<unsigned char>(v & 0xff)};
+  f.write(reinterpret_cast<const char *>(b), 4);
+}
+
+struct Geo {
+  int nx1, nx2, nx3, ni, nj, nk, is, js, ks, two_d, three_d;
+};
+
+Geo geo_of(chq_ctx *ctx) {
+  int iv[16];
+  chq_geometry(ctx, iv, nullptr);
+  return Geo{iv[0], iv[1], iv[2], iv[4], iv[5], iv[6], iv[7], iv[9], iv[11], iv[13], iv[14]};
+}
+
+}  // namespace
+
+extern "C" {
+
+// WriteVTK (output.hpp:32-85): BINARY RECTILINEAR_GRID, big-endian float32 cell data
+// rho prs vx1 vx2 vx3 Bx1 Bx2 Bx3 over the active cells, i fastest.
+int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, double time, int cycle) {
+  if (!ctx || !mesh || !path) return CHQ_EINVAL;
+  const Geo g = geo_of(ctx);
+  std::vector<double> w(chq_array_count(ctx, CHQ_W)), b(chq_array_count(ctx, CHQ_BCC));
+  int rc = chq_download(ctx, CHQ_W, w.data(), w.size());
+  if (!rc) rc = chq_download(ctx, CHQ_BCC, b.data(), b.size());
+  if (rc) return rc;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return CHQ_EINVAL;
+  const double dx1 = (mesh->x1max - mesh->x1min) / g.nx1, dx2 = (mesh->x2max - mesh->x2min) / g.nx2,
+               dx3 = (mesh->x3max - mesh->x3min) / g.nx3;
+  f << "# vtk DataFile Version 2.0\n" << "Chasqui MHD VTK Data\n" << "BINARY\n" << "DATASET RECTILINEAR_GRID\n";
+  f << "FIELD FieldData 2\n" << "TIME 1 1 float\n";
+  put_be_floats(f, {static_cast<float>(time)});
+  f << "\nCYCLE 1 1 int\n";
+  put_be_int(f, cycle);
+  f << "\n";
+  const int n2 = g.two_d ? g.nx2 + 1 : 1, n3 = g.three_d ? g.nx3 + 1 : 1;
+  f << "DIMENSIONS " << g.nx1 + 1 << " " << n2 << " " << n3 << "\n";
+  std::vector<float> x(g.nx1 + 1), y(n2), z(n3);
+  for (int i = 0; i <= g.nx1; ++i) x[i] = static_cast<float>(mesh->x1min + i * dx1);
+  for (int j = 0; j < n2; ++j) y[j] = static_cast<float>(g.two_d ? mesh->x2min + j * dx2 : mesh->x2min);
+  for (int k = 0; k < n3; ++k) z[k] = static_cast<float>(g.three_d ? mesh->x3min + k * dx3 : mesh->x3min);
+  f << "X_COORDINATES " << g.nx1 + 1 << " float\n";
+  put_be_floats(f, x);
+  f << "\nY_COORDINATES " << n2 << " float\n";
+  put_be_floats(f, y);
+  f << "\nZ_COORDINATES " << n3 << " float\n";
+  put_be_floats(f, z);
+  f << "\n";
+  const size_t ncells = size_t(g.nx1) * g.nx2 * g.nx3, nc = size_t(g.ni) * g.nj * g.nk;
+  f << "CELL_DATA " << ncells << "\n";
+  std::vector<float> buf(ncells);
+  auto dump = [&](const char *name, const std::vector<double> &a, int comp) {
+    size_t n = 0;
+    for (int k = 0; k < (g.three_d ? g.nx3 : 1); ++k)
+      for (int j = 0; j < (g.two_d ? g.nx2 : 1); ++j)
+        for (int i = 0; i < g.nx1; ++i)
+          buf[n++] = static_cast<float>(a[comp * nc + (size_t(g.ks + k) * g.nj + (g.js + j)) * g.ni + (g.is + i)]);
+    f << "SCALARS " << name << " float\nLOOKUP_TABLE default\n";
+    put_be_floats(f, buf);
+    f << "\n";
+  };
+  dump("rho", w, 0);
+  dump("prs", w, 4);
+  dump("vx1", w, 1);
+  dump("vx2", w, 2);
+  dump("vx3", w, 3);
+  dump("Bx1", b, 0);
+  dump("Bx2", b, 1);
+  dump("Bx3", b, 2);
+  return f ? CHQ_OK : CHQ_EINVAL;
+}
+
+// History (output.hpp:91-119): mass, mom1..3, etot, ekin, emag, eint (volume integrals over
+// the active cells), max |div B| and its mean, from u and face B of set `buf` and the
+// current primitives.
+int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]) {
+  if (!ctx || !mesh || !out || (buf & ~1)) return CHQ_EINVAL;
+  const Geo g = geo_of(ctx);
+  std::vector<double> u(chq_array_count(ctx, CHQ_U0)), w(chq_array_count(ctx, CHQ_W)),
+      bc(chq_array_count(ctx, CHQ_BCC)), b1(chq_array_count(ctx, CHQ_B1F0)), b2(b1.size()), b3(b1.size());
+  int rc = chq_download(ctx, buf ? CHQ_U1 : CHQ_U0, u.data(), u.size());
+  if (!rc) rc = chq_download(ctx, CHQ_W, w.data(), w.size());
+  if (!rc) rc = chq_download(ctx, CHQ_BCC, bc.data(), bc.size());
+  if (!rc) rc = chq_download(ctx, buf ? CHQ_B1F1 : CHQ_B1F0, b1.data(), b1.size());
+  if (!rc) rc = chq_download(ctx, buf ? CHQ_B2F1 : CHQ_B2F0, b2.data(), b2.size());
+  if (!rc) rc = chq_download(ctx, buf ? CHQ_B3F1 : CHQ_B3F0, b3.data(), b3.size());
+  if (rc) return rc;
+  const double dx1 = (mesh->x1max - mesh->x1min) / g.nx1, dx2 = (mesh->x2max - mesh->x2min) / g.nx2,
+               dx3 = (mesh->x3max - mesh->x3min) / g.nx3;
+  const double idx1 = 1.0 / dx1, idx2 = 1.0 / dx2, idx3 = 1.0 / dx3;
+  const size_t nc = size_t(g.ni) * g.nj * g.nk;
+  auto C = [&](int k, int j, int i) { return (size_t(k) * g.nj + j) * g.ni + i; };
+  auto F = [&](int k, int j, int i) { return (size_t(k) * (g.nj + 1) + j) * (g.ni + 1) + i; };
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, dbmax = 0.0, dbl1 = 0.0;
+  const int ke = g.three_d ? g.ks + g.nx3 - 1 : 0, je = g.two_d ? g.js + g.nx2 - 1 : 0, ie = g.is + g.nx1 - 1;
+  for (int k = g.ks; k <= ke; ++k)
+    for (int j = g.js; j <= je; ++j)
+      for (int i = g.is; i <= ie; ++i) {
+        const size_t c = C(k, j, i);
+        for (int n = 0; n < 5; ++n) s[n] += u[n * nc + c];
+        const double d = w[c];
+        const double v2 = w[nc + c] * w[nc + c] + w[2 * nc + c] * w[2 * nc + c] + w[3 * nc + c] * w[3 * nc + c];
+        const double bb = bc[c] * bc[c] + bc[nc + c] * bc[nc + c] + bc[2 * nc + c] * bc[2 * nc + c];
+        s[5] += 0.5 * d * v2;
+        s[6] += 0.5 * bb;
+        s[7] += u[4 * nc + c] - 0.5 * d * v2 - 0.5 * bb;
+        double db = (b1[F(k, j, i + 1)] - b1[F(k, j, i)]) * idx1;
+        if (g.two_d) db += (b2[F(k, j + 1, i)] - b2[F(k, j, i)]) * idx2;
+        if (g.three_d) db += (b3[F(k + 1, j, i)] - b3[F(k, j, i)]) * idx3;
+        dbmax = std::fmax(dbmax, std::fabs(db));
+        dbl1 += std::fabs(db);
+      }
+  const double vol = dx1 * (g.two_d ? dx2 : 1.0) * (g.three_d ? dx3 : 1.0);
+  for (int n = 0; n < 8; ++n) out[n] = s[n] * vol;
+  out[8] = dbmax;
+  out[9] = dbl1 / (double(g.nx1) * g.nx2 * g.nx3);
+  return CHQ_OK;
+}
+
+}  // extern "C"

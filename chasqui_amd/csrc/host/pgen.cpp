@@ -1,0 +1,428 @@
+// Run configuration and problem generators (host side), and the chq_deck_* C ABI.
+//
+// Problems: cpaw, orszag_tang, blast, shock_tube (pgen.hpp:54-184), with the keys the
+// benchmark decks need and the reference lacks: cpaw problem.angle2 (propagation direction
+// out of the x-y plane), orszag_tang problem.pres / problem.b0, blast problem.perturb /
+// problem.seed (counter-based density noise). Absent keys reproduce the reference bitwise.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <functional>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+
+#include "deck.hpp"
+
+namespace chq {
+
+constexpr double kPi = 3.14159265358979323846;
+
+void HostGrid::build() {
+  ni = nx1 + 2 * ng;
+  nj = (nx2 > 1) ? nx2 + 2 * ng : 1;
+  nk = (nx3 > 1) ? nx3 + 2 * ng : 1;
+  is = ng;
+  ie = ng + nx1 - 1;
+  js = (nx2 > 1) ? ng : 0;
+  je = (nx2 > 1) ? ng + nx2 - 1 : 0;
+  ks = (nx3 > 1) ? ng : 0;
+  ke = (nx3 > 1) ? ng + nx3 - 1 : 0;
+  two_d = nx2 > 1;
+  three_d = nx3 > 1;
+  dx1 = (x1max - x1min) / nx1;
+  dx2 = (x2max - x2min) / nx2;
+  dx3 = (x3max - x3min) / nx3;
+}
+
+namespace {
+
+int parse_bc(const std::string &s) {
+  if (s == "periodic") return 0;
+  if (s == "outflow") return 1;
+  if (s == "reflect") return 2;
+  throw std::runtime_error("unknown boundary condition: " + s);
+}
+
+struct Fields {
+  const HostGrid &g;
+  std::vector<double> &w, &b1, &b2, &b3;
+  double &W(int n, int k, int j, int i) { return w[size_t(n) * g.ncell() + g.c(k, j, i)]; }
+};
+
+// B = curl A from an edge-sampled potential (pgen.hpp:30-50)
+void curl_potential(Fields &h, const std::function<double(int, double, double, double)> &A) {
+  const HostGrid &m = h.g;
+  const double idx = 1.0 / m.dx1, idy = 1.0 / m.dx2, idz = 1.0 / m.dx3;
+  for (int k = 0; k < m.nk + 1; ++k)
+    for (int j = 0; j < m.nj + 1; ++j)
+      for (int i = 0; i < m.ni + 1; ++i) {
+        const double xf = m.x1f(i), yf = m.x2f(j), zf = m.x3f(k);
+        const double xv = m.x1v(i), yv = m.x2v(j), zv = m.x3v(k);
+        double bx = 0.0;
+        if (m.two_d) bx += (A(2, xf, m.x2f(j + 1), zv) - A(2, xf, yf, zv)) * idy;
+        if (m.three_d) bx -= (A(1, xf, yv, m.x3f(k + 1)) - A(1, xf, yv, zf)) * idz;
+        double by = -(A(2, m.x1f(i + 1), yf, zv) - A(2, xf, yf, zv)) * idx;
+        if (m.three_d) by += (A(0, xv, yf, m.x3f(k + 1)) - A(0, xv, yf, zf)) * idz;
+        double bz = (A(1, m.x1f(i + 1), yv, zf) - A(1, xf, yv, zf)) * idx;
+        if (m.two_d) bz -= (A(0, xv, m.x2f(j + 1), zf) - A(0, xv, yf, zf)) * idy;
+        const size_t f = m.f(k, j, i);
+        h.b1[f] = bx;
+        h.b2[f] = by;
+        h.b3[f] = bz;
+      }
+}
+
+// circularly polarized Alfven wave, pgen.hpp:54-91 (+ angle2, 3D propagation direction)
+void cpaw(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const double dens = p.get_real("problem", "dens", 1.0);
+  const double pres = p.get_real("problem", "pres", 0.1);
+  const double b0 = p.get_real("problem", "b_par", 1.0);
+  const double amp = p.get_real("problem", "amp", 0.1);
+  const double lambda = p.get_real("problem", "lambda", 1.0);
+  const double th = p.get_real("problem", "angle", 0.0) * kPi / 180.0;
+  const double ph = p.get_real("problem", "angle2", 0.0) * kPi / 180.0;
+  const double vflow = p.get_real("problem", "vflow", 0.0);
+  const double dir = p.get_real("problem", "dir", 1.0);
+  const double kw = 2.0 * kPi / lambda;
+  const double cth = std::cos(th), sth = std::sin(th);
+  const double fac = -dir * amp / std::sqrt(dens);
+  if (ph == 0.0) {
+    curl_potential(h, [=](int c, double x, double y, double) {
+      const double xp = x * cth + y * sth, yp = -x * sth + y * cth;
+      const double a_y = (amp / kw) * std::sin(kw * xp);
+      const double a_z = b0 * yp + (amp / kw) * std::cos(kw * xp);
+      return c == 0 ? -a_y * sth : c == 1 ? a_y * cth : a_z;
+    });
+    if (!m.two_d)   // the uniform B1 of a 1D run cannot come from the potential
+      for (size_t f = 0; f < m.nface(); ++f) h.b1[f] += b0 * cth;
+    for (int k = 0; k < m.nk; ++k)
+      for (int j = 0; j < m.nj; ++j)
+        for (int i = 0; i < m.ni; ++i) {
+          const double xp = m.x1v(i) * cth + m.x2v(j) * sth;
+          const double vyp = fac * std::sin(kw * xp), vz = fac * std::cos(kw * xp);
+          h.W(0, k, j, i) = dens;
+          h.W(1, k, j, i) = vflow * cth - vyp * sth;
+          h.W(2, k, j, i) = vflow * sth + vyp * cth;
+          h.W(3, k, j, i) = vz;
+          h.W(4, k, j, i) = pres;
+        }
+    return;
+  }
+  if (!m.three_d) throw std::runtime_error("cpaw: problem.angle2 != 0 needs a 3D mesh");
+  const double cph = std::cos(ph), sph = std::sin(ph);
+  const double e1[3] = {cph * cth, cph * sth, sph};      // along the wave vector
+  const double e2[3] = {-sth, cth, 0.0};                 // in the x-y plane
+  const double e3[3] = {-sph * cth, -sph * sth, cph};    // e1 x e2
+  curl_potential(h, [=](int c, double x, double y, double z) {
+    const double xp = (x * e1[0] + y * e1[1]) + z * e1[2];
+    const double yp = x * e2[0] + y * e2[1];
+    const double a2 = (amp / kw) * std::sin(kw * xp);
+    const double a3 = b0 * yp + (amp / kw) * std::cos(kw * xp);
+    return a2 * e2[c] + a3 * e3[c];
+  });
+  for (int k = 0; k < m.nk; ++k)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        const double xp = (m.x1v(i) * e1[0] + m.x2v(j) * e1[1]) + m.x3v(k) * e1[2];
+        const double v2 = fac * std::sin(kw * xp), v3 = fac * std::cos(kw * xp);
+        h.W(0, k, j, i) = dens;
+        for (int c = 0; c < 3; ++c) h.W(1 + c, k, j, i) = (vflow * e1[c] + v2 * e2[c]) + v3 * e3[c];
+        h.W(4, k, j, i) = pres;
+      }
+}
+
+// Orszag-Tang vortex, pgen.hpp:94-111 (+ problem.pres, problem.b0)
+void orszag_tang(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const double b0 = p.get_real("problem", "b0", 1.0 / std::sqrt(4.0 * kPi));
+  const double pres = p.get_real("problem", "pres", 5.0 / (12.0 * kPi));
+  curl_potential(h, [=](int c, double x, double y, double) {
+    if (c != 2) return 0.0;
+    return b0 / (2.0 * kPi) * std::cos(2.0 * kPi * y) + b0 / (4.0 * kPi) * std::cos(4.0 * kPi * x);
+  });
+  for (int k = 0; k < m.nk; ++k)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        h.W(0, k, j, i) = 25.0 / (36.0 * kPi);
+        h.W(1, k, j, i) = -std::sin(2.0 * kPi * m.x2v(j));
+        h.W(2, k, j, i) = std::sin(2.0 * kPi * m.x1v(i));
+        h.W(3, k, j, i) = 0.0;
+        h.W(4, k, j, i) = pres;
+      }
+}
+
+// splitmix64 of (seed, all-cell index) mapped to [-1, 1): a counter-based generator, so the
+// noise does not depend on traversal order or thread count
+double cell_noise(uint64_t seed, uint64_t idx) {
+  uint64_t z = seed * 0x9E3779B97F4A7C15ull + (idx + 1) * 0xD1B54A32D192ED03ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return double(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+// MHD blast, pgen.hpp:114-135 (+ problem.perturb, problem.seed)
+void blast(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const double pout = p.get_real("problem", "pres_out", 0.1);
+  const double pin = p.get_real("problem", "pres_in", 10.0);
+  const double rad = p.get_real("problem", "radius", 0.1);
+  const double b0 = p.get_real("problem", "b0", 1.0);
+  const double ang = p.get_real("problem", "angle", 45.0) * kPi / 180.0;
+  const double pert = p.get_real("problem", "perturb", 0.0);
+  const uint64_t seed = uint64_t(p.get_int("problem", "seed", 42));
+  double bx = b0 * std::cos(ang), by = b0 * std::sin(ang);
+  if (std::fabs(bx) < 1e-14 * b0) bx = 0.0;
+  if (std::fabs(by) < 1e-14 * b0) by = 0.0;
+  for (size_t f = 0; f < m.nface(); ++f) { h.b1[f] = bx; h.b2[f] = by; h.b3[f] = 0.0; }
+  for (int k = 0; k < m.nk; ++k)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        const double x = m.x1v(i), y = m.two_d ? m.x2v(j) : 0.0, z = m.three_d ? m.x3v(k) : 0.0;
+        const double r = std::sqrt(x * x + y * y + z * z);
+        double d = 1.0;
+        if (pert != 0.0) d = 1.0 * (1.0 + pert * cell_noise(seed, m.c(k, j, i)));
+        h.W(0, k, j, i) = d;
+        h.W(1, k, j, i) = 0.0;
+        h.W(2, k, j, i) = 0.0;
+        h.W(3, k, j, i) = 0.0;
+        h.W(4, k, j, i) = (r < rad) ? pin : pout;
+      }
+}
+
+// Brio-Wu shock tube, pgen.hpp:158-184 (density noise is not provided)
+void shock_tube(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const double dl = p.get_real("problem", "dl", 1.0), pl = p.get_real("problem", "pl", 1.0);
+  const double dr = p.get_real("problem", "dr", 0.125), pr = p.get_real("problem", "pr", 0.1);
+  const double byl = p.get_real("problem", "byl", 1.0), byr = p.get_real("problem", "byr", -1.0);
+  const double bx = p.get_real("problem", "bx", 0.75);
+  const double x0 = p.get_real("problem", "x0", 0.0);
+  const double ul = p.get_real("problem", "ul", 0.0), ur = p.get_real("problem", "ur", 0.0);
+  if (p.get_real("problem", "perturb", 0.0) != 0.0) throw std::runtime_error("shock_tube: problem.perturb is not supported");
+  for (int k = 0; k < m.nk + 1; ++k)
+    for (int j = 0; j < m.nj + 1; ++j)
+      for (int i = 0; i < m.ni + 1; ++i) {
+        const size_t f = m.f(k, j, i);
+        h.b1[f] = bx;
+        h.b2[f] = (m.x1v(i) < x0) ? byl : byr;
+        h.b3[f] = 0.0;
+      }
+  for (int k = 0; k < m.nk; ++k)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        const bool left = m.x1v(i) < x0;
+        h.W(0, k, j, i) = (left ? dl : dr) * (1.0 + 0.0);
+        h.W(1, k, j, i) = left ? ul : ur;
+        h.W(2, k, j, i) = 0.0;
+        h.W(3, k, j, i) = 0.0;
+        h.W(4, k, j, i) = left ? pl : pr;
+      }
+}
+
+}  // namespace
+
+void Deck::configure() {
+  const std::string r = p.get_string("mhd", "reconstruct", "ppm");
+  cfg.recon = r == "dc" ? 0 : r == "plm" ? 1 : r == "ppm" ? 2 : r == "wenoz" ? 3 : r == "ppmx" ? 4 : -1;
+  if (cfg.recon < 0) throw std::runtime_error("unknown reconstruct: " + r);
+  const std::string s = p.get_string("mhd", "rsolver", "hlld");
+  cfg.rsolver = s == "hlle" ? 0 : s == "hlld" ? 1 : s == "lhlld" ? 2 : -1;
+  if (cfg.rsolver < 0) throw std::runtime_error("unknown rsolver: " + s);
+  if (p.get_bool("mhd", "characteristic", false)) throw std::runtime_error("mhd.characteristic = true is not supported");
+  cfg.lhlld_phi = p.get_real("mhd", "lhlld_phi", -1.0);
+  cfg.dual_energy = p.get_bool("mhd", "dual_energy", false);
+  cfg.de_eta = p.get_real("mhd", "de_eta", 1.0e-3);
+  const std::string t = p.get_string("time", "integrator", "rk3");
+  cfg.nstage = t == "rk1" ? 1 : t == "rk2" ? 2 : t == "rk3" ? 3 : (t == "rk4" || t == "ssprk43") ? 4 : -1;
+  if (cfg.nstage < 0) throw std::runtime_error("unknown integrator: " + t);
+  cfg.low_mach = p.get_bool("mhd", "low_mach", false);
+  cfg.flatten = p.get_bool("mhd", "flatten", true);
+  cfg.cfl = p.get_real("time", "cfl", 0.3);
+  cfg.gamma = p.get_real("mhd", "gamma", 5.0 / 3.0);
+  cfg.dfloor = p.get_real("mhd", "dfloor", 1.0e-10);
+  cfg.pfloor = p.get_real("mhd", "pfloor", 1.0e-12);
+  cfg.bc[0] = parse_bc(p.get_string("mesh", "bc_x1", "periodic"));
+  cfg.bc[1] = parse_bc(p.get_string("mesh", "bc_x2", "periodic"));
+  cfg.bc[2] = parse_bc(p.get_string("mesh", "bc_x3", "periodic"));
+  cfg.fofc = p.get_bool("mhd", "fofc", true);
+  if (p.get_bool("mhd", "flux_limiter", false)) throw std::runtime_error("mhd.flux_limiter is not supported");
+  if (p.get_bool("diffusion", "sts", false) || p.get_real("diffusion", "nu", 0.0) != 0.0 ||
+      p.get_real("diffusion", "kappa", 0.0) != 0.0 || p.get_real("diffusion", "eta", 0.0) != 0.0)
+    throw std::runtime_error("diffusive terms are not supported");
+  if (p.get_bool("gravity", "uniform", false) || p.get_bool("gravity", "point_mass", false) ||
+      p.get_bool("cooling", "enabled", false))
+    throw std::runtime_error("gravity and cooling source terms are not supported");
+  g.nx1 = p.get_int("mesh", "nx1");
+  g.nx2 = p.get_int("mesh", "nx2", 1);
+  g.nx3 = p.get_int("mesh", "nx3", 1);
+  const int ngmin = (cfg.recon == 2 || cfg.recon == 3 || cfg.recon == 4) ? 3 : 2;
+  g.ng = p.get_int("mesh", "nghost", ngmin);
+  if (g.ng < ngmin) throw std::runtime_error("nghost too small for the reconstruction method");
+  g.x1min = p.get_real("mesh", "x1min", 0.0); g.x1max = p.get_real("mesh", "x1max", 1.0);
+  g.x2min = p.get_real("mesh", "x2min", 0.0); g.x2max = p.get_real("mesh", "x2max", 1.0);
+  g.x3min = p.get_real("mesh", "x3min", 0.0); g.x3max = p.get_real("mesh", "x3max", 1.0);
+  g.build();
+  mesh.nx1 = g.nx1; mesh.nx2 = g.nx2; mesh.nx3 = g.nx3; mesh.ng = g.ng;
+  mesh.x1min = g.x1min; mesh.x1max = g.x1max; mesh.x2min = g.x2min; mesh.x2max = g.x2max;
+  mesh.x3min = g.x3min; mesh.x3max = g.x3max;
+  tlim = p.get_real("time", "tlim", 1.0);
+  nlim = p.get_int("time", "nlim", -1);
+  ndiag = p.get_int("time", "ndiag", 100);
+  problem = p.get_string("problem", "name");
+  basename = p.get_string("output", "basename", problem);
+  vtk_dt = p.get_real("output", "vtk_dt", -1.0);
+  hst_dt = p.get_real("output", "hst_dt", -1.0);
+  rst_dt = p.get_real("output", "rst_dt", -1.0);
+  compute_error = p.get_bool("output", "compute_error", false);
+}
+
+void Deck::initialize() {
+  const size_t nc = g.ncell(), nf = g.nface();
+  w.assign(6 * nc, 0.0);
+  u0.assign(6 * nc, 0.0);
+  b1.assign(nf, 0.0);
+  b2.assign(nf, 0.0);
+  b3.assign(nf, 0.0);
+  Fields h{g, w, b1, b2, b3};
+  if (problem == "cpaw") cpaw(p, h);
+  else if (problem == "orszag_tang") orszag_tang(p, h);
+  else if (problem == "blast") blast(p, h);
+  else if (problem == "shock_tube") shock_tube(p, h);
+  else throw std::runtime_error("unknown problem: " + problem);
+  // conserved variables from primitives and face fields, pgen.hpp:286-301
+  const double gm1 = cfg.gamma - 1.0;
+  for (int k = 0; k < g.nk; ++k)
+    for (int j = 0; j < g.nj; ++j)
+      for (int i = 0; i < g.ni; ++i) {
+        const size_t c = g.c(k, j, i);
+        const double bx = 0.5 * (b1[g.f(k, j, i)] + b1[g.f(k, j, i + 1)]);
+        const double by = g.two_d ? 0.5 * (b2[g.f(k, j, i)] + b2[g.f(k, j + 1, i)]) : b2[g.f(k, j, i)];
+        const double bz = g.three_d ? 0.5 * (b3[g.f(k, j, i)] + b3[g.f(k + 1, j, i)]) : b3[g.f(k, j, i)];
+        const double d = w[c], vx = w[nc + c], vy = w[2 * nc + c], vz = w[3 * nc + c], pr = w[4 * nc + c];
+        u0[c] = d;
+        u0[nc + c] = d * vx;
+        u0[2 * nc + c] = d * vy;
+        u0[3 * nc + c] = d * vz;
+        u0[4 * nc + c] = pr / gm1 + 0.5 * d * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz);
+        w[5 * nc + c] = pr / std::pow(d, cfg.gamma);
+        u0[5 * nc + c] = d * w[5 * nc + c];
+      }
+}
+
+}  // namespace chq
+
+// ---------------------------------------------------------------- C ABI
+struct chq_deck {
+  chq::Deck d;
+};
+
+namespace {
+thread_local std::string g_deck_err;
+}
+
+extern "C" {
+
+const char *chq_deck_last_error(void) { return g_deck_err.c_str(); }
+
+int chq_deck_open(const char *text, const char *overrides, chq_deck **out) {
+  if (!out || !text) { g_deck_err = "null argument"; return CHQ_EINVAL; }
+  *out = nullptr;
+  chq_deck *k = new chq_deck();
+  try {
+    k->d.p.read_text(text);
+    if (overrides) {
+      std::istringstream in(overrides);
+      std::string line;
+      while (std::getline(in, line)) {
+        const auto a = line.find_first_not_of(" \t\r");
+        if (a == std::string::npos) continue;
+        k->d.p.override_kv(line.substr(a));
+      }
+    }
+    k->d.configure();
+    k->d.initialize();
+  } catch (const std::exception &e) {
+    g_deck_err = e.what();
+    delete k;
+    return CHQ_EINVAL;
+  }
+  *out = k;
+  return CHQ_OK;
+}
+
+int chq_deck_close(chq_deck *k) {
+  delete k;
+  return CHQ_OK;
+}
+
+int chq_deck_mesh(const chq_deck *k, chq_mesh_desc *m) {
+  if (!k || !m) return CHQ_EINVAL;
+  *m = k->d.mesh;
+  return CHQ_OK;
+}
+
+int chq_deck_config(const chq_deck *k, chq_config *c) {
+  if (!k || !c) return CHQ_EINVAL;
+  *c = k->d.cfg;
+  return CHQ_OK;
+}
+
+int chq_deck_run(const chq_deck *k, double *tlim, int *nlim, int *ndiag) {
+  if (!k) return CHQ_EINVAL;
+  if (tlim) *tlim = k->d.tlim;
+  if (nlim) *nlim = k->d.nlim;
+  if (ndiag) *ndiag = k->d.ndiag;
+  return CHQ_OK;
+}
+
+int chq_deck_initial(const chq_deck *k, int array, void *host, size_t count) {
+  if (!k || !host) return CHQ_EINVAL;
+  const std::vector<double> *v = nullptr;
+  switch (array) {
+    case CHQ_U0: v = &k->d.u0; break;
+    case CHQ_W: v = &k->d.w; break;
+    case CHQ_B1F0: v = &k->d.b1; break;
+    case CHQ_B2F0: v = &k->d.b2; break;
+    case CHQ_B3F0: v = &k->d.b3; break;
+    default: g_deck_err = "the deck holds u0, w and b1f0..b3f0 only"; return CHQ_EINVAL;
+  }
+  if (count != v->size()) { g_deck_err = "count mismatch"; return CHQ_EINVAL; }
+  std::memcpy(host, v->data(), count * sizeof(double));
+  return CHQ_OK;
+}
+
+const char *chq_deck_get(const chq_deck *k, const char *block, const char *key) {
+  static thread_local std::string v;
+  if (!k || !block || !key || !k->d.p.has(block, key)) return "";
+  v = k->d.p.get_string(block, key);
+  return v.c_str();
+}
+
+size_t chq_deck_print(const chq_deck *k, char *buf, size_t len) {
+  if (!k) return 0;
+  std::ostringstream o;
+  k->d.p.print(o);
+  const std::string s = o.str();
+  if (buf && len > 0) {
+    const size_t n = std::min(len - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return s.size() + 1;
+}
+
+int chq_deck_upload(const chq_deck *k, chq_ctx *ctx) {
+  if (!k || !ctx) return CHQ_EINVAL;
+  int rc = chq_upload(ctx, CHQ_U0, k->d.u0.data(), k->d.u0.size());
+  if (!rc) rc = chq_upload(ctx, CHQ_W, k->d.w.data(), k->d.w.size());
+  if (!rc) rc = chq_upload(ctx, CHQ_B1F0, k->d.b1.data(), k->d.b1.size());
+  if (!rc) rc = chq_upload(ctx, CHQ_B2F0, k->d.b2.data(), k->d.b2.size());
+  if (!rc) rc = chq_upload(ctx, CHQ_B3F0, k->d.b3.data(), k->d.b3.size());
+  return rc;
+}
+
+}  // extern "C"

@@ -136,6 +136,17 @@ class Mesh:
         check(self.L.chq_step_timed(self.h, int(n), C.byref(ms)), "chq_step_timed")
         return ms.value
 
+    def kernel_timing(self, kernel_class):
+        """Select a kernel class ('flux', 'corner', 'update', 'bc', 'c2p', 'repair', or None)
+        whose launches inside step() are bracketed by HIP events."""
+        code = -1 if kernel_class is None else _capi.KERNEL_CLASSES[kernel_class]
+        check(self.L.chq_kernel_timing(self.h, code), "chq_kernel_timing")
+
+    def kernel_timing_read(self):
+        ms, n = C.c_double(0.0), C.c_int(0)
+        check(self.L.chq_kernel_timing_read(self.h, C.byref(ms), C.byref(n)), "chq_kernel_timing_read")
+        return ms.value, n.value
+
     def status(self):
         s = _capi.Status()
         check(self.L.chq_get_status(self.h, C.byref(s)), "chq_get_status")

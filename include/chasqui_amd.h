@@ -117,7 +117,43 @@ int chq_step(chq_ctx *ctx, int nsteps);
 int chq_step_timed(chq_ctx *ctx, int nsteps, double *ms);
 int chq_get_status(chq_ctx *ctx, chq_status *st);
 int chq_sync(chq_ctx *ctx);
+
+/* Per-kernel timing of chq_step (no reference counterpart: measurement support). When a
+ * class is selected, HIP events are recorded on the context stream around each launch of
+ * that class; chq_kernel_timing_read synchronises and returns the summed duration and the
+ * number of launches since selection (then starts over). kernel_class -1 turns it off. */
+enum chq_kernel_class {
+  CHQ_K_FLUX = 0, CHQ_K_CORNER, CHQ_K_UPDATE, CHQ_K_BC, CHQ_K_C2P, CHQ_K_REPAIR, CHQ_K_NCLASS
+};
+int chq_kernel_timing(chq_ctx *ctx, int kernel_class);
+int chq_kernel_timing_read(chq_ctx *ctx, double *total_ms, int *launches);
 void *chq_stream(chq_ctx *ctx);
+
+/* ---- host layer: decks, problem setups, outputs (host-only, no device needed) ---- */
+typedef struct chq_deck chq_deck;
+/* Params::ReadFile + Override (params.hpp:19-44) on `text`, then the driver's configuration
+ * (main.cpp:112-185) and the problem generator (pgen.hpp:269-307: cpaw, orszag_tang, blast,
+ * shock_tube). `overrides`: newline-separated block.key=value lines, or NULL. */
+int chq_deck_open(const char *text, const char *overrides, chq_deck **out);
+int chq_deck_close(chq_deck *deck);
+const char *chq_deck_last_error(void);
+int chq_deck_mesh(const chq_deck *deck, chq_mesh_desc *mesh);
+int chq_deck_config(const chq_deck *deck, chq_config *cfg);
+int chq_deck_run(const chq_deck *deck, double *tlim, int *nlim, int *ndiag);
+/* value of block.key after defaults were recorded ("" if absent); valid until close */
+const char *chq_deck_get(const chq_deck *deck, const char *block, const char *key);
+/* initial state (CHQ_U0, CHQ_W, CHQ_B1F0, CHQ_B2F0, CHQ_B3F0) in the reference layout */
+int chq_deck_initial(const chq_deck *deck, int array, void *host, size_t count);
+/* Params::Print (params.hpp:82-90) into buf; returns the length needed (with the NUL) */
+size_t chq_deck_print(const chq_deck *deck, char *buf, size_t len);
+/* upload the initial state (u0, w, b?f0) into a context created from chq_deck_mesh/config */
+int chq_deck_upload(const chq_deck *deck, chq_ctx *ctx);
+
+/* output::WriteVTK (output.hpp:32-85): legacy VTK, PLUTO layout, of the current primitives */
+int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, double time, int cycle);
+/* output::History (output.hpp:91-119) of set `buf`: out[10] = mass mom1 mom2 mom3 etot ekin
+ * emag eint divB_max divB_L1 */
+int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]);
 
 #ifdef __cplusplus
 }

@@ -872,8 +872,9 @@ struct Sim {
   std::string problem;
   A4 u_init, bcc_init;
 
-  // configuration + mesh + initial state (main.cpp:112-208)
-  void setup() {
+  // configuration + mesh + initial state (main.cpp:112-208); pgen_only stops after the
+  // problem generator (no boundaries, no con2prim) so its output can be compared as is
+  void setup(bool pgen_only = false) {
     Deck &p = deck;
     const std::string r = p.str("mhd", "reconstruct", "ppm");
     cfg.recon = r == "dc" ? Recon::dc : r == "plm" ? Recon::plm : r == "ppm" ? Recon::ppm : r == "ppmx" ? Recon::ppmx
@@ -920,6 +921,7 @@ struct Sim {
     nlim = p.integer("time", "nlim", -1);
     problem = p.str("problem", "name");
     initialize(problem, p, m, cfg);
+    if (pgen_only) return;
     boundaries(m, cfg, 0);
     min_dt = cons_to_prim(m, cfg, 0, true);
     u_init = m.u0;

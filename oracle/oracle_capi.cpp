@@ -24,7 +24,7 @@ void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 int orc_get_threads() { return omp_get_max_threads(); }
 
 // deck_text: INI text; overrides: newline-separated block.key=value lines (may be null)
-void *orc_create(const char *deck_text, const char *overrides, char *err, int errlen) {
+void *orc_create_ex(const char *deck_text, const char *overrides, int pgen_only, char *err, int errlen) {
   try {
     Sim *s = new Sim();
     s->deck.parse(deck_text ? deck_text : "");
@@ -36,12 +36,16 @@ void *orc_create(const char *deck_text, const char *overrides, char *err, int er
         if (!line.empty()) s->deck.override_kv(line);
       }
     }
-    s->setup();
+    s->setup(pgen_only != 0);
     return s;
   } catch (const std::exception &e) {
     set_err(err, errlen, e.what());
     return nullptr;
   }
+}
+
+void *orc_create(const char *deck_text, const char *overrides, char *err, int errlen) {
+  return orc_create_ex(deck_text, overrides, 0, err, errlen);
 }
 
 void orc_destroy(void *p) { delete S(p); }

@@ -36,6 +36,8 @@ def lib():
     vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
     L.orc_create.restype = vp
     L.orc_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+    L.orc_create_ex.restype = vp
+    L.orc_create_ex.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
     L.orc_destroy.argtypes = [vp]
     L.orc_geometry.argtypes = [vp, ip, dp]
     L.orc_array.restype = dp
@@ -83,11 +85,11 @@ def set_threads(n):
 class OracleSim:
     """One mesh of the CPU oracle, initialised from a deck (text) plus overrides."""
 
-    def __init__(self, deck_text, overrides=()):
+    def __init__(self, deck_text, overrides=(), pgen_only=False):
         L = lib()
         err = C.create_string_buffer(1024)
         ov = "\n".join(overrides).encode()
-        self.h = L.orc_create(deck_text.encode(), ov, err, 1024)
+        self.h = L.orc_create_ex(deck_text.encode(), ov, int(pgen_only), err, 1024)
         if not self.h:
             raise RuntimeError(err.value.decode())
         self.L = L

@@ -1,0 +1,84 @@
+"""GPU: the deck path (host problem setup -> device pipeline) and the chasqui_amd driver
+executable, against the CPU oracle and the reference's output formats."""
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import DECKS, deck
+
+pytestmark = pytest.mark.gpu
+
+po = pytest.importorskip("pyoracle")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "chasqui_amd", "_lib", "chasqui_amd")
+
+
+@pytest.mark.parametrize("name,ov,steps", [
+    ("blast_3d.in", ["mesh.nx1=20", "mesh.nx2=20", "mesh.nx3=20"], 5),
+    ("orszag_tang.in", ["mesh.nx1=32", "mesh.nx2=32", "problem.pres=0.14"], 8),
+    ("cpaw_3d_oblique.in", ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12"], 3),
+    ("brio_wu.in", [], 40),
+])
+def test_deck_pipeline_bitwise(name, ov, steps):
+    from chasqui_amd.deck import Deck
+    print(f"case {name} {ov}", flush=True)
+    text = deck(name)
+    d = Deck(text, ov)
+    m = d.make_mesh()
+    m.step(steps)
+    o = po.OracleSim(text, ov)
+    o.step(steps)
+    st = m.status()
+    assert st["cycle"] == o.cycle and st["t"] == o.time
+    u, ref = m.download("u0"), o.get("u0")
+    assert np.array_equal(u[:5], ref[:5])
+    assert np.max(np.abs(u[5] - ref[5])) <= 1e-14 * np.max(np.abs(ref[5]))
+    m.close()
+
+
+def _run(args, cwd):
+    r = subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_driver_cpaw_l1(tmp_path):
+    out = _run(["-i", os.path.join(DECKS, "cpaw_1d.in"), "time.ndiag=0", "output.basename=cp1"], tmp_path)
+    assert "# zone-cycles/second" in out
+    m = re.search(r"# L1 errors vs initial state .*\n# (.*)\| (\S+)", out)
+    vals = [float(x) for x in m.group(1).split()]
+    o = po.OracleSim(deck("cpaw_1d.in"))
+    o.run()
+    e = o.l1_error()
+    assert np.allclose(vals, e[:9], rtol=1e-5, atol=1e-18)   # printed with 6 digits
+    assert f"{sum(vals) - vals[5]:.2e}" == "6.65e-04"
+    assert (tmp_path / "cp1_errors.dat").exists()
+
+
+def test_driver_outputs(tmp_path):
+    out = _run(["-i", os.path.join(DECKS, "orszag_tang.in"), "mesh.nx1=32", "mesh.nx2=32", "time.tlim=0.1",
+                "output.basename=ot", "output.vtk_dt=0.05", "output.hst_dt=0.02", "time.ndiag=5"], tmp_path)
+    assert "cycle=5 " in out
+    vtks = sorted(p.name for p in tmp_path.glob("ot.*.vtk"))
+    assert vtks[0] == "ot.0000.vtk" and len(vtks) >= 3
+    raw = (tmp_path / "ot.0000.vtk").read_bytes()
+    assert raw.startswith(b"# vtk DataFile Version 2.0\nChasqui MHD VTK Data\nBINARY\nDATASET RECTILINEAR_GRID\n")
+    assert b"DIMENSIONS 33 33 1\n" in raw and b"CELL_DATA 1024\n" in raw
+    i = raw.index(b"SCALARS rho float\nLOOKUP_TABLE default\n") + len(b"SCALARS rho float\nLOOKUP_TABLE default\n")
+    rho = struct.unpack(">1024f", raw[i:i + 4096])
+    assert abs(rho[0] - 25.0 / (36.0 * np.pi)) < 1e-6
+    rows = [list(map(float, l.split())) for l in (tmp_path / "ot.hst").read_text().splitlines() if not l.startswith("#")]
+    assert len(rows) >= 5
+    mass = [r[2] for r in rows]
+    assert max(abs(x - mass[0]) for x in mass) <= 1e-12 * mass[0]
+    assert max(r[10] for r in rows) <= 5e-13          # divB_max
+
+
+def test_driver_rejects_bad_deck(tmp_path):
+    r = subprocess.run([EXE, "-i", os.path.join(DECKS, "cpaw_1d.in"), "mhd.reconstruct=bogus"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "unknown reconstruct" in r.stderr

@@ -1,0 +1,82 @@
+"""The C++ host layer on a CPU-only host: deck parsing and problem setups (bitwise equal to
+the oracle's restatement of pgen.hpp), error handling of bad decks, parameter echo."""
+import numpy as np
+import pytest
+
+from helpers import deck
+
+po = pytest.importorskip("pyoracle")
+from chasqui_amd.deck import Deck, DeckError  # noqa: E402
+
+CASES = [
+    ("blast_3d.in", ["mesh.nx1=12", "mesh.nx2=10", "mesh.nx3=8"]),
+    ("blast_3d.in", ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=1", "problem.perturb=0"]),
+    ("cpaw_1d.in", ["mesh.nx1=64"]),
+    ("cpaw_1d.in", ["mesh.nx1=16", "mesh.nx2=16", "mesh.x1max=1.4142135623730951",
+                    "mesh.x2max=1.4142135623730951", "problem.angle=45"]),
+    ("cpaw_3d_oblique.in", ["mesh.nx1=8", "mesh.nx2=8", "mesh.nx3=8"]),
+    ("orszag_tang.in", ["mesh.nx1=20", "mesh.nx2=24"]),
+    ("orszag_tang.in", ["mesh.nx1=16", "mesh.nx2=16", "problem.pres=0.14", "problem.b0=0.3"]),
+    ("brio_wu.in", ["mesh.nx1=100"]),
+]
+
+
+@pytest.mark.parametrize("name,ov", CASES)
+def test_initial_state_bitwise_vs_oracle(name, ov):
+    text = deck(name)
+    d = Deck(text, ov)
+    o = po.OracleSim(text, ov, pgen_only=True)
+    for host, ora in (("u0", "u0"), ("w", "w"), ("b1f0", "b1f0"), ("b2f0", "b2f0"), ("b3f0", "b3f0")):
+        a, b = d.initial(host), o.get(ora)
+        assert a.shape == b.shape
+        assert np.array_equal(a, b), f"{host}: {np.count_nonzero(a != b)} entries differ"
+
+
+def test_blast_noise_statistics():
+    d = Deck(deck("blast_3d.in"), ["mesh.nx1=32", "mesh.nx2=32", "mesh.nx3=32", "problem.perturb=1e-3"])
+    rho = d.initial("w")[0]
+    dev = (rho - 1.0) / 1e-3
+    assert dev.min() >= -1.0 and dev.max() < 1.0
+    assert abs(dev.mean()) < 0.02 and abs(dev.std() - 1 / np.sqrt(3)) < 0.01   # U[-1, 1)
+
+
+def test_defaults_reproduce_reference_orszag_tang():
+    """problem.pres / problem.b0 absent: the reference constants 5/(12 pi), 1/sqrt(4 pi)."""
+    d = Deck(deck("orszag_tang.in"), ["mesh.nx1=8", "mesh.nx2=8"])
+    assert float(d.get("problem", "pres")) == 5.0 / (12.0 * np.pi)
+    assert float(d.get("problem", "b0")) == 1.0 / np.sqrt(4.0 * np.pi)
+
+
+def test_deck_defaults_recorded_and_printed():
+    d = Deck(deck("cpaw_1d.in"))
+    assert d.get("mhd", "fofc") == "true"
+    assert d.get("time", "nlim") == "-1"
+    assert d.get("mesh", "nghost") == "3"
+    txt = d.text()
+    assert "[mhd]" in txt and "  reconstruct = ppm" in txt
+    assert d.tlim == 1.0 and d.nlim == -1
+
+
+def test_overrides_win_over_file():
+    d = Deck(deck("cpaw_1d.in"), ["mesh.nx1=32", "mhd.reconstruct=wenoz", "time.cfl=0.25"])
+    assert d.mesh_desc.nx1 == 32 and d.config.recon == 3 and d.config.cfl == 0.25
+
+
+@pytest.mark.parametrize("ov,msg", [
+    (["mhd.reconstruct=nonsense"], "unknown reconstruct"),
+    (["mhd.rsolver=roe"], "unknown rsolver"),
+    (["mesh.bc_x1=bogus"], "unknown boundary"),
+    (["time.integrator=rk7"], "unknown integrator"),
+    (["mesh.nghost=2"], "nghost too small"),
+    (["problem.name=nothere"], "unknown problem"),
+    (["mhd.characteristic=true"], "not supported"),
+    (["diffusion.eta=0.1"], "not supported"),
+])
+def test_bad_decks_raise(ov, msg):
+    with pytest.raises(DeckError, match=msg):
+        Deck(deck("cpaw_1d.in"), ov)
+
+
+def test_missing_required_key():
+    with pytest.raises(DeckError, match="missing parameter mesh.nx1"):
+        Deck("[problem]\nname = cpaw\n")

@@ -1,0 +1,88 @@
+"""Whole-pipeline invariants of the oracle (README.md:100, 150-152; physics.sh:57, 91-92):
+div B at round-off, conservation in periodic and closed boxes, bitwise mirror symmetry of a
+blast, fofc forced from the host."""
+import numpy as np
+import pytest
+
+from helpers import deck
+
+po = pytest.importorskip("pyoracle")
+
+
+def div_b(o, s=0):
+    b1, b2, b3 = o.get(f"b1f{s}"), o.get(f"b2f{s}"), o.get(f"b3f{s}")
+    ks, ke, js, je, is_, ie = o.ks, o.ke, o.js, o.je, o.is_, o.ie
+    g, r = o.geometry()
+    dx1, dx2, dx3 = r[0], r[1], r[2]
+    k, j, i = slice(ks, ke + 1), slice(js, je + 1), slice(is_, ie + 1)
+    d = (b1[k, j, is_ + 1:ie + 2] - b1[k, j, i]) / dx1
+    if o.two_d:
+        d = d + (b2[k, js + 1:je + 2, i] - b2[k, j, i]) / dx2
+    if o.three_d:
+        d = d + (b3[ks + 1:ke + 2, j, i] - b3[k, j, i]) / dx3
+    return float(np.max(np.abs(d)))
+
+
+def totals(o):
+    u = o.get("u0")[:, o.ks:o.ke + 1, o.js:o.je + 1, o.is_:o.ie + 1]
+    return u.reshape(6, -1).sum(axis=1)
+
+
+@pytest.mark.parametrize("name,ov,steps", [
+    ("blast_3d.in", ["mesh.nx1=24", "mesh.nx2=24", "mesh.nx3=24"], 15),
+    ("orszag_tang.in", ["mesh.nx1=48", "mesh.nx2=48"], 60),
+    ("cpaw_3d_oblique.in", ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12"], 10),
+])
+def test_divb_and_conservation(name, ov, steps):
+    o = po.OracleSim(deck(name), ov)
+    t0 = totals(o)
+    assert div_b(o) <= 5e-13
+    o.step(steps)
+    assert o.cycle == steps
+    assert div_b(o) <= 5e-13
+    t1 = totals(o)
+    for n in (0, 4):    # mass and total energy in a periodic domain
+        assert abs(t1[n] - t0[n]) <= 1e-12 * abs(t0[n]), (n, t0[n], t1[n])
+    for n in (1, 2, 3):
+        assert abs(t1[n] - t0[n]) <= 1e-12 * max(abs(t0[0]), 1.0)
+
+
+def test_reflecting_box_conserves_mass():
+    """physics.sh:57, 91-92: blast in a box with reflecting x walls (angle 90: B normal to
+    the walls vanishes) keeps its mass to 1e-9."""
+    o = po.OracleSim(deck("blast_3d.in"), ["mesh.nx1=32", "mesh.nx2=32", "mesh.nx3=1", "mesh.bc_x1=reflect",
+                                           "problem.angle=90", "problem.perturb=0"])
+    m0 = totals(o)[0]
+    o.step(40)
+    assert abs(totals(o)[0] - m0) <= 1e-9 * m0
+
+
+def test_blast_mirror_symmetry_bitwise():
+    """README.md:150-152: a blast keeps its mirror symmetry to 0.0. With B in the x-y plane
+    the z-mirror and the combined (x, y) flip are symmetries of ideal MHD (B -> -B)."""
+    o = po.OracleSim(deck("blast_3d.in"), ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=16", "problem.perturb=0"])
+    o.step(8)
+    a = (slice(None), slice(o.ks, o.ke + 1), slice(o.js, o.je + 1), slice(o.is_, o.ie + 1))
+    u = o.get("u0")[a]
+    assert np.array_equal(u[0], u[0][::-1, :, :])                    # density, z-mirror
+    assert np.array_equal(u[3], -u[3][::-1, :, :])                   # z-momentum flips
+    assert np.array_equal(u[4], u[4][::-1, :, :])
+    assert np.array_equal(u[0], u[0][:, ::-1, ::-1])                 # density, (x, y) flip
+    assert np.array_equal(u[1], -u[1][:, ::-1, ::-1])
+
+
+def test_fofc_forced_by_floor():
+    """Raising the pressure floor above the ambient state forces fofc (main.cpp:135-136, 78-91)."""
+    o = po.OracleSim(deck("blast_3d.in"), ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12", "mhd.pfloor=0.1"])
+    o.step(2)
+    assert o.fofc_count() > 0
+    o2 = po.OracleSim(deck("blast_3d.in"), ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12"])
+    o2.step(2)
+    assert o2.fofc_count() == 0
+
+
+def test_bad_deck_key_raises():
+    with pytest.raises(RuntimeError):
+        po.OracleSim(deck("blast_3d.in"), ["mhd.reconstruct=nonsense"])
+    with pytest.raises(RuntimeError):
+        po.OracleSim(deck("blast_3d.in"), ["mesh.bc_x1=bogus"])

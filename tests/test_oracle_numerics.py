@@ -1,0 +1,192 @@
+"""Pins the oracle's pointwise numerics: the reference's bitwise mirror-symmetry invariant
+(tests/symmetry_test.cpp:20-70, same sample classes, extended to DC, LHLLD and the
+flattening coefficient) and closed forms that a plausible slip would break."""
+import numpy as np
+import pytest
+
+po = pytest.importorskip("pyoracle")
+
+N = 200_000
+
+
+def _recon_samples(rng):
+    q = rng.uniform(-2.0, 2.0, size=(N, 6))
+    idx = np.arange(N)
+    q[idx % 3 == 0] = np.abs(q[idx % 3 == 0])                     # positive data
+    ev = idx % 5 == 0                                               # even-symmetric stencil
+    q[ev, 3], q[ev, 4], q[ev, 5] = q[ev, 2], q[ev, 1], q[ev, 0]
+    return q
+
+
+@pytest.mark.parametrize("method", ["dc", "plm", "ppm", "ppmx", "wenoz"])
+def test_recon_mirror_bitwise(method):
+    rng = np.random.default_rng(12345)
+    q = _recon_samples(rng)
+    L, R = po.reconstruct(method, q)
+    Lm, Rm = po.reconstruct(method, q[:, ::-1].copy())
+    La, Ra = po.reconstruct(method, -q[:, ::-1].copy())
+    assert np.array_equal(Lm, R) and np.array_equal(Rm, L)
+    assert np.array_equal(La, -R) and np.array_equal(Ra, -L)
+
+
+def _riemann_samples(rng):
+    P = lambda: rng.uniform(0.05, 2.0, N)  # noqa: E731
+    U = lambda: rng.uniform(-1.0, 1.0, N)  # noqa: E731
+    L = np.stack([P(), U(), U(), U(), P(), U(), U()], axis=1)
+    R = np.stack([P(), U(), U(), U(), P(), U(), U()], axis=1)
+    bn = U()
+    idx = np.arange(N)
+    bn[idx % 4 == 0] = 0.0                                          # degenerate branch
+    m = idx % 7 == 0                                                # exact mirror pair
+    R[m] = L[m]
+    R[m, 1] = -L[m, 1]
+    return L, R, bn
+
+
+def _mirror(L, R):
+    Lm = R.copy()
+    Lm[:, 1] = -R[:, 1]
+    Rm = L.copy()
+    Rm[:, 1] = -L[:, 1]
+    return Lm, Rm
+
+
+SIGNS = np.array([-1, 1, -1, -1, -1, -1, -1], dtype=float)   # FD FMU FMV FMW FEN FBY FBZ
+
+
+@pytest.mark.parametrize("solver", ["hlle", "hlld", "lhlld"])
+def test_riemann_mirror_bitwise(solver):
+    rng = np.random.default_rng(777)
+    L, R, bn = _riemann_samples(rng)
+    theta = rng.uniform(0.0, 1.0, N) if solver == "lhlld" else None
+    phi = rng.uniform(0.0, 1.0, N) if solver == "lhlld" else None
+    f = po.riemann(solver, L, R, bn, theta=theta, phi=phi)
+    Lm, Rm = _mirror(L, R)
+    g = po.riemann(solver, Lm, Rm, -bn, theta=theta, phi=phi)
+    assert np.array_equal(g, f * SIGNS), int(np.count_nonzero(g != f * SIGNS))
+
+
+def test_flatten_mirror_bitwise():
+    rng = np.random.default_rng(99)
+    p = rng.uniform(0.01, 5.0, size=(N, 5))
+    um, up = rng.uniform(-1, 1, N), rng.uniform(-1, 1, N)
+    f = po.flatten(p, um, up)
+    fm = po.flatten(p[:, ::-1].copy(), -up, -um)
+    assert np.array_equal(f, fm)
+
+
+@pytest.mark.parametrize("method", ["dc", "plm", "ppm", "ppmx", "wenoz"])
+def test_recon_constant_data(method):
+    q = np.full((5, 6), 0.0) + np.array([0.3, -1.7, 2.5, 1e-8, 7.0])[:, None]
+    L, R = po.reconstruct(method, q)
+    assert np.array_equal(L, q[:, 0]) and np.array_equal(R, q[:, 0])
+
+
+@pytest.mark.parametrize("method", ["plm", "ppm", "ppmx", "wenoz"])
+def test_recon_linear_data_exact(method):
+    # cell averages of a linear profile: every method is exact at the face (x = 0)
+    a, b = 0.7, -1.3
+    x = np.arange(-3, 3) + 0.5
+    q = (a + b * x)[None, :]
+    L, R = po.reconstruct(method, q)
+    assert abs(L[0] - a) < 1e-14 and abs(R[0] - a) < 1e-14
+
+
+def test_wenoz_smooth_quadratic_is_fifth_order_linear_scheme():
+    # For cell averages of a quadratic the three smoothness indicators are equal, tau = 0,
+    # so WENO-Z uses the ideal weights and reduces to the 5th-order upwind formula
+    # (2a - 13b + 47c + 27d - 3e) / 60 of Jiang & Shu 1996 — exact for polynomials of
+    # degree <= 4, hence equal to the true face value of the quadratic.
+    c0, c1, c2 = 0.4, -0.9, 0.35
+    avg = lambda x0: c0 + c1 * x0 + c2 * (x0 ** 2 + 1.0 / 12.0)  # noqa: E731 cell average on [x0-1/2, x0+1/2]
+    x = np.arange(-3, 3) + 0.5
+    q = avg(x)[None, :]
+    L, R = po.reconstruct("wenoz", q)
+    a, b, c, d, e = q[0, :5]
+    lin = (2 * a - 13 * b + 47 * c + 27 * d - 3 * e) / 60.0
+    assert abs(L[0] - lin) < 1e-14 and abs(L[0] - c0) < 1e-14 and abs(R[0] - c0) < 1e-14
+
+
+def test_ppm_extremum_gives_donor_cell():
+    # cell i-1 (q[2]) is a local maximum: the CW84 limiter flattens its parabola
+    q = np.array([[0.0, 1.0, 3.0, 1.0, 0.5, 0.2]])
+    L, _ = po.reconstruct("ppm", q)
+    assert L[0] == 3.0
+
+
+def test_plm_van_leer_slope_closed_form():
+    q = np.array([[0.0, 1.0, 2.0, 4.0, 5.0, 6.0]])
+    L, R = po.reconstruct("plm", q)
+    vl = lambda a, b, c: 2 * (b - a) * (c - b) / ((b - a) + (c - b)) if (b - a) * (c - b) > 0 else 0.0  # noqa: E731
+    assert L[0] == 2.0 + 0.5 * vl(1.0, 2.0, 4.0)
+    assert R[0] == 4.0 - 0.5 * vl(2.0, 4.0, 5.0)
+
+
+def test_flattening_step_closed_form():
+    p = np.array([[1.0, 1.0, 1.0, 10.0, 10.0], [1.0, 1.0, 5.0, 10.0, 10.0], [1.0, 1.0, 1.0, 1.2, 1.2]])
+    f_comp = po.flatten(p, np.array([1.0, 1.0, 1.0]), np.array([0.0, 0.0, 0.0]))
+    f_exp = po.flatten(p, np.array([0.0, 0.0, 0.0]), np.array([1.0, 1.0, 1.0]))
+    # dp / dp2 = 1 -> 10 * (1 - 0.75) clipped to 1; expanding flow or weak jump -> 0
+    assert list(f_comp) == [1.0, 1.0, 0.0]
+    assert list(f_exp) == [0.0, 0.0, 0.0]
+
+
+@pytest.mark.parametrize("solver", ["hlle", "hlld", "lhlld"])
+def test_riemann_consistency(solver):
+    rng = np.random.default_rng(5)
+    S = np.stack([rng.uniform(0.1, 2, 500), rng.uniform(-1, 1, 500), rng.uniform(-1, 1, 500),
+                  rng.uniform(-1, 1, 500), rng.uniform(0.1, 2, 500), rng.uniform(-1, 1, 500),
+                  rng.uniform(-1, 1, 500)], axis=1)
+    bn = rng.uniform(-1, 1, 500)
+    f = po.riemann(solver, S, S, bn)
+    F = po.phys_flux(S, bn)
+    assert np.allclose(f, F, rtol=1e-12, atol=1e-13)
+
+
+def _phys_flux_closed_form(s, bn, gamma):
+    d, u, v, w, p, by, bz = s
+    b2 = bn * bn + by * by + bz * bz
+    e = p / (gamma - 1) + 0.5 * (d * (u * u + v * v + w * w) + b2)
+    pt = p + 0.5 * b2
+    vb = u * bn + v * by + w * bz
+    return np.array([d * u, d * u * u + pt - bn * bn, d * u * v - bn * by, d * u * w - bn * bz,
+                     (e + pt) * u - bn * vb, by * u - bn * v, bz * u - bn * w])
+
+
+def test_phys_flux_closed_form():
+    s = np.array([1.3, 0.4, -0.2, 0.7, 0.9, 0.3, -0.6])
+    assert np.allclose(po.phys_flux(s, 0.8)[0], _phys_flux_closed_form(s, 0.8, 5.0 / 3.0), rtol=1e-14)
+
+
+@pytest.mark.parametrize("solver", ["hlle", "hlld"])
+def test_supersonic_upwind(solver):
+    L = np.array([[1.0, 20.0, 0.1, -0.2, 0.5, 0.3, 0.2]])
+    R = np.array([[0.5, 19.0, 0.0, 0.1, 0.4, -0.2, 0.1]])
+    f = po.riemann(solver, L, R, 0.4)
+    assert np.array_equal(f, po.phys_flux(L, 0.4))
+    f = po.riemann(solver, -L * np.array([-1, 1, -1, -1, -1, -1, -1]), R * np.array([1, -1, 1, 1, 1, 1, 1]), 0.4)
+    assert np.isfinite(f).all()
+
+
+def test_hlld_resolves_stationary_contact():
+    """Miyoshi & Kusano 2005: HLLD is exact for an isolated contact (density jump only)."""
+    bn = 0.7
+    L = np.array([[1.0, 0.0, 0.0, 0.0, 1.0, 0.4, -0.3]])
+    R = np.array([[0.2, 0.0, 0.0, 0.0, 1.0, 0.4, -0.3]])
+    f = po.riemann("hlld", L, R, bn)[0]
+    pt = 1.0 + 0.5 * (bn * bn + 0.4 ** 2 + 0.3 ** 2)
+    exact = np.array([0.0, pt - bn * bn, -bn * 0.4, bn * 0.3, 0.0, 0.0, 0.0])
+    assert np.allclose(f, exact, atol=1e-15)
+    fe = po.riemann("hlle", L, R, bn)[0]
+    assert abs(fe[0]) > 1e-3   # HLLE smears the contact (sanity of the test itself)
+
+
+def test_fast_speed_closed_form():
+    # c_f^2 = 1/2 (a^2 + b^2 + sqrt((a^2 + b^2)^2 - 4 a^2 b_n^2)) with a^2 = gamma p / rho
+    s = np.array([2.0, 0.0, 0.0, 0.0, 0.6, 0.5, -0.2])
+    bn, gamma = 0.8, 5.0 / 3.0
+    a2 = gamma * 0.6 / 2.0
+    b2 = (bn ** 2 + 0.5 ** 2 + 0.2 ** 2) / 2.0
+    bx2 = bn ** 2 / 2.0
+    cf = np.sqrt(0.5 * ((a2 + b2) + np.sqrt((a2 + b2) ** 2 - 4 * a2 * bx2)))
+    assert abs(po.fast_speed(s, bn, gamma) - cf) < 1e-14
