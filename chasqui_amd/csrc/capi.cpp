@@ -61,6 +61,7 @@ struct chq_ctx {
   double *u[2]{}, *bf[2][3]{}, *w[2]{}, *bcc[2]{}, *ecc[2]{};
   double *flx[3]{}, *ef[6]{}, *ee[3]{};
   int *flag = nullptr;
+  int *badlist = nullptr;    // flagged cells of the current stage (fused fofc path)
   Ctrl *ctrl = nullptr;      // time loop
   Ctrl *ctrl_api = nullptr;  // per-kernel entry points (always active)
   void *staging = nullptr;   // device staging buffer for layout conversion
@@ -144,7 +145,9 @@ struct chq_ctx {
     a.u = u[buf];
     a.w = w[wset]; a.bcc = bcc[wset]; a.ecc = ecc[wset];
     a.write_went = 1;
+    a.entropy_out = 1;
     a.flag = flag;
+    a.badlist = badlist;
     a.ctrl = c;
     return a;
   }
@@ -289,7 +292,7 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
     return cleanup(fail(CHQ_EHIP, std::string("stream/event: ") + hipGetErrorString(e)));
   // arena: 70 component arrays of cstride doubles + two control blocks
   const size_t comp = size_t(g.cstride) * sizeof(double);
-  const size_t ncomp = 12 + 6 + 12 + 6 + 6 + 18 + 6 + 3 + 1;
+  const size_t ncomp = 12 + 6 + 12 + 6 + 6 + 18 + 6 + 3 + 1 + 1;
   c->arena_bytes = ncomp * comp + 2 * 256;
   e = hipMalloc((void **)&c->arena, c->arena_bytes);
   if (e != hipSuccess) { c->arena = nullptr; return cleanup(fail(CHQ_ENOMEM, "device arena of " + std::to_string(c->arena_bytes) + " bytes: " + hipGetErrorString(e))); }
@@ -306,6 +309,7 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   for (int q = 0; q < 6; ++q) c->ef[q] = take(1);
   for (int q = 0; q < 3; ++q) c->ee[q] = take(1);
   c->flag = (int *)take(1);
+  c->badlist = (int *)take(1);
   c->ctrl = (Ctrl *)p;
   c->ctrl_api = (Ctrl *)(p + 256);
   Ctrl h{};
@@ -525,20 +529,22 @@ static int enqueue_step(chq_ctx *c) {
       c->tmark(CHQ_K_BC, false);
       C2pArgs ca = c->c2p_args(cf.out, wn, k);
       ca.write_went = c->entropy;
+      ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
       ca.compute_dt = last;
       ca.slot = 0;
-      ca.flag_mode = c->cfg.fofc ? 1 : 0;
+      ca.flag_mode = c->cfg.fofc ? 2 : 0;
       c->tmark(CHQ_K_C2P, true);
       HIPCHK(launch_c2p(ca, c->stream));
       c->tmark(CHQ_K_C2P, false);
       if (c->cfg.fofc) {
         c->tmark(CHQ_K_REPAIR, true);
+        FluxArgs fr[3];
         for (int d = 0; d < c->ndir(); ++d) {
-          FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
-          fa.entropy = c->entropy;
-          HIPCHK(launch_repair(fa, d, true, c->stream));
+          fr[d] = c->flux_args(d, cf.cur, wc, k);
+          fr[d].entropy = c->entropy;
         }
-        HIPCHK(launch_corner(c->corner_args(wc, k, 1), c->stream));
+        HIPCHK(launch_repair_list(fr, c->ndir(), c->badlist, c->stream));
+        HIPCHK(launch_corner_list(c->corner_args(wc, k, 1), c->badlist, c->stream));
         UpdateArgs ub = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 1);
         ub.use_ctrl_dt = 1;
         ub.beta = cf.beta;

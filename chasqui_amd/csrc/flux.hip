@@ -1,120 +1,52 @@
-// Fused face-flux kernel (reconstruction + flattening + positivity fallback + Riemann solve
+// Fused face-flux kernels (reconstruction + flattening + positivity fallback + Riemann solve
 // + flux/face-EMF store) and the fofc donor-cell repair kernel, for gfx950.
 //
 // Reference: kernels::Fluxes<DIR,R,S> (physics.hpp:65-226) and fofc::RepairFluxes<DIR>
-// (fofc.hpp:46-96). L/R states live only in registers. One thread per face; a block is a
-// 64 x 4 tile of faces (i fastest), so every stencil load of a wave64 is one coalesced
-// 512 B row segment, and the six loads of a stencil along j or k re-hit the same rows in
-// L1/L2 from the neighbouring waves of the block.
+// (fofc.hpp:46-96). L/R states never leave registers. Each cell's two half-states are
+// computed once (recon_cell.hpp) and shared by its two faces:
+//   * x faces (DIR 0): one wave64 per row segment, lane = cell; neighbour values move
+//     between lanes with cross-lane shuffles; a wave yields 64 - 5 faces (halo 3 + 2).
+//   * y and z faces (DIR 1, 2): lane = i (coalesced loads), each thread marches along j or
+//     k over a segment of SEG faces with a register window: every primitive is loaded
+//     once per segment and each cell is reconstructed once.
+// For FP64 the arithmetic and its order are the reference's, so fluxes are bitwise equal.
 #include "kernels.hpp"
 #include "numerics.hpp"
+#include "recon_cell.hpp"
 
 namespace chq {
 
 namespace {
 
 constexpr int BX = 64, BY = 4;
+constexpr int SEG = 16;   // faces per thread in the y/z march
 
-template <int DIR>
-__device__ __forceinline__ long long dir_stride(const Geom &g) {
-  if constexpr (DIR == 0) return 1;
-  else if constexpr (DIR == 1) return g.pitch;
-  else return g.plane;
-}
+template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
 
-template <int DIR, Recon RC, RSolver RS, bool ENT>
-__global__ void __launch_bounds__(BX * BY) k_flux(const FluxArgs a) {
-  if (!a.ctrl->active) return;
-  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = a.jl + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int k = a.kl + int(blockIdx.z);
-  if (i > a.iu || j > a.ju) return;
-  const Geom &g = a.g;
-  using R = Rec<double>;
+// ---------------------------------------------------------------- face solve + store
+// Riemann solve of one face from its two half-states and store (physics.hpp:167-224).
+// c = face index (device layout); sd = stride along DIR.
+template <int DIR, RSolver RS, bool ENT, int NV>
+__device__ __forceinline__ void solve_store(const FluxArgs &a, long long c, long long sd, const double *Ls,
+                                            const double *Rs_) {
   using V = Rsv<double>;
-  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
-  constexpr int B1 = (DIR + 1) % 3, B2 = (DIR + 2) % 3;
-  const long long c = g.at(k, j, i);
-  const long long sd = dir_stride<DIR>(g);
-  const long long cs = g.cstride;
-  constexpr bool wide = !(RC == Recon::dc || RC == Recon::plm);
-
-  // stencils of d, u, v, w, p, B_t1, B_t2 in the rotated frame (physics.hpp:99-106)
-  double st[7][6];
-  const double *src[7] = {a.w + IDN * cs, a.w + IU * cs, a.w + IV * cs, a.w + IW * cs,
-                          a.w + IPR * cs, a.bcc + B1 * cs, a.bcc + B2 * cs};
-#pragma unroll
-  for (int n = 0; n < 7; ++n) {
-    const double *p = src[n] + c;
-    if constexpr (wide) {
-#pragma unroll
-      for (int s = 0; s < 6; ++s) st[n][s] = p[(s - 3) * sd];
-    } else {
-#pragma unroll
-      for (int s = 1; s < 5; ++s) st[n][s] = p[(s - 3) * sd];
-      st[n][0] = st[n][1];
-      st[n][5] = st[n][4];
-    }
-  }
-  double wl[7], wr[7];
-#pragma unroll
-  for (int n = 0; n < 7; ++n) R::template face<RC>(st[n], wl[n], wr[n]);
-
-  double sl = 0.0, sr = 0.0, sl0 = 0.0, sr0 = 0.0;   // entropy (physics.hpp:152-154)
-  if constexpr (ENT) {
-    double q[6];
-    const double *p = a.w + IENT * cs + c;
-    if constexpr (wide) {
-#pragma unroll
-      for (int s = 0; s < 6; ++s) q[s] = p[(s - 3) * sd];
-    } else {
-#pragma unroll
-      for (int s = 1; s < 5; ++s) q[s] = p[(s - 3) * sd];
-      q[0] = q[1];
-      q[5] = q[4];
-    }
-    R::template face<RC>(q, sl, sr);
-    sl0 = q[2];
-    sr0 = q[3];
-  }
-  if (wide && a.flatten) {                                         // physics.hpp:156-162
-    const double fL = R::flat(st[4][0], st[4][1], st[4][2], st[4][3], st[4][4], st[1][1], st[1][3]);
-    const double fR = R::flat(st[4][1], st[4][2], st[4][3], st[4][4], st[4][5], st[1][2], st[1][4]);
-    if (fL > 0.0) {
-#pragma unroll
-      for (int n = 0; n < 7; ++n) wl[n] += fL * (st[n][2] - wl[n]);
-      if constexpr (ENT) sl += fL * (sl0 - sl);
-    }
-    if (fR > 0.0) {
-#pragma unroll
-      for (int n = 0; n < 7; ++n) wr[n] += fR * (st[n][3] - wr[n]);
-      if constexpr (ENT) sr += fR * (sr0 - sr);
-    }
-  }
+  const Geom &g = a.g;
   const Eos e = a.eos;
-  if (wl[0] <= e.dfloor || wl[4] <= e.pfloor) {                    // physics.hpp:163-165
-#pragma unroll
-    for (int n = 0; n < 7; ++n) wl[n] = st[n][2];
-    sl = sl0;
-  }
-  if (wr[0] <= e.dfloor || wr[4] <= e.pfloor) {
-#pragma unroll
-    for (int n = 0; n < 7; ++n) wr[n] = st[n][3];
-    sr = sr0;
-  }
-  St<double> L{wl[0], wl[1], wl[2], wl[3], wl[4], wl[5], wl[6]};
-  St<double> Rs{wr[0], wr[1], wr[2], wr[3], wr[4], wr[5], wr[6]};
+  const long long cs = g.cstride;
+  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
+  St<double> L{Ls[0], Ls[1], Ls[2], Ls[3], Ls[4], Ls[5], Ls[6]};
+  St<double> R{Rs_[0], Rs_[1], Rs_[2], Rs_[3], Rs_[4], Rs_[5], Rs_[6]};
   const double bn = a.bn[c];
   if (a.low_mach) {                                                // physics.hpp:170-178
-    const double cl = V::cfast(L, bn, e.gamma), cr = V::cfast(Rs, bn, e.gamma);
-    const double z = fmin(1.0, fmax(fabs(L.u) / cl, fabs(Rs.u) / cr));
-    const double um = 0.5 * (L.u + Rs.u), ud = 0.5 * (z * (L.u - Rs.u));
+    const double cl = V::cfast(L, bn, e.gamma), cr = V::cfast(R, bn, e.gamma);
+    const double z = fmin(1.0, fmax(fabs(L.u) / cl, fabs(R.u) / cr));
+    const double um = 0.5 * (L.u + R.u), ud = 0.5 * (z * (L.u - R.u));
     L.u = um + ud;
-    Rs.u = um - ud;
+    R.u = um - ud;
   }
   double f[NFLX];
   if constexpr (RS == RSolver::hlld) {
-    V::template hlld<false>(L, Rs, bn, e.gamma, e.gm1, f);
+    V::template hlld<false>(L, R, bn, e.gamma, e.gm1, f);
   } else if constexpr (RS == RSolver::lhlld) {                     // physics.hpp:182-211
     constexpr int t1 = (DIR + 1) % 3, t2 = (DIR + 2) % 3;
     const double *wu = a.w + IU * cs, *wv = a.w + IV * cs, *ww = a.w + IW * cs;
@@ -123,7 +55,7 @@ __global__ void __launch_bounds__(BX * BY) k_flux(const FluxArgs a) {
     const long long s2 = t2 == 0 ? 1 : t2 == 1 ? g.pitch : g.plane;
     const bool has1 = t1 == 0 || (t1 == 1 && g.two_d) || (t1 == 2 && g.three_d);
     const bool has2 = t2 == 0 || (t2 == 1 && g.two_d) || (t2 == 2 && g.three_d);
-    const double cfm = fmax(V::cfast(L, bn, e.gamma), V::cfast(Rs, bn, e.gamma));
+    const double cfm = fmax(V::cfast(L, bn, e.gamma), V::cfast(R, bn, e.gamma));
     const double du = wu[c] - wu[cL];
     double dvw = 0.0;
     if (has1) {
@@ -137,47 +69,234 @@ __global__ void __launch_bounds__(BX * BY) k_flux(const FluxArgs a) {
     const double ratio = fmin(1.0, (-fmin(du, 0.0) + cfm) / (-dvw + cfm));
     const double r2 = ratio * ratio;
     const double theta = r2 * r2;
-    const double chi = fmin(1.0, fmax(V::cflow(L, bn), V::cflow(Rs, bn)) / cfm);
+    const double chi = fmin(1.0, fmax(V::cflow(L, bn), V::cflow(R, bn)) / cfm);
     const double phi = (a.lhlld_phi >= 0.0) ? a.lhlld_phi : chi * (2.0 - chi);
-    V::template hlld<true>(L, Rs, bn, e.gamma, e.gm1, f, theta, phi);
+    V::template hlld<true>(L, R, bn, e.gamma, e.gm1, f, theta, phi);
   } else {
-    V::hlle(L, Rs, bn, e.gamma, e.gm1, f);
+    V::hlle(L, R, bn, e.gamma, e.gm1, f);
   }
-  // store (physics.hpp:215-224)
   double *fx = a.flx + c;
   fx[IDN * cs] = f[FD];
   fx[IU * cs] = f[FMU];
   fx[IV * cs] = f[FMV];
   fx[IW * cs] = f[FMW];
   fx[IEN * cs] = f[FEN];
-  if constexpr (ENT)
+  if constexpr (ENT) {
+    const double sl = Ls[7], sr = Rs_[7];
     fx[IENT * cs] = (f[FD] > 0.0) ? f[FD] * sl : (f[FD] < 0.0) ? f[FD] * sr : 0.5 * (f[FD] * (sl + sr));
+  }
   a.et2[c] = -f[FBY];
   a.et1[c] = f[FBZ];
 }
 
-// Donor-cell HLLE on faces that touch a flagged active cell (fofc.hpp:66-94). In the
-// fused step it runs after every stage and exits at once unless the stage flagged cells.
-template <int DIR>
-__global__ void __launch_bounds__(BX * BY) k_repair(const FluxArgs a, const int only_if_bad) {
+// component offsets (in units of cstride) of the NV reconstructed variables in the rotated
+// frame: d, u, v, w, p (from w), B_t1, B_t2 (from bcc, stored after w's 6 components), IENT
+template <int DIR, int NV>
+__device__ __forceinline__ void var_ptrs(const FluxArgs &a, const double *p[NV]) {
+  const long long cs = a.g.cstride;
+  p[0] = a.w + IDN * cs;
+  p[1] = a.w + (IVX + DIR) * cs;
+  p[2] = a.w + (IVX + (DIR + 1) % 3) * cs;
+  p[3] = a.w + (IVX + (DIR + 2) % 3) * cs;
+  p[4] = a.w + IPR * cs;
+  p[5] = a.bcc + ((DIR + 1) % 3) * cs;
+  p[6] = a.bcc + ((DIR + 2) % 3) * cs;
+  if constexpr (NV == 8) p[7] = a.w + IENT * cs;
+}
+
+// ---------------------------------------------------------------- x faces: lane = cell
+template <Recon RC, RSolver RS, bool ENT>
+__global__ void __launch_bounds__(256) k_flux_x(const FluxArgs a, const int nseg, const int nwaves) {
+  constexpr int NV = ENT ? 8 : 7;
+  constexpr bool W = wide_v<RC>;
+  constexpr int H = W ? 3 : 2;            // lanes before the first face: cell halo + 1
+  constexpr int FPW = 64 - H - (W ? 2 : 1);   // faces per wave
   if (!a.ctrl->active) return;
-  if (only_if_bad && a.ctrl->nbad == 0) return;
-  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = a.jl + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int k = a.kl + int(blockIdx.z);
-  if (i > a.iu || j > a.ju) return;
+  const int wave = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  const int lane = int(threadIdx.x) & 63;
+  const int seg = wave % nseg, row = wave / nseg;
+  const int nrow_j = a.ju - a.jl + 1;
+  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
   const Geom &g = a.g;
-  constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  const int F0 = a.il + seg * FPW;
+  int ci = F0 - H + lane;                 // this lane's cell
+  ci = ci < 0 ? 0 : (ci > g.ni - 1 ? g.ni - 1 : ci);   // halo lanes past the row end: any valid cell
+  const long long row0 = g.at(k, j, 0);
+  const double *src[NV];
+  var_ptrs<0, NV>(a, src);
+  double q[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) q[n] = src[n][row0 + ci];
+  using R = Rec<double>;
+  double Lf[NV], Rf[NV];
+  double fco = 0.0;
+  if constexpr (RC == Recon::ppm) {
+    double d[NV], e[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      const double qm = __shfl_up(q[n], 1, 64), qp = __shfl_down(q[n], 1, 64);
+      d[n] = R::mc(qm, q[n], qp);
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {   // e[n] = edge at c+1/2
+      const double qp = __shfl_down(q[n], 1, 64), dp = __shfl_down(d[n], 1, 64);
+      e[n] = R::edge(q[n], qp, d[n], dp);
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      double lo = __shfl_up(e[n], 1, 64), hi = e[n];
+      R::mono(q[n], lo, hi);
+      Rf[n] = lo;
+      Lf[n] = hi;
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      const double qm1 = __shfl_up(q[n], 1, 64), qp1 = __shfl_down(q[n], 1, 64);
+      double qm2 = qm1, qp2 = qp1;
+      if constexpr (W) {
+        qm2 = __shfl_up(q[n], 2, 64);
+        qp2 = __shfl_down(q[n], 2, 64);
+      }
+      cell_halves<double, RC>(qm2, qm1, q[n], qp1, qp2, Rf[n], Lf[n]);
+    }
+  }
+  if constexpr (W) {
+    if (a.flatten) {
+      const double p = q[4];
+      const double pm2 = __shfl_up(p, 2, 64), pm1 = __shfl_up(p, 1, 64);
+      const double pp1 = __shfl_down(p, 1, 64), pp2 = __shfl_down(p, 2, 64);
+      const double um1 = __shfl_up(q[1], 1, 64), up1 = __shfl_down(q[1], 1, 64);
+      fco = R::flat(pm2, pm1, p, pp1, pp2, um1, up1);
+    }
+  }
+  finish_cell<double, NV>(Lf, Rf, q, fco, W && a.flatten, a.eos.dfloor, a.eos.pfloor);
+  // face i = cell index of this lane: left state = Lf of the lane below
+  double Lface[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
+  const int i = F0 - H + lane;
+  if (lane < H || lane >= H + FPW || i > a.iu) return;
+  solve_store<0, RS, ENT, NV>(a, row0 + i, 1, Lface, Rf);
+}
+
+// ---------------------------------------------------------------- y/z faces: register march
+template <int DIR, Recon RC, RSolver RS, bool ENT>
+__global__ void __launch_bounds__(BX * BY) k_flux_m(const FluxArgs a) {
+  constexpr int NV = ENT ? 8 : 7;
+  constexpr bool W = wide_v<RC>;
+  if (!a.ctrl->active) return;
+  const Geom &g = a.g;
+  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int t = int(blockIdx.y) * BY + int(threadIdx.y);     // transverse index: k (DIR 1) or j (DIR 2)
+  const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
+  const int F0 = lo_d + int(blockIdx.z) * SEG;
+  const int F1 = min(F0 + SEG - 1, hi_d);
+  if (i > a.iu) return;
+  int j, k;
+  if constexpr (DIR == 1) { k = a.kl + t; j = 0; if (k > a.ku) return; }
+  else { j = a.jl + t; k = 0; if (j > a.ju) return; }
+  const long long sd = DIR == 1 ? g.pitch : g.plane;
+  const long long base = DIR == 1 ? g.at(k, 0, i) : g.at(0, j, i);   // index of (d = 0)
+  const double *src[NV];
+  var_ptrs<DIR, NV>(a, src);
+  using R = Rec<double>;
+  const bool flat = W && a.flatten;
+  const double dfl = a.eos.dfloor, pfl = a.eos.pfloor;
+  // window around the cell being finished, c: wm2 = q_{c-2}, wm1 = q_{c-1}, w0 = q_c, wp1 = q_{c+1}
+  // (PPM keeps d_c and e_{c-1/2} instead of q_{c-2}, q_{c-1}, plus p_{c-2}, p_{c-1}, u_{c-1})
+  double w0[NV], wp1[NV], x1[NV], x2[NV];
+  double pm2 = 0.0, pm1 = 0.0, um1 = 0.0;
+  int c = F0 - 1;
+  {
+    const long long cc = base + (long long)c * sd;
+    double qm2[NV], qm1[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      qm1[n] = src[n][cc - sd];
+      w0[n] = src[n][cc];
+      wp1[n] = src[n][cc + sd];
+      qm2[n] = W ? src[n][cc - 2 * sd] : qm1[n];
+    }
+    if constexpr (RC == Recon::ppm) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        const double dm = R::mc(qm2[n], qm1[n], w0[n]);
+        x1[n] = R::mc(qm1[n], w0[n], wp1[n]);            // d_c
+        x2[n] = R::edge(qm1[n], w0[n], dm, x1[n]);       // e_{c-1/2}
+      }
+      pm2 = qm2[4];
+      pm1 = qm1[4];
+      um1 = qm1[1];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { x1[n] = qm2[n]; x2[n] = qm1[n]; }
+    }
+  }
+  double Lprev[NV];
+  for (; c <= F1; ++c) {
+    // finish cell c: needs q_{c+2} (wide) or q_{c+1} (narrow, already in wp1)
+    const long long cc = base + (long long)c * sd;
+    double qn[NV];
+    if constexpr (W) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) qn[n] = src[n][cc + 2 * sd];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) qn[n] = wp1[n];
+    }
+    double Lf[NV], Rf[NV], fco = 0.0;
+    if constexpr (RC == Recon::ppm) {
+      double dn[NV], en[NV];
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        dn[n] = R::mc(w0[n], wp1[n], qn[n]);             // d_{c+1}
+        en[n] = R::edge(w0[n], wp1[n], x1[n], dn[n]);    // e_{c+1/2}
+        double lo = x2[n], hi = en[n];
+        R::mono(w0[n], lo, hi);
+        Rf[n] = lo;
+        Lf[n] = hi;
+      }
+      if (flat) fco = R::flat(pm2, pm1, w0[4], wp1[4], qn[4], um1, wp1[1]);
+      pm2 = pm1;
+      pm1 = w0[4];
+      um1 = w0[1];
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { x1[n] = dn[n]; x2[n] = en[n]; }
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) cell_halves<double, RC>(x1[n], x2[n], w0[n], wp1[n], qn[n], Rf[n], Lf[n]);
+      if (flat) fco = R::flat(x1[4], x2[4], w0[4], wp1[4], qn[4], x2[1], wp1[1]);
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { x1[n] = x2[n]; x2[n] = w0[n]; }
+    }
+    double qc[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) qc[n] = w0[n];
+    finish_cell<double, NV>(Lf, Rf, qc, fco, flat, dfl, pfl);
+    if (c >= F0) solve_store<DIR, RS, ENT, NV>(a, cc, sd, Lprev, Rf);   // face c (between c-1 and c)
+    // advance the window
+    if constexpr (W) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { w0[n] = wp1[n]; wp1[n] = qn[n]; }
+    } else if (c < F1) {   // narrow stencil: the next cell's right neighbour
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { w0[n] = wp1[n]; wp1[n] = src[n][cc + 2 * sd]; }
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) Lprev[n] = Lf[n];
+  }
+}
+
+// Donor-cell HLLE flux and face EMFs of one face from the stage-input primitives
+// (fofc.hpp:76-88).
+template <int DIR>
+__device__ __forceinline__ void repair_face(const FluxArgs &a, long long c, long long sd) {
+  const Geom &g = a.g;
   constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
   constexpr int B1 = (DIR + 1) % 3, B2 = (DIR + 2) % 3;
-  const int kL = k - dk, jL = j - dj, iL = i - di;
-  auto inside = [&](int kk, int jj, int ii) {
-    return ii >= g.is && ii <= g.ie && jj >= g.js && jj <= g.je && kk >= g.ks && kk <= g.ke;
-  };
-  const long long c = g.at(k, j, i), cL = g.at(kL, jL, iL), cs = g.cstride;
-  const bool fl = inside(k, j, i) && a.flag[c];
-  const bool fr = inside(kL, jL, iL) && a.flag[cL];
-  if (!(fl || fr)) return;
+  const long long cL = c - sd, cs = g.cstride;
   const double *w = a.w, *b = a.bcc;
   St<double> L{w[IDN * cs + cL], w[IU * cs + cL], w[IV * cs + cL], w[IW * cs + cL], w[IPR * cs + cL],
                b[B1 * cs + cL], b[B2 * cs + cL]};
@@ -199,12 +318,62 @@ __global__ void __launch_bounds__(BX * BY) k_repair(const FluxArgs a, const int 
   a.et1[c] = f[FBZ];
 }
 
+// RepairFluxes<DIR> over the whole face range with the flag array (fofc.hpp:66-94).
+template <int DIR>
+__global__ void __launch_bounds__(BX * BY) k_repair(const FluxArgs a, const int only_if_bad) {
+  if (!a.ctrl->active) return;
+  if (only_if_bad && a.ctrl->nbad == 0) return;
+  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = a.jl + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int k = a.kl + int(blockIdx.z);
+  if (i > a.iu || j > a.ju) return;
+  const Geom &g = a.g;
+  constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  const int kL = k - dk, jL = j - dj, iL = i - di;
+  auto inside = [&](int kk, int jj, int ii) {
+    return ii >= g.is && ii <= g.ie && jj >= g.js && jj <= g.je && kk >= g.ks && kk <= g.ke;
+  };
+  const long long c = g.at(k, j, i), cL = g.at(kL, jL, iL);
+  const bool fl = inside(k, j, i) && a.flag[c];
+  const bool fr = inside(kL, jL, iL) && a.flag[cL];
+  if (!(fl || fr)) return;
+  repair_face<DIR>(a, c, c - cL);
+}
+
+// The same repair driven by the list of flagged (active) cells that con2prim builds: every
+// face touching a listed cell — its two faces in each direction — is recomputed. A face
+// shared by two listed cells is written twice with the same value.
+__global__ void __launch_bounds__(256) k_repair_list(const FluxArgs a0, const FluxArgs a1, const FluxArgs a2,
+                                                     const int *list, const int ndir) {
+  const Ctrl *ctl = a0.ctrl;
+  if (!ctl->active || ctl->nbad == 0) return;
+  const Geom &g = a0.g;
+  const int per = 2 * ndir;
+  const int n = ctl->nbad * per;
+  for (int t = int(blockIdx.x) * 256 + int(threadIdx.x); t < n; t += int(gridDim.x) * 256) {
+    const long long c0 = list[t / per];
+    const int r = t % per, dir = r >> 1, side = r & 1;
+    if (dir == 0) repair_face<0>(a0, c0 + side, 1);
+    else if (dir == 1) repair_face<1>(a1, c0 + side * (long long)g.pitch, g.pitch);
+    else repair_face<2>(a2, c0 + side * g.plane, g.plane);
+  }
+}
+
 template <int DIR, Recon RC, RSolver RS, bool ENT>
 hipError_t go_flux(const FluxArgs &a, hipStream_t s) {
   const int nx = a.iu - a.il + 1, ny = a.ju - a.jl + 1, nz = a.ku - a.kl + 1;
   if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
-  dim3 grid(div_up(nx, BX), div_up(ny, BY), nz), block(BX, BY, 1);
-  hipLaunchKernelGGL((k_flux<DIR, RC, RS, ENT>), grid, block, 0, s, a);
+  if constexpr (DIR == 0) {
+    constexpr bool W = wide_v<RC>;
+    constexpr int FPW = 64 - (W ? 3 : 2) - (W ? 2 : 1);
+    const int nseg = div_up(nx, FPW);
+    const int nwaves = nseg * ny * nz;
+    hipLaunchKernelGGL((k_flux_x<RC, RS, ENT>), dim3(div_up(nwaves, 4)), dim3(256), 0, s, a, nseg, nwaves);
+  } else {
+    const int nt = DIR == 1 ? nz : ny, nd = DIR == 1 ? ny : nz;
+    dim3 grid(div_up(nx, BX), div_up(nt, BY), div_up(nd, SEG)), block(BX, BY, 1);
+    hipLaunchKernelGGL((k_flux_m<DIR, RC, RS, ENT>), grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -234,6 +403,11 @@ hipError_t go_flux_dir(const FluxArgs &a, Recon rc, RSolver rs, bool ent, hipStr
 }
 
 }  // namespace
+
+hipError_t launch_repair_list(const FluxArgs a[3], int ndir, const int *list, hipStream_t s) {
+  hipLaunchKernelGGL(k_repair_list, dim3(256), dim3(256), 0, s, a[0], a[ndir > 1 ? 1 : 0], a[ndir > 2 ? 2 : 0], list, ndir);
+  return hipGetLastError();
+}
 
 hipError_t launch_flux(const FluxArgs &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s) {
   if (dir == 0) return go_flux_dir<0>(a, rc, rs, entropy, s);

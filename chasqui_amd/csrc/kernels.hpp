@@ -63,10 +63,12 @@ struct C2pArgs {
   double *u;
   double *w, *bcc, *ecc;
   int write_went;              // also store w[IENT] (needed when entropy is transported)
+  int entropy_out;             // resynchronise u[IENT] = rho p / rho^gamma
   int compute_dt;
   int slot;                    // which running minimum receives the CFL dt (0 or 1)
-  int flag_mode;               // 0: none; 1: fused fofc flag + count (tentative state)
+  int flag_mode;               // 0: none; 1: flag array + count; 2: append flagged cells to badlist
   int *flag;
+  int *badlist;
   Ctrl *ctrl;
   int only_if_bad;
 };
@@ -74,7 +76,9 @@ struct C2pArgs {
 // launches (defined in stage.hip / flux.hip)
 hipError_t launch_flux(const FluxArgs &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s);
 hipError_t launch_repair(const FluxArgs &a, int dir, bool only_if_bad, hipStream_t s);
+hipError_t launch_repair_list(const FluxArgs a[3], int ndir, const int *list, hipStream_t s);
 hipError_t launch_corner(const CornerArgs &a, hipStream_t s);
+hipError_t launch_corner_list(const CornerArgs &a, const int *list, hipStream_t s);
 hipError_t launch_update(const UpdateArgs &a, hipStream_t s);
 hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s);
 hipError_t launch_c2p(const C2pArgs &a, hipStream_t s);

@@ -23,6 +23,23 @@ __device__ __forceinline__ bool gate(const Ctrl *c, int only_if_bad) {
   return true;
 }
 
+// Blocks iterate over "virtual blocks" (vb -> bx, by, bz of a gx x gy x gz tile grid): a
+// normal launch has one block per tile; the launches of the fofc branch, which usually
+// exit at once, use a small grid-stride grid so an empty branch costs a few microseconds.
+// A virtual block is a chunk of 256 consecutive points of one (j, i) plane of an nx x ny
+// box (rows are linearised, so rows of 257 or 262 points waste no lanes), times nz planes.
+struct VB {
+  int nx, ny, cpp, n;   // plane size, chunks per plane, total chunks
+  __device__ __forceinline__ bool at(int vb, int &k, int &j, int &i) const {
+    k = vb / cpp;
+    const int t = (vb - k * cpp) * 256 + int(threadIdx.x);
+    j = t / nx;
+    i = t - j * nx;
+    return t < nx * ny;
+  }
+};
+constexpr int COND_BLOCKS = 16384;   // grid of the conditional (fofc-branch) launches
+
 // ---------------------------------------------------------------- corner EMF
 __device__ __forceinline__ double upw(double flux, double left, double right) {   // physics.hpp:233-236
   const double tol = Lit<double>::upwind_tol;
@@ -31,13 +48,8 @@ __device__ __forceinline__ double upw(double flux, double left, double right) { 
 
 // 3D: one thread per (k, j, i) of the union box [ks..ke+1] x [js..je+1] x [is..ie+1];
 // e1 for i <= ie, e2 for j <= je, e3 for k <= ke (physics.hpp:265-296).
-__global__ void __launch_bounds__(BX * BY) k_corner3(const CornerArgs a) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
+__device__ __forceinline__ void corner3_at(const CornerArgs &a, int k, int j, int i) {
   const Geom &g = a.g;
-  const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int k = g.ks + int(blockIdx.z);
-  if (i > g.ie + 1 || j > g.je + 1) return;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
   const double *ecc = a.ecc;
   if (i <= g.ie) {   // E1 at (k-1/2, j-1/2, i)
@@ -68,13 +80,9 @@ __global__ void __launch_bounds__(BX * BY) k_corner3(const CornerArgs a) {
 
 // 2D and 1D: E1/E2 are copies of the face EMFs on the two k-planes (physics.hpp:250-256);
 // E3 is the Gardiner-Stone average in 2D (288-296) or a copy in 1D (257-261).
-__global__ void __launch_bounds__(BX * BY) k_corner12(const CornerArgs a) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
+__device__ __forceinline__ void corner12_at(const CornerArgs &a, int j, int i) {
   const Geom &g = a.g;
-  const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
   const int k = g.ks;
-  if (i > g.ie + 1 || j > g.je + 1) return;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
   a.e1[c] = a.e1x2[c];
   a.e1[c + sk] = a.e1x2[c];
@@ -93,18 +101,49 @@ __global__ void __launch_bounds__(BX * BY) k_corner12(const CornerArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(BX * BY) k_corner(const CornerArgs a, const VB v) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+    int k, j, i;
+    if (!v.at(vb, k, j, i)) continue;
+    if (g.three_d) corner3_at(a, g.ks + k, g.js + j, g.is + i);
+    else corner12_at(a, g.js + j, g.is + i);
+  }
+}
+
+// Recompute the edge EMFs that can depend on a repaired face: the corners (k..k+1, j..j+1,
+// i..i+1) of every listed (flagged) cell; every other edge keeps its value, which is what a
+// whole-domain recompute would produce from unchanged inputs. Overlapping cells write the
+// same value to a shared edge.
+__global__ void __launch_bounds__(256) k_corner_list(const CornerArgs a, const int *list) {
+  if (!gate(a.ctrl, 1)) return;
+  const Geom &g = a.g;
+  const int n = a.ctrl->nbad * 8;
+  for (int t = int(blockIdx.x) * 256 + int(threadIdx.x); t < n; t += int(gridDim.x) * 256) {
+    const long long c0 = list[t >> 3];
+    const int o = t & 7;
+    const long long r = c0 - g.ioff;
+    const int k = int(r / g.plane), j = int((r % g.plane) / g.pitch), i = int(r % g.pitch);
+    const int kk = k + ((o >> 2) & 1), jj = j + ((o >> 1) & 1), ii = i + (o & 1);
+    if (g.three_d) {
+      // the edge kernel's union box and per-component ranges (physics.hpp:266, 277, 288)
+      if (kk > g.ke + 1 || jj > g.je + 1 || ii > g.ie + 1) continue;
+      corner3_at(a, kk, jj, ii);
+    } else {
+      if (kk != g.ks || jj > g.je + 1 || ii > g.ie + 1) continue;
+      if (!g.two_d && jj != g.js) continue;
+      corner12_at(a, jj, ii);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- update
 // One thread per (k, j, i) of [ks..ke+1] x [js..je+1] x [is..ie+1]: the conserved update of
 // the active cell (physics.hpp:314-322) and the CT update of its three lower faces
 // (physics.hpp:327-347). In-place (uo aliasing ua or ub) is pointwise safe.
-__global__ void __launch_bounds__(BX * BY) k_update(const UpdateArgs a) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
+__device__ __forceinline__ void update_at(const UpdateArgs &a, const double bdt, int k, int j, int i) {
   const Geom &g = a.g;
-  const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int k = g.ks + int(blockIdx.z);
-  if (i > g.ie + 1 || j > g.je + 1) return;
-  const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
   const double g0 = a.g0, g1 = a.g1;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
   const double idx1 = g.idx1, idx2 = g.idx2, idx3 = g.idx3;
@@ -136,6 +175,17 @@ __global__ void __launch_bounds__(BX * BY) k_update(const UpdateArgs a) {
     const double v = g0 * a.b3a[c] + g1 * a.b3b[c] - bdt * cr;
     a.b3o[c] = v;
     if (!g.three_d) a.b3o[c + sk] = v;
+  }
+}
+
+__global__ void __launch_bounds__(BX * BY) k_update(const UpdateArgs a, const VB v) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+    int k, j, i;
+    if (!v.at(vb, k, j, i)) continue;
+    update_at(a, bdt, g.ks + k, g.js + j, g.is + i);
   }
 }
 
@@ -263,93 +313,114 @@ __device__ __forceinline__ int wave_sum(int v) {
 // cons -> prim over all cells, physics.hpp:437-494, with the CFL minimum reduced by wave64
 // shuffles, one LDS pass and one 64-bit atomic per block. flag_mode 1 adds the fofc flag of
 // the (tentative) active cells with the expressions of fofc.hpp:24-40.
-__global__ void __launch_bounds__(BX * BY) k_c2p(const C2pArgs a) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
+// flag_mode 1 writes the flag array (reference semantics); 2 appends flagged cells to a list.
+// DT: reduce the CFL minimum. ENT: resynchronise u[IENT] (and w[IENT] if write_went); the
+// lean variant (no pow) serves RK stages whose u[IENT] nothing reads.
+template <bool DT, bool ENT>
+__device__ __forceinline__ void c2p_at(const C2pArgs &a, int k, int j, int i, double &mdt, int &nbad_local) {
   const Geom &g = a.g;
-  const int i = int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = int(blockIdx.y) * BY + int(threadIdx.y);
-  const int k = int(blockIdx.z);
   const Eos e = a.eos;
-  double mdt = 1e300;
-  int bad_count = 0;
-  if (i < g.ni && j < g.nj) {
-    const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
-    const double bx = 0.5 * (a.b1[c] + a.b1[c + 1]);
-    const double by = g.two_d ? 0.5 * (a.b2[c] + a.b2[c + sj]) : a.b2[c];
-    const double bz = g.three_d ? 0.5 * (a.b3[c] + a.b3[c + sk]) : a.b3[c];
-    double *u = a.u + c;
-    const double u0 = u[IDN * cs], u1 = u[IM1 * cs], u2 = u[IM2 * cs], u3 = u[IM3 * cs], u4 = u[IEN * cs];
-    const bool active = i >= g.is && i <= g.ie && j >= g.js && j <= g.je && k >= g.ks && k <= g.ke;
-    if (a.flag_mode && active) {   // fofc.hpp:24-40 on the tentative state
-      bool bad = u0 <= e.dfloor;
-      if (!bad) {
-        const double kin = 0.5 * (sq(u1) + sq(u2) + sq(u3)) / u0;
-        const double ei = u4 - kin - 0.5 * (bx * bx + by * by + bz * bz);
-        double p = e.gm1 * ei;
-        if (e.dual_energy && ei < e.de_eta * fabs(u4)) p = (u[IENT * cs] / u0) * pow(u0, e.gamma);
-        bad = p <= e.pfloor;
-      }
-      a.flag[c] = bad ? 1 : 0;
-      bad_count = bad ? 1 : 0;
+  const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
+  const double bx = 0.5 * (a.b1[c] + a.b1[c + 1]);
+  const double by = g.two_d ? 0.5 * (a.b2[c] + a.b2[c + sj]) : a.b2[c];
+  const double bz = g.three_d ? 0.5 * (a.b3[c] + a.b3[c + sk]) : a.b3[c];
+  double *u = a.u + c;
+  const double u0 = u[IDN * cs], u1 = u[IM1 * cs], u2 = u[IM2 * cs], u3 = u[IM3 * cs], u4 = u[IEN * cs];
+  const bool active = i >= g.is && i <= g.ie && j >= g.js && j <= g.je && k >= g.ks && k <= g.ke;
+  if (a.flag_mode && active) {   // fofc.hpp:24-40 on the tentative state
+    bool bad = u0 <= e.dfloor;
+    if (!bad) {
+      const double kin = 0.5 * (sq(u1) + sq(u2) + sq(u3)) / u0;
+      const double ei = u4 - kin - 0.5 * (bx * bx + by * by + bz * bz);
+      double p = e.gm1 * ei;
+      if (e.dual_energy && ei < e.de_eta * fabs(u4)) p = (u[IENT * cs] / u0) * pow(u0, e.gamma);
+      bad = p <= e.pfloor;
     }
-    const double d = fmax(u0, e.dfloor);
-    const double id = 1.0 / d;
-    const double vx = u1 * id, vy = u2 * id, vz = u3 * id;
-    const double bsq = bx * bx + by * by + bz * bz;
-    const double kin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
-    const double ei = u4 - kin - 0.5 * bsq;
-    double p = e.gm1 * ei;
-    double se;
-    if (e.dual_energy && ei < e.de_eta * fabs(u4)) {
-      se = u[IENT * cs] * id;
-      p = se * pow(d, e.gamma);
-      p = fmax(p, e.pfloor);
-      u[IEN * cs] = kin + 0.5 * bsq + p / e.gm1;
-    } else {
-      p = fmax(p, e.pfloor);
+    if (a.flag_mode == 1) {
+      a.flag[c] = bad ? 1 : 0;
+      nbad_local += bad ? 1 : 0;
+    } else if (bad) {
+      const int pos = atomicAdd(&a.ctrl->nbad, 1);
+      a.badlist[pos] = int(c);
+    }
+  }
+  const double d = fmax(u0, e.dfloor);
+  const double id = 1.0 / d;
+  const double vx = u1 * id, vy = u2 * id, vz = u3 * id;
+  const double bsq = bx * bx + by * by + bz * bz;
+  const double kin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+  const double ei = u4 - kin - 0.5 * bsq;
+  double p = e.gm1 * ei;
+  double se = 0.0;
+  if (ENT && e.dual_energy && ei < e.de_eta * fabs(u4)) {
+    se = u[IENT * cs] * id;
+    p = se * pow(d, e.gamma);
+    p = fmax(p, e.pfloor);
+    u[IEN * cs] = kin + 0.5 * bsq + p / e.gm1;
+  } else {
+    p = fmax(p, e.pfloor);
+    if constexpr (ENT) {
       se = p / pow(d, e.gamma);
       u[IENT * cs] = d * se;
     }
-    double *w = a.w + c;
-    w[IDN * cs] = d;
-    w[IVX * cs] = vx;
-    w[IVY * cs] = vy;
-    w[IVZ * cs] = vz;
-    w[IPR * cs] = p;
-    if (a.write_went) w[IENT * cs] = se;
-    a.bcc[c] = bx;
-    a.bcc[c + cs] = by;
-    a.bcc[c + 2 * cs] = bz;
-    a.ecc[c] = vz * by - vy * bz;
-    a.ecc[c + cs] = vx * bz - vz * bx;
-    a.ecc[c + 2 * cs] = vy * bx - vx * by;
-    if (a.compute_dt && active) {
-      const double gp = e.gamma * p;
-      const double aa = (gp + bsq) * id;
-      auto cf = [&](double b) {
-        const double disc = aa * aa - 4.0 * gp * b * b * id * id;
-        return sqrt(0.5 * (aa + sqrt(fmax(disc, 0.0))));
-      };
-      double dt = g.dx1 / (fabs(vx) + cf(bx));
-      if (g.two_d) dt = fmin(dt, g.dx2 / (fabs(vy) + cf(by)));
-      if (g.three_d) dt = fmin(dt, g.dx3 / (fabs(vz) + cf(bz)));
-      mdt = dt;
-    }
   }
-  if (!a.compute_dt && !a.flag_mode) return;
+  double *w = a.w + c;
+  w[IDN * cs] = d;
+  w[IVX * cs] = vx;
+  w[IVY * cs] = vy;
+  w[IVZ * cs] = vz;
+  w[IPR * cs] = p;
+  if (ENT && a.write_went) w[IENT * cs] = se;
+  a.bcc[c] = bx;
+  a.bcc[c + cs] = by;
+  a.bcc[c + 2 * cs] = bz;
+  a.ecc[c] = vz * by - vy * bz;
+  a.ecc[c + cs] = vx * bz - vz * bx;
+  a.ecc[c + 2 * cs] = vy * bx - vx * by;
+  if (DT && active) {
+    const double gp = e.gamma * p;
+    const double aa = (gp + bsq) * id;
+    auto cf = [&](double b) {
+      const double disc = aa * aa - 4.0 * gp * b * b * id * id;
+      return sqrt(0.5 * (aa + sqrt(fmax(disc, 0.0))));
+    };
+    double dt = g.dx1 / (fabs(vx) + cf(bx));
+    if (g.two_d) dt = fmin(dt, g.dx2 / (fabs(vy) + cf(by)));
+    if (g.three_d) dt = fmin(dt, g.dx3 / (fabs(vz) + cf(bz)));
+    mdt = fmin(mdt, dt);
+  }
+}
+
+// cons -> prim over all cells, physics.hpp:437-494. The CFL minimum is reduced with wave64
+// shuffles and one LDS pass per block, then merged with a 64-bit atomicMin on the bit
+// pattern, issued only when the block's minimum beats the current one.
+template <bool DT, bool ENT>
+__global__ void __launch_bounds__(BX * BY) k_c2p(const C2pArgs a, const VB v) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  double mdt = 1e300;
+  int nbad_local = 0;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+    int k, j, i;
+    if (v.at(vb, k, j, i)) c2p_at<DT, ENT>(a, k, j, i, mdt, nbad_local);
+  }
+  if (!DT && a.flag_mode != 1) return;
   __shared__ double smin[BX * BY / 64];
   __shared__ int ssum[BX * BY / 64];
-  const int lane = int(threadIdx.x) & 63, wid = (int(threadIdx.y) * BX + int(threadIdx.x)) >> 6;
+  const int lane = int(threadIdx.x) & 63, wid = int(threadIdx.x) >> 6;
   const double wm = wave_min(mdt);
-  const int ws = wave_sum(bad_count);
+  const int ws = wave_sum(nbad_local);
   if (lane == 0) { smin[wid] = wm; ssum[wid] = ws; }
   __syncthreads();
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
+  if (threadIdx.x == 0) {
     double m = smin[0];
     int s = ssum[0];
     for (int q = 1; q < BX * BY / 64; ++q) { m = fmin(m, smin[q]); s += ssum[q]; }
-    if (a.compute_dt && m < 1e300) atomicMin(&a.ctrl->mdt_bits[a.slot], (unsigned long long)__double_as_longlong(m));
-    if (a.flag_mode && s) atomicAdd(&a.ctrl->nbad, s);
+    if (DT && m < 1e300) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+      if (bits < __atomic_load_n(&a.ctrl->mdt_bits[a.slot], __ATOMIC_RELAXED)) atomicMin(&a.ctrl->mdt_bits[a.slot], bits);
+    }
+    if (a.flag_mode == 1 && s) atomicAdd(&a.ctrl->nbad, s);
   }
 }
 
@@ -417,24 +488,34 @@ __global__ void k_end_step(Ctrl *c) {
 
 }  // namespace
 
+namespace {
+VB vgrid(int nx, int ny, int nz) {
+  VB v;
+  v.nx = nx;
+  v.ny = ny;
+  v.cpp = div_up(nx * ny, 256);
+  v.n = v.cpp * nz;
+  return v;
+}
+int grid_of(const VB &v, bool conditional) { return conditional ? (v.n < COND_BLOCKS ? v.n : COND_BLOCKS) : v.n; }
+}  // namespace
+
 hipError_t launch_corner(const CornerArgs &a, hipStream_t s) {
   const Geom &g = a.g;
-  dim3 block(BX, BY, 1);
-  if (g.three_d) {
-    dim3 grid(div_up(g.ie - g.is + 2, BX), div_up(g.je - g.js + 2, BY), g.ke - g.ks + 2);
-    hipLaunchKernelGGL(k_corner3, grid, block, 0, s, a);
-  } else {
-    dim3 grid(div_up(g.ie - g.is + 2, BX), div_up(g.je - g.js + 2, BY), 1);
-    hipLaunchKernelGGL(k_corner12, grid, block, 0, s, a);
-  }
+  const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, g.three_d ? g.ke - g.ks + 2 : 1);
+  hipLaunchKernelGGL(k_corner, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_corner_list(const CornerArgs &a, const int *list, hipStream_t s) {
+  hipLaunchKernelGGL(k_corner_list, dim3(256), dim3(256), 0, s, a, list);
   return hipGetLastError();
 }
 
 hipError_t launch_update(const UpdateArgs &a, hipStream_t s) {
   const Geom &g = a.g;
-  dim3 block(BX, BY, 1);
-  dim3 grid(div_up(g.ie - g.is + 2, BX), div_up(g.je - g.js + 2, BY), g.ke - g.ks + 2);
-  hipLaunchKernelGGL(k_update, grid, block, 0, s, a);
+  const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, g.ke - g.ks + 2);
+  hipLaunchKernelGGL(k_update, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
   return hipGetLastError();
 }
 
@@ -453,8 +534,16 @@ hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s) {
 
 hipError_t launch_c2p(const C2pArgs &a, hipStream_t s) {
   const Geom &g = a.g;
-  dim3 grid(div_up(g.ni, BX), div_up(g.nj, BY), g.nk), block(BX, BY, 1);
-  hipLaunchKernelGGL(k_c2p, grid, block, 0, s, a);
+  const VB v = vgrid(g.ni, g.nj, g.nk);
+  const dim3 grid(grid_of(v, a.only_if_bad)), block(256);
+  const bool ent = a.entropy_out || a.eos.dual_energy;
+  if (a.compute_dt) {
+    if (ent) hipLaunchKernelGGL((k_c2p<true, true>), grid, block, 0, s, a, v);
+    else hipLaunchKernelGGL((k_c2p<true, false>), grid, block, 0, s, a, v);
+  } else {
+    if (ent) hipLaunchKernelGGL((k_c2p<false, true>), grid, block, 0, s, a, v);
+    else hipLaunchKernelGGL((k_c2p<false, false>), grid, block, 0, s, a, v);
+  }
   return hipGetLastError();
 }
 
