@@ -11,7 +11,9 @@ pipeline's stream, after W warm-up steps, bracketed by stream synchronisation.
 Output: one JSON line. `roofline` is for the dominant kernel (the face-flux kernel),
 timed with HIP events on its stream inside the timed region; `step_roofline` applies the
 reference's traffic model (1040 B per cell per RK stage, README.md:45-47) to the whole step;
-`cpu_baseline` times the CPU oracle on this host's cores on one step of the same workload.
+`cpu_baseline` times the CPU oracle on this host's cores on one step of the same workload;
+`mixed` is the same workload on the mixed-precision path (FP32 reconstruction, Riemann solve
+and EMFs; FP64 con2prim, RK combine, CT), reported beside the FP64 headline.
 """
 import argparse
 import json
@@ -90,6 +92,32 @@ def cpu_baseline(n, steps):
                       f"{wall:.1f} s wall (oracle/, -O3 -ffp-contract=off)"}
 
 
+def run(n, precision, steps, warmup, breakdown=False):
+    """K timed steps of the device-resident pipeline on the n^3 blast (after W warm-up steps);
+    returns (ms, flux kernel ms, flux launches, status, per-kernel breakdown)."""
+    from chasqui_amd.deck import Deck
+    deck = Deck(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}", f"mhd.precision={precision}"])
+    mesh = deck.make_mesh()
+    deck.close()
+    mesh.step(warmup)
+    mesh.sync()
+    mesh.kernel_timing("flux")
+    ms = mesh.step_timed(steps)
+    flux_ms, flux_launches = mesh.kernel_timing_read()
+    mesh.kernel_timing(None)
+    st = mesh.status()
+    kernels = {}
+    if breakdown:   # per-kernel breakdown on two further (untimed) steps
+        for cls in ("flux", "corner", "update", "bc", "c2p", "repair"):
+            mesh.kernel_timing(cls)
+            mesh.step(2)
+            kms, kn = mesh.kernel_timing_read()
+            kernels[cls] = {"ms_per_step": kms / 2, "launches_per_step": kn // 2}
+        mesh.kernel_timing(None)
+    mesh.close()
+    return ms, flux_ms, flux_launches, st, kernels
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,35 +126,15 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="cells per side (256 = the headline workload)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision side measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one mesh per device; "
               "sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from chasqui_amd.deck import Deck
     n = args.n
-    deck = Deck(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}"])
-    mesh = deck.make_mesh()
-    deck.close()
-    mesh.step(args.warmup)
-    mesh.sync()
-    mesh.kernel_timing("flux")
-    ms = mesh.step_timed(args.steps)
-    flux_ms, flux_launches = mesh.kernel_timing_read()
-    mesh.kernel_timing(None)
-    st = mesh.status()
-
-    # per-kernel breakdown on two further (untimed) steps
-    breakdown = {}
-    for cls in ("flux", "corner", "update", "bc", "c2p", "repair"):
-        mesh.kernel_timing(cls)
-        mesh.step(2)
-        kms, kn = mesh.kernel_timing_read()
-        breakdown[cls] = {"ms_per_step": kms / 2, "launches_per_step": kn // 2}
-    mesh.kernel_timing(None)
-    mesh.close()
-
+    ms, flux_ms, flux_launches, st, breakdown = run(n, "fp64", args.steps, args.warmup, breakdown=True)
     zone_updates = float(n) ** 3 * args.steps
     value = zone_updates / (ms * 1e-3)
     avg_flux_s = flux_ms * 1e-3 / max(flux_launches, 1)
@@ -160,6 +168,14 @@ def main():
                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": model / HBM_PEAK},
         "kernels": breakdown,
     }
+    if not args.no_mixed:
+        # the same workload with FP32 reconstruction/Riemann/EMF and FP64 con2prim/RK/CT
+        mms, mflux, mlaunch, mst, mk = run(n, "mixed", args.steps, args.warmup, breakdown=True)
+        mval = zone_updates / (mms * 1e-3)
+        out["mixed"] = {"value": mval, "unit": "zone-updates/s", "dtype": "f32 flux/EMF, f64 state",
+                        "ms_per_step": mms / args.steps, "speedup_vs_fp64": mval / value,
+                        "flux_avg_launch_ms": mflux / max(mlaunch, 1),
+                        "fofc_corrections_in_run": mst["fofc_total"], "kernels": mk}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     print(json.dumps(out))

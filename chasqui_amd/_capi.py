@@ -23,6 +23,7 @@ RECON = {"dc": 0, "plm": 1, "ppm": 2, "wenoz": 3, "ppmx": 4}
 RSOLVER = {"hlle": 0, "hlld": 1, "lhlld": 2}
 INTEGRATOR = {"rk1": 1, "rk2": 2, "rk3": 3, "rk4": 4, "ssprk43": 4}
 BC = {"periodic": 0, "outflow": 1, "reflect": 2}
+PRECISION = {"fp64": 0, "mixed": 1}
 KERNEL_CLASSES = {"flux": 0, "corner": 1, "update": 2, "bc": 3, "c2p": 4, "repair": 5}
 
 
@@ -37,7 +38,7 @@ class Config(C.Structure):
                 ("gamma", C.c_double), ("dfloor", C.c_double), ("pfloor", C.c_double),
                 ("cfl", C.c_double), ("flatten", C.c_int), ("low_mach", C.c_int),
                 ("fofc", C.c_int), ("dual_energy", C.c_int), ("de_eta", C.c_double),
-                ("lhlld_phi", C.c_double), ("bc", C.c_int * 3)]
+                ("lhlld_phi", C.c_double), ("bc", C.c_int * 3), ("precision", C.c_int)]
 
 
 class Status(C.Structure):

@@ -3,7 +3,8 @@ C ABI + C++ host layer) and the `chasqui_amd` driver executable, in-tree.
 
 Device code is compiled with -ffp-contract=off (no FMA contraction: the FP64 path is
 bitwise equal to the reference method, which is built the same way) and without packed
-FP32 instructions.
+FP32 instructions. The FP32 flux kernels of the mixed-precision path (flux32.hip) are built
+with FMA contraction and the fast FP32 divide / square root.
 """
 import os
 import subprocess
@@ -25,7 +26,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-HIP_SOURCES = ["flux.hip", "stage.hip", "layout.hip"]
+HIP_SOURCES = ["flux.hip", "flux32.hip", "stage.hip", "layout.hip"]
+EXTRA = {"flux32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 HOST_SOURCES = ["capi.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]
 
 
@@ -33,7 +35,7 @@ def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src.replace("/", "_") + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC] + DEVICE + COMMON + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + DEVICE + COMMON + EXTRA.get(src, []) + ["-c", path, "-o", obj]
     else:
         cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -49,6 +49,14 @@ Coef rk_coef(int nstage, int s) {
 }
 }  // namespace
 
+// The arrays the flux / corner-EMF side works in, per precision: double for the FP64 path,
+// float for the mixed-precision path (the conserved state and face B are always double).
+template <class T>
+struct FieldSet {
+  T *w[2]{}, *bcc[2]{}, *ecc[2]{};   // two primitive sets (current / tentative)
+  T *flx[3]{}, *ef[6]{}, *ee[3]{};   // fluxes, face EMFs (e2x1 e3x1 e1x2 e3x2 e1x3 e2x3), edge EMFs
+};
+
 struct chq_ctx {
   Geom g{};
   Eos eos{};
@@ -58,8 +66,10 @@ struct chq_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   char *arena = nullptr;
   size_t arena_bytes = 0;
-  double *u[2]{}, *bf[2][3]{}, *w[2]{}, *bcc[2]{}, *ecc[2]{};
-  double *flx[3]{}, *ef[6]{}, *ee[3]{};
+  double *u[2]{}, *bf[2][3]{};
+  FieldSet<double> f64;      // FP64 path
+  FieldSet<float> f32;       // mixed precision (only one of the two sets is allocated)
+  bool mixed = false;        // primitives, fluxes and EMFs are FP32
   int *flag = nullptr;
   int *badlist = nullptr;    // flagged cells of the current stage (fused fofc path)
   Ctrl *ctrl = nullptr;      // time loop
@@ -81,7 +91,7 @@ struct chq_ctx {
       tev.push_back(e);
     }
     if (!begin && (tev_used % 2) == 0) return;
-    hipEventRecord(tev[tev_used++], stream);
+    (void)hipEventRecord(tev[tev_used++], stream);
   }
 
   // face-flux launch description for direction d from buffer set `cur`
@@ -99,14 +109,14 @@ struct chq_ctx {
     if (d == 1) ju = g.je + 1; else if (g.two_d) { jl = g.js - 1; ju = g.je + 1; }
     if (d == 2) ku = g.ke + 1; else if (g.three_d) { kl = g.ks - 1; ku = g.ke + 1; }
     a.il = il; a.iu = iu; a.jl = jl; a.ju = ju; a.kl = kl; a.ku = ku;
-    a.w = w[wset];
-    a.bcc = bcc[wset];
+    a.w = f64.w[wset];
+    a.bcc = f64.bcc[wset];
     a.bn = bf[cur][d];
-    a.flx = flx[d];
+    a.flx = f64.flx[d];
     // et2 = e3x1 / e1x2 / e2x3 ; et1 = e2x1 / e3x2 / e1x3 (physics.hpp:93-94)
     static const int et2_id[3] = {1, 2, 5}, et1_id[3] = {0, 3, 4};   // ef = e2x1 e3x1 e1x2 e3x2 e1x3 e2x3
-    a.et2 = ef[et2_id[d]];
-    a.et1 = ef[et1_id[d]];
+    a.et2 = f64.ef[et2_id[d]];
+    a.et1 = f64.ef[et1_id[d]];
     a.flag = flag;
     a.ctrl = c;
     return a;
@@ -114,10 +124,10 @@ struct chq_ctx {
   CornerArgs corner_args(int wset, const Ctrl *c, int only_if_bad) const {
     CornerArgs a{};
     a.g = g;
-    a.f1 = flx[0]; a.f2 = flx[1]; a.f3 = flx[2];
-    a.e2x1 = ef[0]; a.e3x1 = ef[1]; a.e1x2 = ef[2]; a.e3x2 = ef[3]; a.e1x3 = ef[4]; a.e2x3 = ef[5];
-    a.ecc = ecc[wset];
-    a.e1 = ee[0]; a.e2 = ee[1]; a.e3 = ee[2];
+    a.f1 = f64.flx[0]; a.f2 = f64.flx[1]; a.f3 = f64.flx[2];
+    a.e2x1 = f64.ef[0]; a.e3x1 = f64.ef[1]; a.e1x2 = f64.ef[2]; a.e3x2 = f64.ef[3]; a.e1x3 = f64.ef[4]; a.e2x3 = f64.ef[5];
+    a.ecc = f64.ecc[wset];
+    a.e1 = f64.ee[0]; a.e2 = f64.ee[1]; a.e3 = f64.ee[2];
     a.ctrl = c;
     a.only_if_bad = only_if_bad;
     return a;
@@ -127,12 +137,12 @@ struct chq_ctx {
     a.g = g;
     a.g0 = g0; a.g1 = g1;
     a.nvar = nvar;
-    a.f1 = flx[0]; a.f2 = flx[1]; a.f3 = flx[2];
+    a.f1 = f64.flx[0]; a.f2 = f64.flx[1]; a.f3 = f64.flx[2];
     a.ua = u[0]; a.ub = u[1]; a.uo = u[out];
     a.b1a = bf[0][0]; a.b2a = bf[0][1]; a.b3a = bf[0][2];
     a.b1b = bf[1][0]; a.b2b = bf[1][1]; a.b3b = bf[1][2];
     a.b1o = bf[out][0]; a.b2o = bf[out][1]; a.b3o = bf[out][2];
-    a.e1 = ee[0]; a.e2 = ee[1]; a.e3 = ee[2];
+    a.e1 = f64.ee[0]; a.e2 = f64.ee[1]; a.e3 = f64.ee[2];
     a.ctrl = c;
     a.only_if_bad = only_if_bad;
     return a;
@@ -143,7 +153,7 @@ struct chq_ctx {
     a.eos = eos;
     a.b1 = bf[buf][0]; a.b2 = bf[buf][1]; a.b3 = bf[buf][2];
     a.u = u[buf];
-    a.w = w[wset]; a.bcc = bcc[wset]; a.ecc = ecc[wset];
+    a.w = f64.w[wset]; a.bcc = f64.bcc[wset]; a.ecc = f64.ecc[wset];
     a.write_went = 1;
     a.entropy_out = 1;
     a.flag = flag;
@@ -167,30 +177,90 @@ struct chq_ctx {
     return e;
   }
   int ndir() const { return 1 + (g.two_d ? 1 : 0) + (g.three_d ? 1 : 0); }
+
+  template <class T> FluxArgsT<T> flux_args_t(const FieldSet<T> &f, int d, int cur, int wset, const Ctrl *c) const {
+    FluxArgs a0 = flux_args(d, cur, wset, c);
+    FluxArgsT<T> a;
+    a.g = a0.g; a.eos = a0.eos; a.flatten = a0.flatten; a.low_mach = a0.low_mach; a.entropy = a0.entropy;
+    a.lhlld_phi = a0.lhlld_phi;
+    a.il = a0.il; a.iu = a0.iu; a.jl = a0.jl; a.ju = a0.ju; a.kl = a0.kl; a.ku = a0.ku;
+    a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.bn = a0.bn; a.flx = f.flx[d];
+    static const int et2_id[3] = {1, 2, 5}, et1_id[3] = {0, 3, 4};
+    a.et2 = f.ef[et2_id[d]];
+    a.et1 = f.ef[et1_id[d]];
+    a.flag = flag;
+    a.ctrl = c;
+    return a;
+  }
+  template <class T> CornerArgsT<T> corner_args_t(const FieldSet<T> &f, int wset, const Ctrl *c, int oib) const {
+    CornerArgsT<T> a{};
+    a.g = g;
+    a.f1 = f.flx[0]; a.f2 = f.flx[1]; a.f3 = f.flx[2];
+    a.e2x1 = f.ef[0]; a.e3x1 = f.ef[1]; a.e1x2 = f.ef[2]; a.e3x2 = f.ef[3]; a.e1x3 = f.ef[4]; a.e2x3 = f.ef[5];
+    a.ecc = f.ecc[wset];
+    a.e1 = f.ee[0]; a.e2 = f.ee[1]; a.e3 = f.ee[2];
+    a.ctrl = c;
+    a.only_if_bad = oib;
+    return a;
+  }
+  template <class T> UpdateArgsT<T> update_args_t(const FieldSet<T> &f, int out, double g0, double g1, int nvar,
+                                                  const Ctrl *c, int oib) const {
+    UpdateArgs a0 = update_args(out, g0, g1, nvar, c, oib);
+    UpdateArgsT<T> a;
+    a.g = a0.g; a.g0 = g0; a.g1 = g1; a.beta = 0.0; a.beta_dt = 0.0; a.use_ctrl_dt = 0; a.nvar = nvar;
+    a.f1 = f.flx[0]; a.f2 = f.flx[1]; a.f3 = f.flx[2];
+    a.ua = a0.ua; a.ub = a0.ub; a.uo = a0.uo;
+    a.b1a = a0.b1a; a.b2a = a0.b2a; a.b3a = a0.b3a; a.b1b = a0.b1b; a.b2b = a0.b2b; a.b3b = a0.b3b;
+    a.b1o = a0.b1o; a.b2o = a0.b2o; a.b3o = a0.b3o;
+    a.e1 = f.ee[0]; a.e2 = f.ee[1]; a.e3 = f.ee[2];
+    a.ctrl = c;
+    a.only_if_bad = oib;
+    return a;
+  }
+  template <class T> C2pArgsT<T> c2p_args_t(const FieldSet<T> &f, int buf, int wset, Ctrl *c) const {
+    C2pArgs a0 = c2p_args(buf, wset, c);
+    C2pArgsT<T> a;
+    a.g = a0.g; a.eos = a0.eos; a.b1 = a0.b1; a.b2 = a0.b2; a.b3 = a0.b3; a.u = a0.u;
+    a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.ecc = f.ecc[wset];
+    a.write_went = 1; a.entropy_out = 1; a.compute_dt = 0; a.slot = 0; a.flag_mode = 0;
+    a.flag = flag; a.badlist = badlist; a.ctrl = c; a.only_if_bad = 0;
+    return a;
+  }
 };
 
 namespace {
 
-bool array_desc(const chq_ctx *c, int id, void **dev, int *nc, int *face, int *ebytes) {
-  *face = 1; *ebytes = 8; *nc = 1; *dev = nullptr;
+template <class T>
+void field_desc(const FieldSet<T> &f, int wcur, int id, void **dev, int *nc, int *face) {
+  switch (id) {
+    case CHQ_W: *dev = f.w[wcur]; *nc = 6; *face = 0; break;
+    case CHQ_BCC: *dev = f.bcc[wcur]; *nc = 3; *face = 0; break;
+    case CHQ_ECC: *dev = f.ecc[wcur]; *nc = 3; *face = 0; break;
+    case CHQ_F1: case CHQ_F2: case CHQ_F3: *dev = f.flx[id - CHQ_F1]; *nc = 6; break;
+    case CHQ_E1: case CHQ_E2: case CHQ_E3: *dev = f.ee[id - CHQ_E1]; break;
+    default: *dev = f.ef[id - CHQ_E2X1]; break;
+  }
+}
+
+// Device pointer, component count, face (+1) extent and device element size of an array.
+// `wide` = 1 when the device holds float and the host side is double (mixed precision).
+bool array_desc(const chq_ctx *c, int id, void **dev, int *nc, int *face, int *ebytes, int *wide) {
+  *face = 1; *ebytes = 8; *nc = 1; *dev = nullptr; *wide = 0;
   switch (id) {
     case CHQ_U0: *dev = c->u[0]; *nc = 6; *face = 0; break;
     case CHQ_U1: *dev = c->u[1]; *nc = 6; *face = 0; break;
-    case CHQ_W: *dev = c->w[c->wcur]; *nc = 6; *face = 0; break;
-    case CHQ_BCC: *dev = c->bcc[c->wcur]; *nc = 3; *face = 0; break;
-    case CHQ_ECC: *dev = c->ecc[c->wcur]; *nc = 3; *face = 0; break;
+    case CHQ_W: case CHQ_BCC: case CHQ_ECC: case CHQ_F1: case CHQ_F2: case CHQ_F3:
+    case CHQ_E2X1: case CHQ_E3X1: case CHQ_E1X2: case CHQ_E3X2: case CHQ_E1X3: case CHQ_E2X3:
+    case CHQ_E1: case CHQ_E2: case CHQ_E3:
+      if (c->mixed) { field_desc(c->f32, c->wcur, id, dev, nc, face); *ebytes = 4; *wide = 1; }
+      else field_desc(c->f64, c->wcur, id, dev, nc, face);
+      break;
     case CHQ_B1F0: *dev = c->bf[0][0]; break;
     case CHQ_B2F0: *dev = c->bf[0][1]; break;
     case CHQ_B3F0: *dev = c->bf[0][2]; break;
     case CHQ_B1F1: *dev = c->bf[1][0]; break;
     case CHQ_B2F1: *dev = c->bf[1][1]; break;
     case CHQ_B3F1: *dev = c->bf[1][2]; break;
-    case CHQ_F1: *dev = c->flx[0]; *nc = 6; break;
-    case CHQ_F2: *dev = c->flx[1]; *nc = 6; break;
-    case CHQ_F3: *dev = c->flx[2]; *nc = 6; break;
-    case CHQ_E2X1: case CHQ_E3X1: case CHQ_E1X2: case CHQ_E3X2: case CHQ_E1X3: case CHQ_E2X3:
-      *dev = c->ef[id - CHQ_E2X1]; break;
-    case CHQ_E1: case CHQ_E2: case CHQ_E3: *dev = c->ee[id - CHQ_E1]; break;
     case CHQ_FLAG: *dev = c->flag; *face = 0; *ebytes = 4; break;
     default: return false;
   }
@@ -199,14 +269,14 @@ bool array_desc(const chq_ctx *c, int id, void **dev, int *nc, int *face, int *e
 
 int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
   void *dev;
-  int nc, face, eb;
-  if (!array_desc(c, id, &dev, &nc, &face, &eb)) return fail(CHQ_EINVAL, "unknown array id");
+  int nc, face, eb, wide;
+  if (!array_desc(c, id, &dev, &nc, &face, &eb, &wide)) return fail(CHQ_EINVAL, "unknown array id");
   const Geom &g = c->g;
   const int nk = g.nk + face, nj = g.nj + face, ni = g.ni + face;
   const size_t n = size_t(nc) * nk * nj * ni;
   if (count != n) return fail(CHQ_EINVAL, "host buffer has " + std::to_string(count) + " elements, expected " + std::to_string(n));
   if (!host) return fail(CHQ_EINVAL, "null host buffer");
-  const size_t bytes = n * eb;
+  const size_t bytes = n * (wide ? 8 : eb);   // host element size
   if (bytes > c->staging_bytes) {
     if (c->staging) HIPCHK(hipFree(c->staging));
     c->staging = nullptr;
@@ -216,10 +286,10 @@ int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
   }
   if (to_dev) {
     HIPCHK(hipMemcpyAsync(c->staging, host, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_relayout(c->staging, dev, g, nc, nk, nj, ni, 1, eb, c->stream));
+    HIPCHK(launch_relayout(c->staging, dev, g, nc, nk, nj, ni, 1, wide ? -4 : eb, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   } else {
-    HIPCHK(launch_relayout(c->staging, dev, g, nc, nk, nj, ni, 0, eb, c->stream));
+    HIPCHK(launch_relayout(c->staging, dev, g, nc, nk, nj, ni, 0, wide ? -4 : eb, c->stream));
     HIPCHK(hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
@@ -231,6 +301,8 @@ int check_cfg(const chq_mesh_desc *m, const chq_config *c) {
   if (m->nx1 < 1 || m->nx2 < 1 || m->nx3 < 1) return fail(CHQ_EINVAL, "mesh sizes must be >= 1");
   if (m->nx3 > 1 && m->nx2 <= 1) return fail(CHQ_EINVAL, "a 3D mesh needs nx2 > 1");
   if (c->recon < 0 || c->recon > 4) return fail(CHQ_EINVAL, "unknown reconstruction");
+  if (c->precision < 0 || c->precision > 1) return fail(CHQ_EINVAL, "precision must be 0 (fp64) or 1 (mixed)");
+  if (c->precision == 1 && c->dual_energy) return fail(CHQ_EUNSUP, "mixed precision with dual energy is not supported");
   if (c->rsolver < 0 || c->rsolver > 2) return fail(CHQ_EINVAL, "unknown Riemann solver");
   if (c->nstage < 1 || c->nstage > 4) return fail(CHQ_EINVAL, "nstage must be 1..4");
   const int ngmin = (c->recon == 2 || c->recon == 3 || c->recon == 4) ? 3 : 2;   // main.cpp:169-171
@@ -283,6 +355,7 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   c->eos.dual_energy = cfg->dual_energy;
   c->eos.de_eta = cfg->de_eta;
   c->entropy = cfg->dual_energy != 0;
+  c->mixed = cfg->precision == 1;
 
   auto cleanup = [&](int code) { chq_destroy(c); return code; };
   hipError_t e = hipGetDevice(&c->device);
@@ -290,10 +363,13 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
     return cleanup(fail(CHQ_EHIP, std::string("stream/event: ") + hipGetErrorString(e)));
-  // arena: 70 component arrays of cstride doubles + two control blocks
+  // arena: conserved state (12), face B (6), flags (2) as double-sized components, then the
+  // field set (w 12, bcc 6, ecc 6, fluxes 18, face EMFs 6, edge EMFs 3) in double (FP64) or
+  // float (mixed), then two control blocks
   const size_t comp = size_t(g.cstride) * sizeof(double);
-  const size_t ncomp = 12 + 6 + 12 + 6 + 6 + 18 + 6 + 3 + 1 + 1;
-  c->arena_bytes = ncomp * comp + 2 * 256;
+  const size_t nfield = 12 + 6 + 6 + 18 + 6 + 3;
+  const size_t field_bytes = nfield * size_t(g.cstride) * (c->mixed ? sizeof(float) : sizeof(double));
+  c->arena_bytes = (12 + 6 + 2) * comp + ((field_bytes + 255) / 256) * 256 + 2 * 256;
   e = hipMalloc((void **)&c->arena, c->arena_bytes);
   if (e != hipSuccess) { c->arena = nullptr; return cleanup(fail(CHQ_ENOMEM, "device arena of " + std::to_string(c->arena_bytes) + " bytes: " + hipGetErrorString(e))); }
   e = hipMemsetAsync(c->arena, 0, c->arena_bytes, c->stream);
@@ -302,14 +378,21 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   auto take = [&](int n) { double *r = (double *)p; p += size_t(n) * comp; return r; };
   for (int s = 0; s < 2; ++s) c->u[s] = take(6);
   for (int s = 0; s < 2; ++s) for (int d = 0; d < 3; ++d) c->bf[s][d] = take(1);
-  for (int s = 0; s < 2; ++s) c->w[s] = take(6);
-  for (int s = 0; s < 2; ++s) c->bcc[s] = take(3);
-  for (int s = 0; s < 2; ++s) c->ecc[s] = take(3);
-  for (int d = 0; d < 3; ++d) c->flx[d] = take(6);
-  for (int q = 0; q < 6; ++q) c->ef[q] = take(1);
-  for (int q = 0; q < 3; ++q) c->ee[q] = take(1);
   c->flag = (int *)take(1);
   c->badlist = (int *)take(1);
+  auto carve = [&](auto &f) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(f.flx[0])>>;
+    auto t = [&](int n) { T *r = (T *)p; p += size_t(n) * size_t(g.cstride) * sizeof(T); return r; };
+    for (int s = 0; s < 2; ++s) f.w[s] = t(6);
+    for (int s = 0; s < 2; ++s) f.bcc[s] = t(3);
+    for (int s = 0; s < 2; ++s) f.ecc[s] = t(3);
+    for (int d = 0; d < 3; ++d) f.flx[d] = t(6);
+    for (int q = 0; q < 6; ++q) f.ef[q] = t(1);
+    for (int q = 0; q < 3; ++q) f.ee[q] = t(1);
+  };
+  if (c->mixed) carve(c->f32); else carve(c->f64);
+  // float arenas end on a 4-byte multiple of cstride elements; keep the control blocks aligned
+  p = c->arena + (((p - c->arena) + 255) / 256) * 256;
   c->ctrl = (Ctrl *)p;
   c->ctrl_api = (Ctrl *)(p + 256);
   Ctrl h{};
@@ -327,13 +410,13 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
 
 int chq_destroy(chq_ctx *c) {
   if (!c) return CHQ_OK;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->arena) hipFree(c->arena);
-  if (c->staging) hipFree(c->staging);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  for (hipEvent_t e : c->tev) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->staging) (void)hipFree(c->staging);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CHQ_OK;
 }
@@ -354,8 +437,8 @@ size_t chq_device_bytes(const chq_ctx *c) { return c ? c->arena_bytes + c->stagi
 
 size_t chq_array_count(const chq_ctx *c, int id) {
   void *dev;
-  int nc, face, eb;
-  if (!c || !array_desc(c, id, &dev, &nc, &face, &eb)) return 0;
+  int nc, face, eb, wide;
+  if (!c || !array_desc(c, id, &dev, &nc, &face, &eb, &wide)) return 0;
   return size_t(nc) * (c->g.nk + face) * (c->g.nj + face) * (c->g.ni + face);
 }
 
@@ -373,6 +456,12 @@ int chq_download(chq_ctx *c, int id, void *host, size_t count) {
 int chq_fluxes(chq_ctx *c, int dir, int cur) {
   if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
+  if (c->mixed) {   // FP32 fluxes; the entropy component is not computed on this path
+    FluxArgsT<float> a = c->flux_args_t(c->f32, dir, cur, c->wcur, c->ctrl_api);
+    a.entropy = 0;
+    HIPCHK(launch_flux(a, dir, Recon(c->cfg.recon), RSolver(c->cfg.rsolver), false, c->stream));
+    return CHQ_OK;
+  }
   FluxArgs a = c->flux_args(dir, cur, c->wcur, c->ctrl_api);
   HIPCHK(launch_flux(a, dir, Recon(c->cfg.recon), RSolver(c->cfg.rsolver), true, c->stream));
   return CHQ_OK;
@@ -380,16 +469,24 @@ int chq_fluxes(chq_ctx *c, int dir, int cur) {
 
 int chq_corner_emf(chq_ctx *c) {
   if (!c) return fail(CHQ_EINVAL, "null context");
-  HIPCHK(launch_corner(c->corner_args(c->wcur, c->ctrl_api, 0), c->stream));
+  if (c->mixed) HIPCHK(launch_corner<float>(c->corner_args_t(c->f32, c->wcur, c->ctrl_api, 0), c->stream));
+  else HIPCHK(launch_corner<double>(c->corner_args(c->wcur, c->ctrl_api, 0), c->stream));
   return CHQ_OK;
 }
 
 int chq_update(chq_ctx *c, int out, double g0, double g1, double beta_dt) {
   if (!c || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (c->mixed) {
+    UpdateArgsT<float> a = c->update_args_t(c->f32, out, g0, g1, 5, c->ctrl_api, 0);
+    a.use_ctrl_dt = 0;
+    a.beta_dt = beta_dt;
+    HIPCHK(launch_update<float>(a, c->stream));
+    return CHQ_OK;
+  }
   UpdateArgs a = c->update_args(out, g0, g1, 6, c->ctrl_api, 0);
   a.use_ctrl_dt = 0;
   a.beta_dt = beta_dt;
-  HIPCHK(launch_update(a, c->stream));
+  HIPCHK(launch_update<double>(a, c->stream));
   return CHQ_OK;
 }
 
@@ -401,11 +498,18 @@ int chq_boundaries(chq_ctx *c, int buf) {
 
 int chq_cons_to_prim(chq_ctx *c, int buf, int compute_dt, double *min_dt) {
   if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  C2pArgs a = c->c2p_args(buf, c->wcur, c->ctrl_api);
-  a.compute_dt = compute_dt ? 1 : 0;
-  a.slot = 0;
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 1, c->stream));
-  HIPCHK(launch_c2p(a, c->stream));
+  if (c->mixed) {
+    C2pArgsT<float> a = c->c2p_args_t(c->f32, buf, c->wcur, c->ctrl_api);
+    a.compute_dt = compute_dt ? 1 : 0;
+    a.write_went = 0;
+    HIPCHK(launch_c2p<float>(a, c->stream));
+  } else {
+    C2pArgs a = c->c2p_args(buf, c->wcur, c->ctrl_api);
+    a.compute_dt = compute_dt ? 1 : 0;
+    a.slot = 0;
+    HIPCHK(launch_c2p<double>(a, c->stream));
+  }
   if (min_dt) {
     unsigned long long bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, &c->ctrl_api->mdt_bits[0], sizeof(bits), hipMemcpyDeviceToHost, c->stream));
@@ -419,6 +523,7 @@ int chq_cons_to_prim(chq_ctx *c, int buf, int compute_dt, double *min_dt) {
 
 int chq_fofc_flag(chq_ctx *c, int buf, int *count) {
   if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_fofc_flag: FP64 contexts only");
   C2pArgs a = c->c2p_args(buf, c->wcur, c->ctrl_api);
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 0, c->stream));
   HIPCHK(launch_flag(a, c->stream));
@@ -431,6 +536,7 @@ int chq_fofc_flag(chq_ctx *c, int buf, int *count) {
 
 int chq_fofc_repair(chq_ctx *c, int dir, int cur) {
   if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_fofc_repair: FP64 contexts only");
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
   FluxArgs a = c->flux_args(dir, cur, c->wcur, c->ctrl_api);
   HIPCHK(launch_repair(a, dir, false, c->stream));
@@ -441,22 +547,23 @@ int chq_fofc_repair(chq_ctx *c, int dir, int cur) {
 // device when the flag kernel counted cells)
 int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt, long *nbad) {
   if (!c || (cur & ~1) || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_stage: FP64 contexts only");
   const Ctrl *k = c->ctrl_api;
   const Recon rc = Recon(c->cfg.recon);
   const RSolver rs = RSolver(c->cfg.rsolver);
   for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_flux(c->flux_args(d, cur, c->wcur, k), d, rc, rs, true, c->stream));
-  HIPCHK(launch_corner(c->corner_args(c->wcur, k, 0), c->stream));
+  HIPCHK(launch_corner<double>(c->corner_args(c->wcur, k, 0), c->stream));
   UpdateArgs ua = c->update_args(out, g0, g1, 6, k, 0);
   ua.beta_dt = beta_dt;
-  HIPCHK(launch_update(ua, c->stream));
+  HIPCHK(launch_update<double>(ua, c->stream));
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 0, c->stream));
   if (c->cfg.fofc) {
     HIPCHK(launch_flag(c->c2p_args(out, c->wcur, c->ctrl_api), c->stream));
     for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_repair(c->flux_args(d, cur, c->wcur, k), d, true, c->stream));
-    HIPCHK(launch_corner(c->corner_args(c->wcur, k, 1), c->stream));
+    HIPCHK(launch_corner<double>(c->corner_args(c->wcur, k, 1), c->stream));
     UpdateArgs ub = c->update_args(out, g0, g1, 6, k, 1);
     ub.beta_dt = beta_dt;
-    HIPCHK(launch_update(ub, c->stream));
+    HIPCHK(launch_update<double>(ub, c->stream));
   }
   HIPCHK(c->boundaries(out, 6, k, 0));
   if (nbad) {
@@ -487,13 +594,75 @@ int chq_init(chq_ctx *c, double t, double tlim, int nlim) {
   return CHQ_OK;
 }
 
+extern "C++" {
+// One fused RK stage (dual energy off) over field set `fs` (FP64, or FP32 in mixed precision).
+template <class T>
+static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf, int last) {
+  const Recon rc = Recon(c->cfg.recon);
+  const RSolver rs = RSolver(c->cfg.rsolver);
+  Ctrl *k = c->ctrl;
+  const int nvar = c->entropy ? 6 : 5;
+  const int wc = c->wcur, wn = c->wcur ^ 1;
+  for (int d = 0; d < c->ndir(); ++d) {
+    FluxArgsT<T> fa = c->flux_args_t(fs, d, cf.cur, wc, k);
+    fa.entropy = c->entropy;
+    c->tmark(CHQ_K_FLUX, true);
+    HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
+    c->tmark(CHQ_K_FLUX, false);
+  }
+  c->tmark(CHQ_K_CORNER, true);
+  HIPCHK(launch_corner<T>(c->corner_args_t(fs, wc, k, 0), c->stream));
+  c->tmark(CHQ_K_CORNER, false);
+  UpdateArgsT<T> ua = c->update_args_t(fs, cf.out, cf.g0, cf.g1, nvar, k, 0);
+  ua.use_ctrl_dt = 1;
+  ua.beta = cf.beta;
+  c->tmark(CHQ_K_UPDATE, true);
+  HIPCHK(launch_update<T>(ua, c->stream));
+  c->tmark(CHQ_K_UPDATE, false);
+  c->tmark(CHQ_K_BC, true);
+  HIPCHK(c->boundaries(cf.out, nvar, k, 0));
+  c->tmark(CHQ_K_BC, false);
+  C2pArgsT<T> ca = c->c2p_args_t(fs, cf.out, wn, k);
+  ca.write_went = c->entropy;
+  ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
+  ca.compute_dt = last;
+  ca.slot = 0;
+  ca.flag_mode = c->cfg.fofc ? 2 : 0;
+  c->tmark(CHQ_K_C2P, true);
+  HIPCHK(launch_c2p<T>(ca, c->stream));
+  c->tmark(CHQ_K_C2P, false);
+  if (c->cfg.fofc) {
+    c->tmark(CHQ_K_REPAIR, true);
+    FluxArgsT<T> fr[3];
+    for (int d = 0; d < c->ndir(); ++d) {
+      fr[d] = c->flux_args_t(fs, d, cf.cur, wc, k);
+      fr[d].entropy = c->entropy;
+    }
+    HIPCHK(launch_repair_list(fr, c->ndir(), c->badlist, c->stream));
+    HIPCHK(launch_corner_list<T>(c->corner_args_t(fs, wc, k, 1), c->badlist, c->stream));
+    UpdateArgsT<T> ub = c->update_args_t(fs, cf.out, cf.g0, cf.g1, nvar, k, 1);
+    ub.use_ctrl_dt = 1;
+    ub.beta = cf.beta;
+    HIPCHK(launch_update<T>(ub, c->stream));
+    HIPCHK(c->boundaries(cf.out, nvar, k, 1));
+    C2pArgsT<T> cb = ca;
+    cb.flag_mode = 0;
+    cb.slot = 1;
+    cb.only_if_bad = 1;
+    HIPCHK(launch_c2p<T>(cb, c->stream));
+    c->tmark(CHQ_K_REPAIR, false);
+  }
+  return CHQ_OK;
+}
+}  // extern "C++"
+
 // One RK step on the device. Fast order (dual energy off): the fofc flag is evaluated inside
 // con2prim of the tentative state, whose primitives go to the second (w, bcc, ecc) set, so
-// the stage-input primitives the repair needs stay intact; the repair branch (repair,
-// corner, update, boundaries, con2prim) always launches and exits at once on the device
-// when nothing was flagged. Entropy (IENT) is not transported: with dual energy off
-// con2prim recomputes u[IENT] from rho and p in every stage, so its transport cannot
-// change any result.
+// the stage-input primitives the repair needs stay intact; the repair branch (list-driven
+// repair and corner EMF, update, boundaries, con2prim) always launches and exits at once on
+// the device when nothing was flagged. Entropy (IENT) is not transported: with dual energy
+// off con2prim recomputes u[IENT] from rho and p in every stage, so its transport cannot
+// change any result. With dual energy on, the reference kernel order is kept.
 static int enqueue_step(chq_ctx *c) {
   const Recon rc = Recon(c->cfg.recon);
   const RSolver rs = RSolver(c->cfg.rsolver);
@@ -505,76 +674,40 @@ static int enqueue_step(chq_ctx *c) {
   for (int s = 1; s <= nst; ++s) {
     const Coef cf = rk_coef(nst, s);
     const int last = s == nst;
-    const int wc = c->wcur, wn = fused ? (c->wcur ^ 1) : c->wcur;
     HIPCHK(launch_ctrl_begin_stage(k, last, c->stream));
-    for (int d = 0; d < c->ndir(); ++d) {
-      FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
-      fa.entropy = c->entropy;
-      c->tmark(CHQ_K_FLUX, true);
-      HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
-      c->tmark(CHQ_K_FLUX, false);
-    }
-    c->tmark(CHQ_K_CORNER, true);
-    HIPCHK(launch_corner(c->corner_args(wc, k, 0), c->stream));
-    c->tmark(CHQ_K_CORNER, false);
-    UpdateArgs ua = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 0);
-    ua.use_ctrl_dt = 1;
-    ua.beta = cf.beta;
-    c->tmark(CHQ_K_UPDATE, true);
-    HIPCHK(launch_update(ua, c->stream));
-    c->tmark(CHQ_K_UPDATE, false);
     if (fused) {
-      c->tmark(CHQ_K_BC, true);
-      HIPCHK(c->boundaries(cf.out, nvar, k, 0));
-      c->tmark(CHQ_K_BC, false);
-      C2pArgs ca = c->c2p_args(cf.out, wn, k);
-      ca.write_went = c->entropy;
-      ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
-      ca.compute_dt = last;
-      ca.slot = 0;
-      ca.flag_mode = c->cfg.fofc ? 2 : 0;
-      c->tmark(CHQ_K_C2P, true);
-      HIPCHK(launch_c2p(ca, c->stream));
-      c->tmark(CHQ_K_C2P, false);
-      if (c->cfg.fofc) {
-        c->tmark(CHQ_K_REPAIR, true);
-        FluxArgs fr[3];
-        for (int d = 0; d < c->ndir(); ++d) {
-          fr[d] = c->flux_args(d, cf.cur, wc, k);
-          fr[d].entropy = c->entropy;
-        }
-        HIPCHK(launch_repair_list(fr, c->ndir(), c->badlist, c->stream));
-        HIPCHK(launch_corner_list(c->corner_args(wc, k, 1), c->badlist, c->stream));
-        UpdateArgs ub = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 1);
-        ub.use_ctrl_dt = 1;
-        ub.beta = cf.beta;
-        HIPCHK(launch_update(ub, c->stream));
-        HIPCHK(c->boundaries(cf.out, nvar, k, 1));
-        C2pArgs cb = ca;
-        cb.flag_mode = 0;
-        cb.slot = 1;
-        cb.only_if_bad = 1;
-        HIPCHK(launch_c2p(cb, c->stream));
-        c->tmark(CHQ_K_REPAIR, false);
-      }
+      const int rc2 = c->mixed ? enqueue_fused_stage<float>(c, c->f32, cf, last)
+                               : enqueue_fused_stage<double>(c, c->f64, cf, last);
+      if (rc2) return rc2;
+      c->wcur ^= 1;
     } else {
+      const int wc = c->wcur;
+      for (int d = 0; d < c->ndir(); ++d) {
+        FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
+        fa.entropy = c->entropy;
+        HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
+      }
+      HIPCHK(launch_corner<double>(c->corner_args(wc, k, 0), c->stream));
+      UpdateArgs ua = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 0);
+      ua.use_ctrl_dt = 1;
+      ua.beta = cf.beta;
+      HIPCHK(launch_update<double>(ua, c->stream));
       if (c->cfg.fofc) {
         HIPCHK(launch_flag(c->c2p_args(cf.out, wc, k), c->stream));
         for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_repair(c->flux_args(d, cf.cur, wc, k), d, true, c->stream));
-        HIPCHK(launch_corner(c->corner_args(wc, k, 1), c->stream));
+        HIPCHK(launch_corner<double>(c->corner_args(wc, k, 1), c->stream));
         UpdateArgs ub = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 1);
         ub.use_ctrl_dt = 1;
         ub.beta = cf.beta;
-        HIPCHK(launch_update(ub, c->stream));
+        HIPCHK(launch_update<double>(ub, c->stream));
       }
       HIPCHK(c->boundaries(cf.out, nvar, k, 0));
       C2pArgs ca = c->c2p_args(cf.out, wc, k);
       ca.compute_dt = last;
       ca.slot = 0;
-      HIPCHK(launch_c2p(ca, c->stream));
+      HIPCHK(launch_c2p<double>(ca, c->stream));
     }
     HIPCHK(launch_ctrl_end_stage(k, last, fused ? 1 : 0, c->stream));
-    c->wcur = wn;
   }
   HIPCHK(launch_ctrl_end_step(k, c->stream));
   return CHQ_OK;

@@ -1,322 +1,12 @@
-// Fused face-flux kernels (reconstruction + flattening + positivity fallback + Riemann solve
-// + flux/face-EMF store) and the fofc donor-cell repair kernel, for gfx950.
-//
-// Reference: kernels::Fluxes<DIR,R,S> (physics.hpp:65-226) and fofc::RepairFluxes<DIR>
-// (fofc.hpp:46-96). L/R states never leave registers. Each cell's two half-states are
-// computed once (recon_cell.hpp) and shared by its two faces:
-//   * x faces (DIR 0): one wave64 per row segment, lane = cell; neighbour values move
-//     between lanes with cross-lane shuffles; a wave yields 64 - 5 faces (halo 3 + 2).
-//   * y and z faces (DIR 1, 2): lane = i (coalesced loads), each thread marches along j or
-//     k over a segment of SEG faces with a register window: every primitive is loaded
-//     once per segment and each cell is reconstructed once.
-// For FP64 the arithmetic and its order are the reference's, so fluxes are bitwise equal.
-#include "kernels.hpp"
-#include "numerics.hpp"
-#include "recon_cell.hpp"
+// FP64 face-flux kernels (the drop-in path; strict IEEE arithmetic, -ffp-contract=off) and the
+// flag-array form of the fofc repair. The kernel bodies are in flux_impl.hpp.
+#include "flux_impl.hpp"
 
 namespace chq {
 
 namespace {
-
-constexpr int BX = 64, BY = 4;
-constexpr int SEG = 16;   // faces per thread in the y/z march
-
-template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
-
-// ---------------------------------------------------------------- face solve + store
-// Riemann solve of one face from its two half-states and store (physics.hpp:167-224).
-// c = face index (device layout); sd = stride along DIR.
-template <int DIR, RSolver RS, bool ENT, int NV>
-__device__ __forceinline__ void solve_store(const FluxArgs &a, long long c, long long sd, const double *Ls,
-                                            const double *Rs_) {
-  using V = Rsv<double>;
-  const Geom &g = a.g;
-  const Eos e = a.eos;
-  const long long cs = g.cstride;
-  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
-  St<double> L{Ls[0], Ls[1], Ls[2], Ls[3], Ls[4], Ls[5], Ls[6]};
-  St<double> R{Rs_[0], Rs_[1], Rs_[2], Rs_[3], Rs_[4], Rs_[5], Rs_[6]};
-  const double bn = a.bn[c];
-  if (a.low_mach) {                                                // physics.hpp:170-178
-    const double cl = V::cfast(L, bn, e.gamma), cr = V::cfast(R, bn, e.gamma);
-    const double z = fmin(1.0, fmax(fabs(L.u) / cl, fabs(R.u) / cr));
-    const double um = 0.5 * (L.u + R.u), ud = 0.5 * (z * (L.u - R.u));
-    L.u = um + ud;
-    R.u = um - ud;
-  }
-  double f[NFLX];
-  if constexpr (RS == RSolver::hlld) {
-    V::template hlld<false>(L, R, bn, e.gamma, e.gm1, f);
-  } else if constexpr (RS == RSolver::lhlld) {                     // physics.hpp:182-211
-    constexpr int t1 = (DIR + 1) % 3, t2 = (DIR + 2) % 3;
-    const double *wu = a.w + IU * cs, *wv = a.w + IV * cs, *ww = a.w + IW * cs;
-    const long long cL = c - sd;
-    const long long s1 = t1 == 0 ? 1 : t1 == 1 ? g.pitch : g.plane;
-    const long long s2 = t2 == 0 ? 1 : t2 == 1 ? g.pitch : g.plane;
-    const bool has1 = t1 == 0 || (t1 == 1 && g.two_d) || (t1 == 2 && g.three_d);
-    const bool has2 = t2 == 0 || (t2 == 1 && g.two_d) || (t2 == 2 && g.three_d);
-    const double cfm = fmax(V::cfast(L, bn, e.gamma), V::cfast(R, bn, e.gamma));
-    const double du = wu[c] - wu[cL];
-    double dvw = 0.0;
-    if (has1) {
-      const double d1 = fmin(fmin(wv[cL] - wv[cL - s1], wv[cL + s1] - wv[cL]), fmin(wv[c] - wv[c - s1], wv[c + s1] - wv[c]));
-      dvw = fmin(dvw, d1);
-    }
-    if (has2) {
-      const double d2 = fmin(fmin(ww[cL] - ww[cL - s2], ww[cL + s2] - ww[cL]), fmin(ww[c] - ww[c - s2], ww[c + s2] - ww[c]));
-      dvw = fmin(dvw, d2);
-    }
-    const double ratio = fmin(1.0, (-fmin(du, 0.0) + cfm) / (-dvw + cfm));
-    const double r2 = ratio * ratio;
-    const double theta = r2 * r2;
-    const double chi = fmin(1.0, fmax(V::cflow(L, bn), V::cflow(R, bn)) / cfm);
-    const double phi = (a.lhlld_phi >= 0.0) ? a.lhlld_phi : chi * (2.0 - chi);
-    V::template hlld<true>(L, R, bn, e.gamma, e.gm1, f, theta, phi);
-  } else {
-    V::hlle(L, R, bn, e.gamma, e.gm1, f);
-  }
-  double *fx = a.flx + c;
-  fx[IDN * cs] = f[FD];
-  fx[IU * cs] = f[FMU];
-  fx[IV * cs] = f[FMV];
-  fx[IW * cs] = f[FMW];
-  fx[IEN * cs] = f[FEN];
-  if constexpr (ENT) {
-    const double sl = Ls[7], sr = Rs_[7];
-    fx[IENT * cs] = (f[FD] > 0.0) ? f[FD] * sl : (f[FD] < 0.0) ? f[FD] * sr : 0.5 * (f[FD] * (sl + sr));
-  }
-  a.et2[c] = -f[FBY];
-  a.et1[c] = f[FBZ];
-}
-
-// component offsets (in units of cstride) of the NV reconstructed variables in the rotated
-// frame: d, u, v, w, p (from w), B_t1, B_t2 (from bcc, stored after w's 6 components), IENT
-template <int DIR, int NV>
-__device__ __forceinline__ void var_ptrs(const FluxArgs &a, const double *p[NV]) {
-  const long long cs = a.g.cstride;
-  p[0] = a.w + IDN * cs;
-  p[1] = a.w + (IVX + DIR) * cs;
-  p[2] = a.w + (IVX + (DIR + 1) % 3) * cs;
-  p[3] = a.w + (IVX + (DIR + 2) % 3) * cs;
-  p[4] = a.w + IPR * cs;
-  p[5] = a.bcc + ((DIR + 1) % 3) * cs;
-  p[6] = a.bcc + ((DIR + 2) % 3) * cs;
-  if constexpr (NV == 8) p[7] = a.w + IENT * cs;
-}
-
-// ---------------------------------------------------------------- x faces: lane = cell
-template <Recon RC, RSolver RS, bool ENT>
-__global__ void __launch_bounds__(256) k_flux_x(const FluxArgs a, const int nseg, const int nwaves) {
-  constexpr int NV = ENT ? 8 : 7;
-  constexpr bool W = wide_v<RC>;
-  constexpr int H = W ? 3 : 2;            // lanes before the first face: cell halo + 1
-  constexpr int FPW = 64 - H - (W ? 2 : 1);   // faces per wave
-  if (!a.ctrl->active) return;
-  const int wave = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
-  if (wave >= nwaves) return;
-  const int lane = int(threadIdx.x) & 63;
-  const int seg = wave % nseg, row = wave / nseg;
-  const int nrow_j = a.ju - a.jl + 1;
-  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
-  const Geom &g = a.g;
-  const int F0 = a.il + seg * FPW;
-  int ci = F0 - H + lane;                 // this lane's cell
-  ci = ci < 0 ? 0 : (ci > g.ni - 1 ? g.ni - 1 : ci);   // halo lanes past the row end: any valid cell
-  const long long row0 = g.at(k, j, 0);
-  const double *src[NV];
-  var_ptrs<0, NV>(a, src);
-  double q[NV];
-#pragma unroll
-  for (int n = 0; n < NV; ++n) q[n] = src[n][row0 + ci];
-  using R = Rec<double>;
-  double Lf[NV], Rf[NV];
-  double fco = 0.0;
-  if constexpr (RC == Recon::ppm) {
-    double d[NV], e[NV];
-#pragma unroll
-    for (int n = 0; n < NV; ++n) {
-      const double qm = __shfl_up(q[n], 1, 64), qp = __shfl_down(q[n], 1, 64);
-      d[n] = R::mc(qm, q[n], qp);
-    }
-#pragma unroll
-    for (int n = 0; n < NV; ++n) {   // e[n] = edge at c+1/2
-      const double qp = __shfl_down(q[n], 1, 64), dp = __shfl_down(d[n], 1, 64);
-      e[n] = R::edge(q[n], qp, d[n], dp);
-    }
-#pragma unroll
-    for (int n = 0; n < NV; ++n) {
-      double lo = __shfl_up(e[n], 1, 64), hi = e[n];
-      R::mono(q[n], lo, hi);
-      Rf[n] = lo;
-      Lf[n] = hi;
-    }
-  } else {
-#pragma unroll
-    for (int n = 0; n < NV; ++n) {
-      const double qm1 = __shfl_up(q[n], 1, 64), qp1 = __shfl_down(q[n], 1, 64);
-      double qm2 = qm1, qp2 = qp1;
-      if constexpr (W) {
-        qm2 = __shfl_up(q[n], 2, 64);
-        qp2 = __shfl_down(q[n], 2, 64);
-      }
-      cell_halves<double, RC>(qm2, qm1, q[n], qp1, qp2, Rf[n], Lf[n]);
-    }
-  }
-  if constexpr (W) {
-    if (a.flatten) {
-      const double p = q[4];
-      const double pm2 = __shfl_up(p, 2, 64), pm1 = __shfl_up(p, 1, 64);
-      const double pp1 = __shfl_down(p, 1, 64), pp2 = __shfl_down(p, 2, 64);
-      const double um1 = __shfl_up(q[1], 1, 64), up1 = __shfl_down(q[1], 1, 64);
-      fco = R::flat(pm2, pm1, p, pp1, pp2, um1, up1);
-    }
-  }
-  finish_cell<double, NV>(Lf, Rf, q, fco, W && a.flatten, a.eos.dfloor, a.eos.pfloor);
-  // face i = cell index of this lane: left state = Lf of the lane below
-  double Lface[NV];
-#pragma unroll
-  for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
-  const int i = F0 - H + lane;
-  if (lane < H || lane >= H + FPW || i > a.iu) return;
-  solve_store<0, RS, ENT, NV>(a, row0 + i, 1, Lface, Rf);
-}
-
-// ---------------------------------------------------------------- y/z faces: register march
-template <int DIR, Recon RC, RSolver RS, bool ENT>
-__global__ void __launch_bounds__(BX * BY) k_flux_m(const FluxArgs a) {
-  constexpr int NV = ENT ? 8 : 7;
-  constexpr bool W = wide_v<RC>;
-  if (!a.ctrl->active) return;
-  const Geom &g = a.g;
-  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int t = int(blockIdx.y) * BY + int(threadIdx.y);     // transverse index: k (DIR 1) or j (DIR 2)
-  const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
-  const int F0 = lo_d + int(blockIdx.z) * SEG;
-  const int F1 = min(F0 + SEG - 1, hi_d);
-  if (i > a.iu) return;
-  int j, k;
-  if constexpr (DIR == 1) { k = a.kl + t; j = 0; if (k > a.ku) return; }
-  else { j = a.jl + t; k = 0; if (j > a.ju) return; }
-  const long long sd = DIR == 1 ? g.pitch : g.plane;
-  const long long base = DIR == 1 ? g.at(k, 0, i) : g.at(0, j, i);   // index of (d = 0)
-  const double *src[NV];
-  var_ptrs<DIR, NV>(a, src);
-  using R = Rec<double>;
-  const bool flat = W && a.flatten;
-  const double dfl = a.eos.dfloor, pfl = a.eos.pfloor;
-  // window around the cell being finished, c: wm2 = q_{c-2}, wm1 = q_{c-1}, w0 = q_c, wp1 = q_{c+1}
-  // (PPM keeps d_c and e_{c-1/2} instead of q_{c-2}, q_{c-1}, plus p_{c-2}, p_{c-1}, u_{c-1})
-  double w0[NV], wp1[NV], x1[NV], x2[NV];
-  double pm2 = 0.0, pm1 = 0.0, um1 = 0.0;
-  int c = F0 - 1;
-  {
-    const long long cc = base + (long long)c * sd;
-    double qm2[NV], qm1[NV];
-#pragma unroll
-    for (int n = 0; n < NV; ++n) {
-      qm1[n] = src[n][cc - sd];
-      w0[n] = src[n][cc];
-      wp1[n] = src[n][cc + sd];
-      qm2[n] = W ? src[n][cc - 2 * sd] : qm1[n];
-    }
-    if constexpr (RC == Recon::ppm) {
-#pragma unroll
-      for (int n = 0; n < NV; ++n) {
-        const double dm = R::mc(qm2[n], qm1[n], w0[n]);
-        x1[n] = R::mc(qm1[n], w0[n], wp1[n]);            // d_c
-        x2[n] = R::edge(qm1[n], w0[n], dm, x1[n]);       // e_{c-1/2}
-      }
-      pm2 = qm2[4];
-      pm1 = qm1[4];
-      um1 = qm1[1];
-    } else {
-#pragma unroll
-      for (int n = 0; n < NV; ++n) { x1[n] = qm2[n]; x2[n] = qm1[n]; }
-    }
-  }
-  double Lprev[NV];
-  for (; c <= F1; ++c) {
-    // finish cell c: needs q_{c+2} (wide) or q_{c+1} (narrow, already in wp1)
-    const long long cc = base + (long long)c * sd;
-    double qn[NV];
-    if constexpr (W) {
-#pragma unroll
-      for (int n = 0; n < NV; ++n) qn[n] = src[n][cc + 2 * sd];
-    } else {
-#pragma unroll
-      for (int n = 0; n < NV; ++n) qn[n] = wp1[n];
-    }
-    double Lf[NV], Rf[NV], fco = 0.0;
-    if constexpr (RC == Recon::ppm) {
-      double dn[NV], en[NV];
-#pragma unroll
-      for (int n = 0; n < NV; ++n) {
-        dn[n] = R::mc(w0[n], wp1[n], qn[n]);             // d_{c+1}
-        en[n] = R::edge(w0[n], wp1[n], x1[n], dn[n]);    // e_{c+1/2}
-        double lo = x2[n], hi = en[n];
-        R::mono(w0[n], lo, hi);
-        Rf[n] = lo;
-        Lf[n] = hi;
-      }
-      if (flat) fco = R::flat(pm2, pm1, w0[4], wp1[4], qn[4], um1, wp1[1]);
-      pm2 = pm1;
-      pm1 = w0[4];
-      um1 = w0[1];
-#pragma unroll
-      for (int n = 0; n < NV; ++n) { x1[n] = dn[n]; x2[n] = en[n]; }
-    } else {
-#pragma unroll
-      for (int n = 0; n < NV; ++n) cell_halves<double, RC>(x1[n], x2[n], w0[n], wp1[n], qn[n], Rf[n], Lf[n]);
-      if (flat) fco = R::flat(x1[4], x2[4], w0[4], wp1[4], qn[4], x2[1], wp1[1]);
-#pragma unroll
-      for (int n = 0; n < NV; ++n) { x1[n] = x2[n]; x2[n] = w0[n]; }
-    }
-    double qc[NV];
-#pragma unroll
-    for (int n = 0; n < NV; ++n) qc[n] = w0[n];
-    finish_cell<double, NV>(Lf, Rf, qc, fco, flat, dfl, pfl);
-    if (c >= F0) solve_store<DIR, RS, ENT, NV>(a, cc, sd, Lprev, Rf);   // face c (between c-1 and c)
-    // advance the window
-    if constexpr (W) {
-#pragma unroll
-      for (int n = 0; n < NV; ++n) { w0[n] = wp1[n]; wp1[n] = qn[n]; }
-    } else if (c < F1) {   // narrow stencil: the next cell's right neighbour
-#pragma unroll
-      for (int n = 0; n < NV; ++n) { w0[n] = wp1[n]; wp1[n] = src[n][cc + 2 * sd]; }
-    }
-#pragma unroll
-    for (int n = 0; n < NV; ++n) Lprev[n] = Lf[n];
-  }
-}
-
-// Donor-cell HLLE flux and face EMFs of one face from the stage-input primitives
-// (fofc.hpp:76-88).
-template <int DIR>
-__device__ __forceinline__ void repair_face(const FluxArgs &a, long long c, long long sd) {
-  const Geom &g = a.g;
-  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
-  constexpr int B1 = (DIR + 1) % 3, B2 = (DIR + 2) % 3;
-  const long long cL = c - sd, cs = g.cstride;
-  const double *w = a.w, *b = a.bcc;
-  St<double> L{w[IDN * cs + cL], w[IU * cs + cL], w[IV * cs + cL], w[IW * cs + cL], w[IPR * cs + cL],
-               b[B1 * cs + cL], b[B2 * cs + cL]};
-  St<double> R{w[IDN * cs + c], w[IU * cs + c], w[IV * cs + c], w[IW * cs + c], w[IPR * cs + c],
-               b[B1 * cs + c], b[B2 * cs + c]};
-  double f[NFLX];
-  Rsv<double>::hlle(L, R, a.bn[c], a.eos.gamma, a.eos.gm1, f);
-  double *fx = a.flx + c;
-  fx[IDN * cs] = f[FD];
-  fx[IU * cs] = f[FMU];
-  fx[IV * cs] = f[FMV];
-  fx[IW * cs] = f[FMW];
-  fx[IEN * cs] = f[FEN];
-  if (a.entropy) {  // upwind IENT flux with the donor entropies (fofc.hpp:85-86)
-    const double sL = w[IENT * cs + cL], sR = w[IENT * cs + c];
-    fx[IENT * cs] = (f[FD] > 0.0) ? f[FD] * sL : (f[FD] < 0.0) ? f[FD] * sR : 0.5 * (f[FD] * (sL + sR));
-  }
-  a.et2[c] = -f[FBY];
-  a.et1[c] = f[FBZ];
-}
+using fluxk::BX;
+using fluxk::BY;
 
 // RepairFluxes<DIR> over the whole face range with the flag array (fofc.hpp:66-94).
 template <int DIR>
@@ -337,82 +27,16 @@ __global__ void __launch_bounds__(BX * BY) k_repair(const FluxArgs a, const int 
   const bool fl = inside(k, j, i) && a.flag[c];
   const bool fr = inside(kL, jL, iL) && a.flag[cL];
   if (!(fl || fr)) return;
-  repair_face<DIR>(a, c, c - cL);
+  fluxk::repair_face<double, DIR>(a, c, c - cL);
 }
-
-// The same repair driven by the list of flagged (active) cells that con2prim builds: every
-// face touching a listed cell — its two faces in each direction — is recomputed. A face
-// shared by two listed cells is written twice with the same value.
-__global__ void __launch_bounds__(256) k_repair_list(const FluxArgs a0, const FluxArgs a1, const FluxArgs a2,
-                                                     const int *list, const int ndir) {
-  const Ctrl *ctl = a0.ctrl;
-  if (!ctl->active || ctl->nbad == 0) return;
-  const Geom &g = a0.g;
-  const int per = 2 * ndir;
-  const int n = ctl->nbad * per;
-  for (int t = int(blockIdx.x) * 256 + int(threadIdx.x); t < n; t += int(gridDim.x) * 256) {
-    const long long c0 = list[t / per];
-    const int r = t % per, dir = r >> 1, side = r & 1;
-    if (dir == 0) repair_face<0>(a0, c0 + side, 1);
-    else if (dir == 1) repair_face<1>(a1, c0 + side * (long long)g.pitch, g.pitch);
-    else repair_face<2>(a2, c0 + side * g.plane, g.plane);
-  }
-}
-
-template <int DIR, Recon RC, RSolver RS, bool ENT>
-hipError_t go_flux(const FluxArgs &a, hipStream_t s) {
-  const int nx = a.iu - a.il + 1, ny = a.ju - a.jl + 1, nz = a.ku - a.kl + 1;
-  if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
-  if constexpr (DIR == 0) {
-    constexpr bool W = wide_v<RC>;
-    constexpr int FPW = 64 - (W ? 3 : 2) - (W ? 2 : 1);
-    const int nseg = div_up(nx, FPW);
-    const int nwaves = nseg * ny * nz;
-    hipLaunchKernelGGL((k_flux_x<RC, RS, ENT>), dim3(div_up(nwaves, 4)), dim3(256), 0, s, a, nseg, nwaves);
-  } else {
-    const int nt = DIR == 1 ? nz : ny, nd = DIR == 1 ? ny : nz;
-    dim3 grid(div_up(nx, BX), div_up(nt, BY), div_up(nd, SEG)), block(BX, BY, 1);
-    hipLaunchKernelGGL((k_flux_m<DIR, RC, RS, ENT>), grid, block, 0, s, a);
-  }
-  return hipGetLastError();
-}
-
-template <int DIR, Recon RC, bool ENT>
-hipError_t go_flux_rs(const FluxArgs &a, RSolver rs, hipStream_t s) {
-  switch (rs) {
-    case RSolver::hlld: return go_flux<DIR, RC, RSolver::hlld, ENT>(a, s);
-    case RSolver::lhlld: return go_flux<DIR, RC, RSolver::lhlld, ENT>(a, s);
-    default: return go_flux<DIR, RC, RSolver::hlle, ENT>(a, s);
-  }
-}
-
-template <int DIR, bool ENT>
-hipError_t go_flux_rc(const FluxArgs &a, Recon rc, RSolver rs, hipStream_t s) {
-  switch (rc) {
-    case Recon::dc: return go_flux_rs<DIR, Recon::dc, ENT>(a, rs, s);
-    case Recon::plm: return go_flux_rs<DIR, Recon::plm, ENT>(a, rs, s);
-    case Recon::ppm: return go_flux_rs<DIR, Recon::ppm, ENT>(a, rs, s);
-    case Recon::ppmx: return go_flux_rs<DIR, Recon::ppmx, ENT>(a, rs, s);
-    default: return go_flux_rs<DIR, Recon::wenoz, ENT>(a, rs, s);
-  }
-}
-
-template <int DIR>
-hipError_t go_flux_dir(const FluxArgs &a, Recon rc, RSolver rs, bool ent, hipStream_t s) {
-  return ent ? go_flux_rc<DIR, true>(a, rc, rs, s) : go_flux_rc<DIR, false>(a, rc, rs, s);
-}
-
 }  // namespace
 
-hipError_t launch_repair_list(const FluxArgs a[3], int ndir, const int *list, hipStream_t s) {
-  hipLaunchKernelGGL(k_repair_list, dim3(256), dim3(256), 0, s, a[0], a[ndir > 1 ? 1 : 0], a[ndir > 2 ? 2 : 0], list, ndir);
-  return hipGetLastError();
+hipError_t launch_flux(const FluxArgsT<double> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s) {
+  return fluxk::launch_flux_t<double>(a, dir, rc, rs, entropy, s);
 }
 
-hipError_t launch_flux(const FluxArgs &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s) {
-  if (dir == 0) return go_flux_dir<0>(a, rc, rs, entropy, s);
-  if (dir == 1) return go_flux_dir<1>(a, rc, rs, entropy, s);
-  return go_flux_dir<2>(a, rc, rs, entropy, s);
+hipError_t launch_repair_list(const FluxArgsT<double> a[3], int ndir, const int *list, hipStream_t s) {
+  return fluxk::launch_repair_list_t<double>(a, ndir, list, s);
 }
 
 hipError_t launch_repair(const FluxArgs &a, int dir, bool only_if_bad, hipStream_t s) {

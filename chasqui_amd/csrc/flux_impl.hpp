@@ -1,0 +1,395 @@
+// Face-flux kernel templates (reconstruction + flattening + positivity fallback + Riemann
+// solve + flux/face-EMF store) and the fofc donor-cell repair, for gfx950. Included by
+// flux.hip (T = double, strict IEEE arithmetic: the FP64 drop-in) and flux32.hip (T = float:
+// the mixed-precision path's FP32 reconstruction, Riemann solve and EMFs).
+//
+// Reference: kernels::Fluxes<DIR,R,S> (physics.hpp:65-226) and fofc::RepairFluxes<DIR>
+// (fofc.hpp:46-96). L/R states never leave registers. Each cell's two half-states are
+// computed once (recon_cell.hpp) and shared by its two faces:
+//   * x faces (DIR 0): one wave64 per row segment, lane = cell; neighbour values move
+//     between lanes with cross-lane shuffles; a wave yields 64 - 5 faces (halo 3 + 2).
+//   * y and z faces (DIR 1, 2): lane = i (coalesced loads), each thread marches along j or
+//     k over a segment of SEG faces with a register window: every primitive is loaded
+//     once per segment and each cell is reconstructed once.
+// For T = double the arithmetic and its order are the reference's: fluxes are bitwise equal.
+#pragma once
+#include "kernels.hpp"
+#include "numerics.hpp"
+#include "recon_cell.hpp"
+
+namespace chq {
+namespace fluxk {
+
+constexpr int BX = 64, BY = 4;
+constexpr int SEG = 16;   // faces per thread in the y/z march
+
+template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
+
+// Riemann solve of one face from its two half-states and store (physics.hpp:167-224).
+// c = face index (device layout); sd = stride along DIR.
+template <class T, int DIR, RSolver RS, bool ENT, int NV>
+__device__ __forceinline__ void solve_store(const FluxArgsT<T> &a, long long c, long long sd, const T *Ls,
+                                            const T *Rs_) {
+  using V = Rsv<T>;
+  const Geom &g = a.g;
+  const T gam = T(a.eos.gamma), gm1 = T(a.eos.gm1);
+  const long long cs = g.cstride;
+  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
+  St<T> L{Ls[0], Ls[1], Ls[2], Ls[3], Ls[4], Ls[5], Ls[6]};
+  St<T> R{Rs_[0], Rs_[1], Rs_[2], Rs_[3], Rs_[4], Rs_[5], Rs_[6]};
+  const T bn = T(a.bn[c]);
+  if (a.low_mach) {                                                // physics.hpp:170-178
+    const T cl = V::cfast(L, bn, gam), cr = V::cfast(R, bn, gam);
+    const T z = vmin(T(1), vmax(vabs(L.u) / cl, vabs(R.u) / cr));
+    const T um = T(0.5) * (L.u + R.u), ud = T(0.5) * (z * (L.u - R.u));
+    L.u = um + ud;
+    R.u = um - ud;
+  }
+  T f[NFLX];
+  if constexpr (RS == RSolver::hlld) {
+    V::template hlld<false>(L, R, bn, gam, gm1, f);
+  } else if constexpr (RS == RSolver::lhlld) {                     // physics.hpp:182-211
+    constexpr int t1 = (DIR + 1) % 3, t2 = (DIR + 2) % 3;
+    const T *wu = a.w + IU * cs, *wv = a.w + IV * cs, *ww = a.w + IW * cs;
+    const long long cL = c - sd;
+    const long long s1 = t1 == 0 ? 1 : t1 == 1 ? g.pitch : g.plane;
+    const long long s2 = t2 == 0 ? 1 : t2 == 1 ? g.pitch : g.plane;
+    const bool has1 = t1 == 0 || (t1 == 1 && g.two_d) || (t1 == 2 && g.three_d);
+    const bool has2 = t2 == 0 || (t2 == 1 && g.two_d) || (t2 == 2 && g.three_d);
+    const T cfm = vmax(V::cfast(L, bn, gam), V::cfast(R, bn, gam));
+    const T du = wu[c] - wu[cL];
+    T dvw = T(0);
+    if (has1) {
+      const T d1 = vmin(vmin(wv[cL] - wv[cL - s1], wv[cL + s1] - wv[cL]), vmin(wv[c] - wv[c - s1], wv[c + s1] - wv[c]));
+      dvw = vmin(dvw, d1);
+    }
+    if (has2) {
+      const T d2 = vmin(vmin(ww[cL] - ww[cL - s2], ww[cL + s2] - ww[cL]), vmin(ww[c] - ww[c - s2], ww[c + s2] - ww[c]));
+      dvw = vmin(dvw, d2);
+    }
+    const T ratio = vmin(T(1), (-vmin(du, T(0)) + cfm) / (-dvw + cfm));
+    const T r2 = ratio * ratio;
+    const T theta = r2 * r2;
+    const T chi = vmin(T(1), vmax(V::cflow(L, bn), V::cflow(R, bn)) / cfm);
+    const T phi = (a.lhlld_phi >= 0.0) ? T(a.lhlld_phi) : chi * (T(2) - chi);
+    V::template hlld<true>(L, R, bn, gam, gm1, f, theta, phi);
+  } else {
+    V::hlle(L, R, bn, gam, gm1, f);
+  }
+  T *fx = a.flx + c;
+  fx[IDN * cs] = f[FD];
+  fx[IU * cs] = f[FMU];
+  fx[IV * cs] = f[FMV];
+  fx[IW * cs] = f[FMW];
+  fx[IEN * cs] = f[FEN];
+  if constexpr (ENT) {
+    const T sl = Ls[7], sr = Rs_[7];
+    fx[IENT * cs] = (f[FD] > T(0)) ? f[FD] * sl : (f[FD] < T(0)) ? f[FD] * sr : T(0.5) * (f[FD] * (sl + sr));
+  }
+  a.et2[c] = -f[FBY];
+  a.et1[c] = f[FBZ];
+}
+
+// the NV reconstructed variables in the rotated frame: d, u, v, w, p (from w), B_t1, B_t2
+// (from bcc), IENT
+template <class T, int DIR, int NV>
+__device__ __forceinline__ void var_ptrs(const FluxArgsT<T> &a, const T *p[NV]) {
+  const long long cs = a.g.cstride;
+  p[0] = a.w + IDN * cs;
+  p[1] = a.w + (IVX + DIR) * cs;
+  p[2] = a.w + (IVX + (DIR + 1) % 3) * cs;
+  p[3] = a.w + (IVX + (DIR + 2) % 3) * cs;
+  p[4] = a.w + IPR * cs;
+  p[5] = a.bcc + ((DIR + 1) % 3) * cs;
+  p[6] = a.bcc + ((DIR + 2) % 3) * cs;
+  if constexpr (NV == 8) p[7] = a.w + IENT * cs;
+}
+
+// ---------------------------------------------------------------- x faces: lane = cell
+template <class T, Recon RC, RSolver RS, bool ENT>
+__global__ void __launch_bounds__(256) k_flux_x(const FluxArgsT<T> a, const int nseg, const int nwaves) {
+  constexpr int NV = ENT ? 8 : 7;
+  constexpr bool W = wide_v<RC>;
+  constexpr int H = W ? 3 : 2;            // lanes before the first face: cell halo + 1
+  constexpr int FPW = 64 - H - (W ? 2 : 1);   // faces per wave
+  if (!a.ctrl->active) return;
+  const int wave = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  const int lane = int(threadIdx.x) & 63;
+  const int seg = wave % nseg, row = wave / nseg;
+  const int nrow_j = a.ju - a.jl + 1;
+  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
+  const Geom &g = a.g;
+  const int F0 = a.il + seg * FPW;
+  int ci = F0 - H + lane;                 // this lane's cell
+  ci = ci < 0 ? 0 : (ci > g.ni - 1 ? g.ni - 1 : ci);   // halo lanes past the row end: any valid cell
+  const long long row0 = g.at(k, j, 0);
+  const T *src[NV];
+  var_ptrs<T, 0, NV>(a, src);
+  T q[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) q[n] = src[n][row0 + ci];
+  using R = Rec<T>;
+  T Lf[NV], Rf[NV];
+  T fco = T(0);
+  if constexpr (RC == Recon::ppm) {
+    T d[NV], e[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      const T qm = __shfl_up(q[n], 1, 64), qp = __shfl_down(q[n], 1, 64);
+      d[n] = R::mc(qm, q[n], qp);
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {   // e[n] = edge at c+1/2
+      const T qp = __shfl_down(q[n], 1, 64), dp = __shfl_down(d[n], 1, 64);
+      e[n] = R::edge(q[n], qp, d[n], dp);
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      T lo = __shfl_up(e[n], 1, 64), hi = e[n];
+      R::mono(q[n], lo, hi);
+      Rf[n] = lo;
+      Lf[n] = hi;
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      const T qm1 = __shfl_up(q[n], 1, 64), qp1 = __shfl_down(q[n], 1, 64);
+      T qm2 = qm1, qp2 = qp1;
+      if constexpr (W) {
+        qm2 = __shfl_up(q[n], 2, 64);
+        qp2 = __shfl_down(q[n], 2, 64);
+      }
+      cell_halves<T, RC>(qm2, qm1, q[n], qp1, qp2, Rf[n], Lf[n]);
+    }
+  }
+  if constexpr (W) {
+    if (a.flatten) {
+      const T p = q[4];
+      const T pm2 = __shfl_up(p, 2, 64), pm1 = __shfl_up(p, 1, 64);
+      const T pp1 = __shfl_down(p, 1, 64), pp2 = __shfl_down(p, 2, 64);
+      const T um1 = __shfl_up(q[1], 1, 64), up1 = __shfl_down(q[1], 1, 64);
+      fco = R::flat(pm2, pm1, p, pp1, pp2, um1, up1);
+    }
+  }
+  finish_cell<T, NV>(Lf, Rf, q, fco, W && a.flatten, T(a.eos.dfloor), T(a.eos.pfloor));
+  // face i = cell index of this lane: left state = Lf of the lane below
+  T Lface[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
+  const int i = F0 - H + lane;
+  if (lane < H || lane >= H + FPW || i > a.iu) return;
+  solve_store<T, 0, RS, ENT, NV>(a, row0 + i, 1, Lface, Rf);
+}
+
+// ---------------------------------------------------------------- y/z faces: register march
+template <class T, int DIR, Recon RC, RSolver RS, bool ENT>
+__global__ void __launch_bounds__(BX * BY) k_flux_m(const FluxArgsT<T> a) {
+  constexpr int NV = ENT ? 8 : 7;
+  constexpr bool W = wide_v<RC>;
+  if (!a.ctrl->active) return;
+  const Geom &g = a.g;
+  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int t = int(blockIdx.y) * BY + int(threadIdx.y);     // transverse index: k (DIR 1) or j (DIR 2)
+  const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
+  const int F0 = lo_d + int(blockIdx.z) * SEG;
+  const int F1 = min(F0 + SEG - 1, hi_d);
+  if (i > a.iu) return;
+  int j, k;
+  if constexpr (DIR == 1) { k = a.kl + t; j = 0; if (k > a.ku) return; }
+  else { j = a.jl + t; k = 0; if (j > a.ju) return; }
+  const long long sd = DIR == 1 ? g.pitch : g.plane;
+  const long long base = DIR == 1 ? g.at(k, 0, i) : g.at(0, j, i);   // index of (d = 0)
+  const T *src[NV];
+  var_ptrs<T, DIR, NV>(a, src);
+  using R = Rec<T>;
+  const bool flat = W && a.flatten;
+  const T dfl = T(a.eos.dfloor), pfl = T(a.eos.pfloor);
+  // window around the cell being finished, c: x1 = q_{c-2}, x2 = q_{c-1}, w0 = q_c, wp1 = q_{c+1}
+  // (PPM keeps x1 = d_c and x2 = e_{c-1/2} instead, plus p_{c-2}, p_{c-1}, u_{c-1})
+  T w0[NV], wp1[NV], x1[NV], x2[NV];
+  T pm2 = T(0), pm1 = T(0), um1 = T(0);
+  int c = F0 - 1;
+  {
+    const long long cc = base + (long long)c * sd;
+    T qm2[NV], qm1[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      qm1[n] = src[n][cc - sd];
+      w0[n] = src[n][cc];
+      wp1[n] = src[n][cc + sd];
+      qm2[n] = W ? src[n][cc - 2 * sd] : qm1[n];
+    }
+    if constexpr (RC == Recon::ppm) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        const T dm = R::mc(qm2[n], qm1[n], w0[n]);
+        x1[n] = R::mc(qm1[n], w0[n], wp1[n]);            // d_c
+        x2[n] = R::edge(qm1[n], w0[n], dm, x1[n]);       // e_{c-1/2}
+      }
+      pm2 = qm2[4];
+      pm1 = qm1[4];
+      um1 = qm1[1];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { x1[n] = qm2[n]; x2[n] = qm1[n]; }
+    }
+  }
+  T Lprev[NV];
+  for (; c <= F1; ++c) {
+    // finish cell c: needs q_{c+2} (wide) or q_{c+1} (narrow, already in wp1)
+    const long long cc = base + (long long)c * sd;
+    T qn[NV];
+    if constexpr (W) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) qn[n] = src[n][cc + 2 * sd];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) qn[n] = wp1[n];
+    }
+    T Lf[NV], Rf[NV], fco = T(0);
+    if constexpr (RC == Recon::ppm) {
+      T dn[NV], en[NV];
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        dn[n] = R::mc(w0[n], wp1[n], qn[n]);             // d_{c+1}
+        en[n] = R::edge(w0[n], wp1[n], x1[n], dn[n]);    // e_{c+1/2}
+        T lo = x2[n], hi = en[n];
+        R::mono(w0[n], lo, hi);
+        Rf[n] = lo;
+        Lf[n] = hi;
+      }
+      if (flat) fco = R::flat(pm2, pm1, w0[4], wp1[4], qn[4], um1, wp1[1]);
+      pm2 = pm1;
+      pm1 = w0[4];
+      um1 = w0[1];
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { x1[n] = dn[n]; x2[n] = en[n]; }
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) cell_halves<T, RC>(x1[n], x2[n], w0[n], wp1[n], qn[n], Rf[n], Lf[n]);
+      if (flat) fco = R::flat(x1[4], x2[4], w0[4], wp1[4], qn[4], x2[1], wp1[1]);
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { x1[n] = x2[n]; x2[n] = w0[n]; }
+    }
+    T qc[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) qc[n] = w0[n];
+    finish_cell<T, NV>(Lf, Rf, qc, fco, flat, dfl, pfl);
+    if (c >= F0) solve_store<T, DIR, RS, ENT, NV>(a, cc, sd, Lprev, Rf);   // face c (between c-1 and c)
+    // advance the window
+    if constexpr (W) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { w0[n] = wp1[n]; wp1[n] = qn[n]; }
+    } else if (c < F1) {   // narrow stencil: the next cell's right neighbour
+#pragma unroll
+      for (int n = 0; n < NV; ++n) { w0[n] = wp1[n]; wp1[n] = src[n][cc + 2 * sd]; }
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) Lprev[n] = Lf[n];
+  }
+}
+
+// Donor-cell HLLE flux and face EMFs of one face from the stage-input primitives
+// (fofc.hpp:76-88).
+template <class T, int DIR>
+__device__ __forceinline__ void repair_face(const FluxArgsT<T> &a, long long c, long long sd) {
+  const Geom &g = a.g;
+  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
+  constexpr int B1 = (DIR + 1) % 3, B2 = (DIR + 2) % 3;
+  const long long cL = c - sd, cs = g.cstride;
+  const T *w = a.w, *b = a.bcc;
+  St<T> L{w[IDN * cs + cL], w[IU * cs + cL], w[IV * cs + cL], w[IW * cs + cL], w[IPR * cs + cL],
+          b[B1 * cs + cL], b[B2 * cs + cL]};
+  St<T> R{w[IDN * cs + c], w[IU * cs + c], w[IV * cs + c], w[IW * cs + c], w[IPR * cs + c],
+          b[B1 * cs + c], b[B2 * cs + c]};
+  T f[NFLX];
+  Rsv<T>::hlle(L, R, T(a.bn[c]), T(a.eos.gamma), T(a.eos.gm1), f);
+  T *fx = a.flx + c;
+  fx[IDN * cs] = f[FD];
+  fx[IU * cs] = f[FMU];
+  fx[IV * cs] = f[FMV];
+  fx[IW * cs] = f[FMW];
+  fx[IEN * cs] = f[FEN];
+  if (a.entropy) {  // upwind IENT flux with the donor entropies (fofc.hpp:85-86)
+    const T sL = w[IENT * cs + cL], sR = w[IENT * cs + c];
+    fx[IENT * cs] = (f[FD] > T(0)) ? f[FD] * sL : (f[FD] < T(0)) ? f[FD] * sR : T(0.5) * (f[FD] * (sL + sR));
+  }
+  a.et2[c] = -f[FBY];
+  a.et1[c] = f[FBZ];
+}
+
+// The repair driven by the list of flagged (active) cells that con2prim builds: every face
+// touching a listed cell — its two faces in each direction — is recomputed. A face shared by
+// two listed cells is written twice with the same value.
+template <class T>
+__global__ void __launch_bounds__(256) k_repair_list(const FluxArgsT<T> a0, const FluxArgsT<T> a1,
+                                                     const FluxArgsT<T> a2, const int *list, const int ndir) {
+  const Ctrl *ctl = a0.ctrl;
+  if (!ctl->active || ctl->nbad == 0) return;
+  const Geom &g = a0.g;
+  const int per = 2 * ndir;
+  const int n = ctl->nbad * per;
+  for (int t = int(blockIdx.x) * 256 + int(threadIdx.x); t < n; t += int(gridDim.x) * 256) {
+    const long long c0 = list[t / per];
+    const int r = t % per, dir = r >> 1, side = r & 1;
+    if (dir == 0) repair_face<T, 0>(a0, c0 + side, 1);
+    else if (dir == 1) repair_face<T, 1>(a1, c0 + side * (long long)g.pitch, g.pitch);
+    else repair_face<T, 2>(a2, c0 + side * g.plane, g.plane);
+  }
+}
+
+template <class T, int DIR, Recon RC, RSolver RS, bool ENT>
+hipError_t go_flux(const FluxArgsT<T> &a, hipStream_t s) {
+  const int nx = a.iu - a.il + 1, ny = a.ju - a.jl + 1, nz = a.ku - a.kl + 1;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+  if constexpr (DIR == 0) {
+    constexpr bool W = wide_v<RC>;
+    constexpr int FPW = 64 - (W ? 3 : 2) - (W ? 2 : 1);
+    const int nseg = div_up(nx, FPW);
+    const int nwaves = nseg * ny * nz;
+    hipLaunchKernelGGL((k_flux_x<T, RC, RS, ENT>), dim3(div_up(nwaves, 4)), dim3(256), 0, s, a, nseg, nwaves);
+  } else {
+    const int nt = DIR == 1 ? nz : ny, nd = DIR == 1 ? ny : nz;
+    dim3 grid(div_up(nx, BX), div_up(nt, BY), div_up(nd, SEG)), block(BX, BY, 1);
+    hipLaunchKernelGGL((k_flux_m<T, DIR, RC, RS, ENT>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <class T, int DIR, Recon RC, bool ENT>
+hipError_t go_flux_rs(const FluxArgsT<T> &a, RSolver rs, hipStream_t s) {
+  switch (rs) {
+    case RSolver::hlld: return go_flux<T, DIR, RC, RSolver::hlld, ENT>(a, s);
+    case RSolver::lhlld: return go_flux<T, DIR, RC, RSolver::lhlld, ENT>(a, s);
+    default: return go_flux<T, DIR, RC, RSolver::hlle, ENT>(a, s);
+  }
+}
+
+template <class T, int DIR, bool ENT>
+hipError_t go_flux_rc(const FluxArgsT<T> &a, Recon rc, RSolver rs, hipStream_t s) {
+  switch (rc) {
+    case Recon::dc: return go_flux_rs<T, DIR, Recon::dc, ENT>(a, rs, s);
+    case Recon::plm: return go_flux_rs<T, DIR, Recon::plm, ENT>(a, rs, s);
+    case Recon::ppm: return go_flux_rs<T, DIR, Recon::ppm, ENT>(a, rs, s);
+    case Recon::ppmx: return go_flux_rs<T, DIR, Recon::ppmx, ENT>(a, rs, s);
+    default: return go_flux_rs<T, DIR, Recon::wenoz, ENT>(a, rs, s);
+  }
+}
+
+template <class T>
+hipError_t launch_flux_t(const FluxArgsT<T> &a, int dir, Recon rc, RSolver rs, bool ent, hipStream_t s) {
+  if (dir == 0) return ent ? go_flux_rc<T, 0, true>(a, rc, rs, s) : go_flux_rc<T, 0, false>(a, rc, rs, s);
+  if (dir == 1) return ent ? go_flux_rc<T, 1, true>(a, rc, rs, s) : go_flux_rc<T, 1, false>(a, rc, rs, s);
+  return ent ? go_flux_rc<T, 2, true>(a, rc, rs, s) : go_flux_rc<T, 2, false>(a, rc, rs, s);
+}
+
+template <class T>
+hipError_t launch_repair_list_t(const FluxArgsT<T> a[3], int ndir, const int *list, hipStream_t s) {
+  hipLaunchKernelGGL(k_repair_list<T>, dim3(256), dim3(256), 0, s, a[0], a[ndir > 1 ? 1 : 0], a[ndir > 2 ? 2 : 0],
+                     list, ndir);
+  return hipGetLastError();
+}
+
+}  // namespace fluxk
+}  // namespace chq

@@ -249,6 +249,9 @@ void Deck::configure() {
   cfg.bc[1] = parse_bc(p.get_string("mesh", "bc_x2", "periodic"));
   cfg.bc[2] = parse_bc(p.get_string("mesh", "bc_x3", "periodic"));
   cfg.fofc = p.get_bool("mhd", "fofc", true);
+  const std::string pr = p.get_string("mhd", "precision", "fp64");
+  cfg.precision = pr == "fp64" ? 0 : pr == "mixed" ? 1 : -1;
+  if (cfg.precision < 0) throw std::runtime_error("unknown precision: " + pr + " (fp64 | mixed)");
   if (p.get_bool("mhd", "flux_limiter", false)) throw std::runtime_error("mhd.flux_limiter is not supported");
   if (p.get_bool("diffusion", "sts", false) || p.get_real("diffusion", "nu", 0.0) != 0.0 ||
       p.get_real("diffusion", "kappa", 0.0) != 0.0 || p.get_real("diffusion", "eta", 0.0) != 0.0)

@@ -6,8 +6,10 @@ namespace chq {
 
 namespace {
 
-template <class E>
-__global__ void __launch_bounds__(256) k_relayout(E *ref, E *dev, Geom g, int nc, int nk, int nj, int ni, int to_dev) {
+// H: element type of the reference-layout (host-side) buffer, D: of the device array.
+// float device arrays (mixed precision) are widened on download and rounded on upload.
+template <class H, class D>
+__global__ void __launch_bounds__(256) k_relayout(H *ref, D *dev, Geom g, int nc, int nk, int nj, int ni, int to_dev) {
   const long long n_all = (long long)nc * nk * nj * ni;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_all;
        t += (long long)gridDim.x * blockDim.x) {
@@ -18,8 +20,8 @@ __global__ void __launch_bounds__(256) k_relayout(E *ref, E *dev, Geom g, int nc
     const int k = int(r % nk);
     const int n = int(r / nk);
     const long long d = (long long)n * g.cstride + g.at(k, j, i);
-    if (to_dev) dev[d] = ref[t];
-    else ref[t] = dev[d];
+    if (to_dev) dev[d] = D(ref[t]);
+    else ref[t] = H(dev[d]);
   }
 }
 
@@ -30,9 +32,11 @@ hipError_t launch_relayout(void *ref, void *dev, const Geom &g, int nc, int nk, 
   const long long n_all = (long long)nc * nk * nj * ni;
   const int nb = int(std::min<long long>((n_all + 255) / 256, 65536));
   if (elem_bytes == 8)
-    hipLaunchKernelGGL(k_relayout<double>, dim3(nb), dim3(256), 0, s, (double *)ref, (double *)dev, g, nc, nk, nj, ni, to_dev);
+    hipLaunchKernelGGL((k_relayout<double, double>), dim3(nb), dim3(256), 0, s, (double *)ref, (double *)dev, g, nc, nk, nj, ni, to_dev);
+  else if (elem_bytes == -4)   // double on the host side, float on the device
+    hipLaunchKernelGGL((k_relayout<double, float>), dim3(nb), dim3(256), 0, s, (double *)ref, (float *)dev, g, nc, nk, nj, ni, to_dev);
   else
-    hipLaunchKernelGGL(k_relayout<int>, dim3(nb), dim3(256), 0, s, (int *)ref, (int *)dev, g, nc, nk, nj, ni, to_dev);
+    hipLaunchKernelGGL((k_relayout<int, int>), dim3(nb), dim3(256), 0, s, (int *)ref, (int *)dev, g, nc, nk, nj, ni, to_dev);
   return hipGetLastError();
 }
 

@@ -25,8 +25,15 @@ template <> struct Lit<float> {
 
 template <class T> CHQ_FN T sq(T x) { return x * x; }
 template <class T> CHQ_FN T sgn(T x) { return (x > T(0)) ? T(1) : ((x < T(0)) ? T(-1) : T(0)); }
-template <class T> CHQ_FN T vmin(T a, T b) { return fmin(a, b); }
-template <class T> CHQ_FN T vmax(T a, T b) { return fmax(a, b); }
+// math by type: the double forms are the C library's; float calls the single-precision forms
+CHQ_FN double vmin(double a, double b) { return fmin(a, b); }
+CHQ_FN double vmax(double a, double b) { return fmax(a, b); }
+CHQ_FN double vabs(double a) { return fabs(a); }
+CHQ_FN double vsqrt(double a) { return sqrt(a); }
+CHQ_FN float vmin(float a, float b) { return fminf(a, b); }
+CHQ_FN float vmax(float a, float b) { return fmaxf(a, b); }
+CHQ_FN float vabs(float a) { return fabsf(a); }
+CHQ_FN float vsqrt(float a) { return sqrtf(a); }
 
 // ----------------------------------------------------------------- reconstruction
 // s[0..5] = cells i-3..i+2 around face i-1/2; L = right value of cell i-1, R = left
@@ -42,8 +49,8 @@ template <class T> struct Rec {
     const T l = b - a, r = c - b;
     if (l * r <= T(0)) return T(0);
     const T m = T(0.5) * (c - a);
-    const T cap = T(2) * vmin(fabs(l), fabs(r));
-    return sgn(m) * vmin(fabs(m), cap);
+    const T cap = T(2) * vmin(vabs(l), vabs(r));
+    return sgn(m) * vmin(vabs(m), cap);
   }
   // CW84 interface, recon.hpp:42-44
   static CHQ_FN T edge(T qa, T qb, T da, T db) { return T(0.5) * (qa + qb) + (da - db) / T(6); }
@@ -63,7 +70,7 @@ template <class T> struct Rec {
       const T dl = (qm1 + qp1) - T(2) * q0;
       const T dr = (q0 + qp2) - T(2) * qp1;
       T lim = T(0);
-      if (d2 * dl > T(0) && d2 * dr > T(0)) lim = sgn(d2) * vmin(vmin(C * fabs(dl), C * fabs(dr)), fabs(d2));
+      if (d2 * dl > T(0) && d2 * dr > T(0)) lim = sgn(d2) * vmin(vmin(C * vabs(dl), C * vabs(dr)), vabs(d2));
       f = T(0.5) * (q0 + qp1) - lim / T(6);
     }
     return f;
@@ -79,12 +86,12 @@ template <class T> struct Rec {
       const T dr = (q + qp2) - T(2) * qp1;
       T lim = T(0);
       if (d2 * dc > T(0) && d2 * dl > T(0) && d2 * dr > T(0))
-        lim = sgn(d2) * vmin(vmin(C * fabs(dl), C * fabs(dr)), vmin(C * fabs(dc), fabs(d2)));
-      const T rho = (fabs(d2) > Lit<T>::rel * (fabs(q) + fabs(qm1) + fabs(qp1)) + Lit<T>::tiny) ? lim / d2 : T(0);
+        lim = sgn(d2) * vmin(vmin(C * vabs(dl), C * vabs(dr)), vmin(C * vabs(dc), vabs(d2)));
+      const T rho = (vabs(d2) > Lit<T>::rel * (vabs(q) + vabs(qm1) + vabs(qp1)) + Lit<T>::tiny) ? lim / d2 : T(0);
       if (rho < T(1) - Lit<T>::rel) { lo = q - rho * dm; hi = q + rho * dp; }
     } else {
-      if (fabs(dm) >= T(2) * fabs(dp)) lo = q - T(2) * dp;
-      if (fabs(dp) >= T(2) * fabs(dm)) hi = q + T(2) * dm;
+      if (vabs(dm) >= T(2) * vabs(dp)) lo = q - T(2) * dp;
+      if (vabs(dp) >= T(2) * vabs(dm)) hi = q + T(2) * dm;
     }
   }
   // WENO-Z right interface of the cell with 5-point stencil (a..e), recon.hpp:144-164
@@ -94,7 +101,7 @@ template <class T> struct Rec {
     const T s0 = k13 * sq((a + c) - T(2) * b) + T(0.25) * sq((T(3) * c + a) - T(4) * b);
     const T s1 = k13 * sq((b + d) - T(2) * c) + T(0.25) * sq(b - d);
     const T s2 = k13 * sq((e + c) - T(2) * d) + T(0.25) * sq((T(3) * c + e) - T(4) * d);
-    const T tau = fabs(s0 - s2);
+    const T tau = vabs(s0 - s2);
     const T w0 = T(0.1) * (T(1) + tau / (s0 + eps));
     const T w1 = T(0.6) * (T(1) + tau / (s1 + eps));
     const T w2 = T(0.3) * (T(1) + tau / (s2 + eps));
@@ -138,8 +145,8 @@ template <class T> struct Rec {
     (void)p2;
     const T dp = p3 - p1, dp2 = p4 - p0;
     const T pm = vmin(p1, p3);
-    if (!((fabs(dp) > T(0.33) * pm) && (um1 > up1))) return T(0);
-    const T r = (fabs(dp2) > Lit<T>::tiny) ? dp / dp2 : T(0);
+    if (!((vabs(dp) > T(0.33) * pm) && (um1 > up1))) return T(0);
+    const T r = (vabs(dp2) > Lit<T>::tiny) ? dp / dp2 : T(0);
     return vmin(T(1), vmax(T(0), T(10) * (r - T(0.75))));
   }
 };
@@ -156,7 +163,7 @@ template <class T> struct Rsv {
     const T gp = gamma * s.p;
     const T a = (gp + b2) / s.d;
     const T disc = a * a - T(4) * gp * bn * bn / (s.d * s.d);
-    return sqrt(T(0.5) * (a + sqrt(vmax(disc, T(0)))));
+    return vsqrt(T(0.5) * (a + vsqrt(vmax(disc, T(0)))));
   }
   // riemann.hpp:27-40
   static CHQ_FN void pflux(const St<T> &s, T bn, T gm1, T &e, T &pt, T f[NFLX]) {
@@ -180,7 +187,7 @@ template <class T> struct Rsv {
     const T cax2 = bn * bn / s.d;
     const T a = ca2 + v2;
     const T disc = a * a - T(4) * v2 * cax2;
-    return sqrt(T(0.5) * (a + sqrt(vmax(disc, T(0)))));
+    return vsqrt(T(0.5) * (a + vsqrt(vmax(disc, T(0)))));
   }
   // HLLE, riemann.hpp:55-68
   static CHQ_FN void hlle(const St<T> &L, const St<T> &R, T bn, T gamma, T gm1, T f[NFLX]) {
@@ -213,8 +220,8 @@ template <class T> struct Rsv {
                           T &ds, T &vs, T &ws, T &bys, T &bzs, T &es, T &vbs) {
     ds = (S.d * a) / bs;
     const T den = (S.d * a) * bs - bn2;
-    const T scale = fabs((S.d * a) * bs) + bn2;
-    if (fabs(den) <= Lit<T>::rel * scale + Lit<T>::tiny) {
+    const T scale = vabs((S.d * a) * bs) + bn2;
+    if (vabs(den) <= Lit<T>::rel * scale + Lit<T>::tiny) {
       vs = S.v; ws = S.w; bys = S.by; bzs = S.bz;
     } else {
       const T nv = (bn * (sm - S.u)) / den;
@@ -257,8 +264,8 @@ template <class T> struct Rsv {
     T drs, vrs, wrs, byrs, bzrs, ers, vbrs;
     star(L, al, sl - sm, bn, bn2, sm, ptl, el, pts, dls, vls, wls, byls, bzls, els, vbls);
     star(R, ar, sr - sm, bn, bn2, sm, ptr, er, pts, drs, vrs, wrs, byrs, bzrs, ers, vbrs);
-    const T qdl = sqrt(dls), qdr = sqrt(drs);
-    const T abn = fabs(bn);
+    const T qdl = vsqrt(dls), qdr = vsqrt(drs);
+    const T abn = vabs(bn);
     const T sls = sm - abn / qdl;
     const T srs = sm + abn / qdr;
     const T uls[NFLX] = {dls, dls * sm, dls * vls, dls * wls, els, byls, bzls};

@@ -41,46 +41,49 @@ struct VB {
 constexpr int COND_BLOCKS = 16384;   // grid of the conditional (fofc-branch) launches
 
 // ---------------------------------------------------------------- corner EMF
-__device__ __forceinline__ double upw(double flux, double left, double right) {   // physics.hpp:233-236
-  const double tol = Lit<double>::upwind_tol;
-  return (flux > tol) ? left : (flux < -tol) ? right : 0.5 * (left + right);
+template <class T>
+__device__ __forceinline__ T upw(T flux, T left, T right) {   // physics.hpp:233-236
+  const T tol = Lit<T>::upwind_tol;
+  return (flux > tol) ? left : (flux < -tol) ? right : T(0.5) * (left + right);
 }
 
 // 3D: one thread per (k, j, i) of the union box [ks..ke+1] x [js..je+1] x [is..ie+1];
 // e1 for i <= ie, e2 for j <= je, e3 for k <= ke (physics.hpp:265-296).
-__device__ __forceinline__ void corner3_at(const CornerArgs &a, int k, int j, int i) {
+template <class T>
+__device__ __forceinline__ void corner3_at(const CornerArgsT<T> &a, int k, int j, int i) {
   const Geom &g = a.g;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
-  const double *ecc = a.ecc;
+  const T *ecc = a.ecc;
   if (i <= g.ie) {   // E1 at (k-1/2, j-1/2, i)
-    const double *ex = ecc;
-    const double l3 = upw(a.f2[c - sk], a.e1x3[c - sj] - ex[c - sk - sj], a.e1x3[c] - ex[c - sk]);
-    const double r3 = upw(a.f2[c], a.e1x3[c - sj] - ex[c - sj], a.e1x3[c] - ex[c]);
-    const double l2 = upw(a.f3[c - sj], a.e1x2[c - sk] - ex[c - sk - sj], a.e1x2[c] - ex[c - sj]);
-    const double r2 = upw(a.f3[c], a.e1x2[c - sk] - ex[c - sk], a.e1x2[c] - ex[c]);
-    a.e1[c] = 0.25 * (((l3 + r3) + (l2 + r2)) + ((a.e1x2[c - sk] + a.e1x2[c]) + (a.e1x3[c - sj] + a.e1x3[c])));
+    const T *ex = ecc;
+    const T l3 = upw(a.f2[c - sk], a.e1x3[c - sj] - ex[c - sk - sj], a.e1x3[c] - ex[c - sk]);
+    const T r3 = upw(a.f2[c], a.e1x3[c - sj] - ex[c - sj], a.e1x3[c] - ex[c]);
+    const T l2 = upw(a.f3[c - sj], a.e1x2[c - sk] - ex[c - sk - sj], a.e1x2[c] - ex[c - sj]);
+    const T r2 = upw(a.f3[c], a.e1x2[c - sk] - ex[c - sk], a.e1x2[c] - ex[c]);
+    a.e1[c] = T(0.25) * (((l3 + r3) + (l2 + r2)) + ((a.e1x2[c - sk] + a.e1x2[c]) + (a.e1x3[c - sj] + a.e1x3[c])));
   }
   if (j <= g.je) {   // E2 at (k-1/2, j, i-1/2)
-    const double *ey = ecc + cs;
-    const double l3 = upw(a.f1[c - sk], a.e2x3[c - 1] - ey[c - sk - 1], a.e2x3[c] - ey[c - sk]);
-    const double r3 = upw(a.f1[c], a.e2x3[c - 1] - ey[c - 1], a.e2x3[c] - ey[c]);
-    const double l1 = upw(a.f3[c - 1], a.e2x1[c - sk] - ey[c - sk - 1], a.e2x1[c] - ey[c - 1]);
-    const double r1 = upw(a.f3[c], a.e2x1[c - sk] - ey[c - sk], a.e2x1[c] - ey[c]);
-    a.e2[c] = 0.25 * (((l3 + r3) + (l1 + r1)) + ((a.e2x3[c - 1] + a.e2x3[c]) + (a.e2x1[c - sk] + a.e2x1[c])));
+    const T *ey = ecc + cs;
+    const T l3 = upw(a.f1[c - sk], a.e2x3[c - 1] - ey[c - sk - 1], a.e2x3[c] - ey[c - sk]);
+    const T r3 = upw(a.f1[c], a.e2x3[c - 1] - ey[c - 1], a.e2x3[c] - ey[c]);
+    const T l1 = upw(a.f3[c - 1], a.e2x1[c - sk] - ey[c - sk - 1], a.e2x1[c] - ey[c - 1]);
+    const T r1 = upw(a.f3[c], a.e2x1[c - sk] - ey[c - sk], a.e2x1[c] - ey[c]);
+    a.e2[c] = T(0.25) * (((l3 + r3) + (l1 + r1)) + ((a.e2x3[c - 1] + a.e2x3[c]) + (a.e2x1[c - sk] + a.e2x1[c])));
   }
   if (k <= g.ke) {   // E3 at (k, j-1/2, i-1/2)
-    const double *ez = ecc + 2 * cs;
-    const double l2 = upw(a.f1[c - sj], a.e3x2[c - 1] - ez[c - sj - 1], a.e3x2[c] - ez[c - sj]);
-    const double r2 = upw(a.f1[c], a.e3x2[c - 1] - ez[c - 1], a.e3x2[c] - ez[c]);
-    const double l1 = upw(a.f2[c - 1], a.e3x1[c - sj] - ez[c - sj - 1], a.e3x1[c] - ez[c - 1]);
-    const double r1 = upw(a.f2[c], a.e3x1[c - sj] - ez[c - sj], a.e3x1[c] - ez[c]);
-    a.e3[c] = 0.25 * (((l2 + r2) + (l1 + r1)) + ((a.e3x2[c - 1] + a.e3x2[c]) + (a.e3x1[c - sj] + a.e3x1[c])));
+    const T *ez = ecc + 2 * cs;
+    const T l2 = upw(a.f1[c - sj], a.e3x2[c - 1] - ez[c - sj - 1], a.e3x2[c] - ez[c - sj]);
+    const T r2 = upw(a.f1[c], a.e3x2[c - 1] - ez[c - 1], a.e3x2[c] - ez[c]);
+    const T l1 = upw(a.f2[c - 1], a.e3x1[c - sj] - ez[c - sj - 1], a.e3x1[c] - ez[c - 1]);
+    const T r1 = upw(a.f2[c], a.e3x1[c - sj] - ez[c - sj], a.e3x1[c] - ez[c]);
+    a.e3[c] = T(0.25) * (((l2 + r2) + (l1 + r1)) + ((a.e3x2[c - 1] + a.e3x2[c]) + (a.e3x1[c - sj] + a.e3x1[c])));
   }
 }
 
 // 2D and 1D: E1/E2 are copies of the face EMFs on the two k-planes (physics.hpp:250-256);
 // E3 is the Gardiner-Stone average in 2D (288-296) or a copy in 1D (257-261).
-__device__ __forceinline__ void corner12_at(const CornerArgs &a, int j, int i) {
+template <class T>
+__device__ __forceinline__ void corner12_at(const CornerArgsT<T> &a, int j, int i) {
   const Geom &g = a.g;
   const int k = g.ks;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
@@ -89,19 +92,20 @@ __device__ __forceinline__ void corner12_at(const CornerArgs &a, int j, int i) {
   a.e2[c] = a.e2x1[c];
   a.e2[c + sk] = a.e2x1[c];
   if (g.two_d) {
-    const double *ez = a.ecc + 2 * cs;
-    const double l2 = upw(a.f1[c - sj], a.e3x2[c - 1] - ez[c - sj - 1], a.e3x2[c] - ez[c - sj]);
-    const double r2 = upw(a.f1[c], a.e3x2[c - 1] - ez[c - 1], a.e3x2[c] - ez[c]);
-    const double l1 = upw(a.f2[c - 1], a.e3x1[c - sj] - ez[c - sj - 1], a.e3x1[c] - ez[c - 1]);
-    const double r1 = upw(a.f2[c], a.e3x1[c - sj] - ez[c - sj], a.e3x1[c] - ez[c]);
-    a.e3[c] = 0.25 * (((l2 + r2) + (l1 + r1)) + ((a.e3x2[c - 1] + a.e3x2[c]) + (a.e3x1[c - sj] + a.e3x1[c])));
+    const T *ez = a.ecc + 2 * cs;
+    const T l2 = upw(a.f1[c - sj], a.e3x2[c - 1] - ez[c - sj - 1], a.e3x2[c] - ez[c - sj]);
+    const T r2 = upw(a.f1[c], a.e3x2[c - 1] - ez[c - 1], a.e3x2[c] - ez[c]);
+    const T l1 = upw(a.f2[c - 1], a.e3x1[c - sj] - ez[c - sj - 1], a.e3x1[c] - ez[c - 1]);
+    const T r1 = upw(a.f2[c], a.e3x1[c - sj] - ez[c - sj], a.e3x1[c] - ez[c]);
+    a.e3[c] = T(0.25) * (((l2 + r2) + (l1 + r1)) + ((a.e3x2[c - 1] + a.e3x2[c]) + (a.e3x1[c - sj] + a.e3x1[c])));
   } else if (j == g.js) {
     a.e3[c] = a.e3x1[c];
     a.e3[c + sj] = a.e3x1[c];
   }
 }
 
-__global__ void __launch_bounds__(BX * BY) k_corner(const CornerArgs a, const VB v) {
+template <class T>
+__global__ void __launch_bounds__(256) k_corner(const CornerArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
   for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
@@ -116,7 +120,8 @@ __global__ void __launch_bounds__(BX * BY) k_corner(const CornerArgs a, const VB
 // i..i+1) of every listed (flagged) cell; every other edge keeps its value, which is what a
 // whole-domain recompute would produce from unchanged inputs. Overlapping cells write the
 // same value to a shared edge.
-__global__ void __launch_bounds__(256) k_corner_list(const CornerArgs a, const int *list) {
+template <class T>
+__global__ void __launch_bounds__(256) k_corner_list(const CornerArgsT<T> a, const int *list) {
   if (!gate(a.ctrl, 1)) return;
   const Geom &g = a.g;
   const int n = a.ctrl->nbad * 8;
@@ -142,7 +147,8 @@ __global__ void __launch_bounds__(256) k_corner_list(const CornerArgs a, const i
 // One thread per (k, j, i) of [ks..ke+1] x [js..je+1] x [is..ie+1]: the conserved update of
 // the active cell (physics.hpp:314-322) and the CT update of its three lower faces
 // (physics.hpp:327-347). In-place (uo aliasing ua or ub) is pointwise safe.
-__device__ __forceinline__ void update_at(const UpdateArgs &a, const double bdt, int k, int j, int i) {
+template <class T>
+__device__ __forceinline__ void update_at(const UpdateArgsT<T> &a, const double bdt, int k, int j, int i) {
   const Geom &g = a.g;
   const double g0 = a.g0, g1 = a.g1;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
@@ -150,35 +156,36 @@ __device__ __forceinline__ void update_at(const UpdateArgs &a, const double bdt,
   if (i <= g.ie && j <= g.je && k <= g.ke) {
     for (int n = 0; n < a.nvar; ++n) {
       const long long o = n * cs + c;
-      double dv = (a.f1[o + 1] - a.f1[o]) * idx1;
-      if (g.two_d) dv += (a.f2[o + sj] - a.f2[o]) * idx2;
-      if (g.three_d) dv += (a.f3[o + sk] - a.f3[o]) * idx3;
+      double dv = (double(a.f1[o + 1]) - double(a.f1[o])) * idx1;
+      if (g.two_d) dv += (double(a.f2[o + sj]) - double(a.f2[o])) * idx2;
+      if (g.three_d) dv += (double(a.f3[o + sk]) - double(a.f3[o])) * idx3;
       a.uo[o] = g0 * a.ua[o] + g1 * a.ub[o] - bdt * dv;
     }
   }
   if (j <= g.je && k <= g.ke) {   // B1 on x1-faces
     double cr = 0.0;
-    if (g.two_d) cr += (a.e3[c + sj] - a.e3[c]) * idx2;
-    if (g.three_d) cr -= (a.e2[c + sk] - a.e2[c]) * idx3;
+    if (g.two_d) cr += (double(a.e3[c + sj]) - double(a.e3[c])) * idx2;
+    if (g.three_d) cr -= (double(a.e2[c + sk]) - double(a.e2[c])) * idx3;
     a.b1o[c] = g0 * a.b1a[c] + g1 * a.b1b[c] - bdt * cr;
   }
   if (i <= g.ie && k <= g.ke && (g.two_d || j == g.js)) {   // B2 on x2-faces
-    double cr = -(a.e3[c + 1] - a.e3[c]) * idx1;
-    if (g.three_d) cr += (a.e1[c + sk] - a.e1[c]) * idx3;
+    double cr = -(double(a.e3[c + 1]) - double(a.e3[c])) * idx1;
+    if (g.three_d) cr += (double(a.e1[c + sk]) - double(a.e1[c])) * idx3;
     const double v = g0 * a.b2a[c] + g1 * a.b2b[c] - bdt * cr;
     a.b2o[c] = v;
     if (!g.two_d) a.b2o[c + sj] = v;
   }
   if (i <= g.ie && j <= g.je && (g.three_d || k == g.ks)) {   // B3 on x3-faces
-    double cr = (a.e2[c + 1] - a.e2[c]) * idx1;
-    if (g.two_d) cr -= (a.e1[c + sj] - a.e1[c]) * idx2;
+    double cr = (double(a.e2[c + 1]) - double(a.e2[c])) * idx1;
+    if (g.two_d) cr -= (double(a.e1[c + sj]) - double(a.e1[c])) * idx2;
     const double v = g0 * a.b3a[c] + g1 * a.b3b[c] - bdt * cr;
     a.b3o[c] = v;
     if (!g.three_d) a.b3o[c + sk] = v;
   }
 }
 
-__global__ void __launch_bounds__(BX * BY) k_update(const UpdateArgs a, const VB v) {
+template <class T>
+__global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
   const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
@@ -316,8 +323,8 @@ __device__ __forceinline__ int wave_sum(int v) {
 // flag_mode 1 writes the flag array (reference semantics); 2 appends flagged cells to a list.
 // DT: reduce the CFL minimum. ENT: resynchronise u[IENT] (and w[IENT] if write_went); the
 // lean variant (no pow) serves RK stages whose u[IENT] nothing reads.
-template <bool DT, bool ENT>
-__device__ __forceinline__ void c2p_at(const C2pArgs &a, int k, int j, int i, double &mdt, int &nbad_local) {
+template <class T, bool DT, bool ENT>
+__device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i, double &mdt, int &nbad_local) {
   const Geom &g = a.g;
   const Eos e = a.eos;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
@@ -364,19 +371,19 @@ __device__ __forceinline__ void c2p_at(const C2pArgs &a, int k, int j, int i, do
       u[IENT * cs] = d * se;
     }
   }
-  double *w = a.w + c;
-  w[IDN * cs] = d;
-  w[IVX * cs] = vx;
-  w[IVY * cs] = vy;
-  w[IVZ * cs] = vz;
-  w[IPR * cs] = p;
-  if (ENT && a.write_went) w[IENT * cs] = se;
-  a.bcc[c] = bx;
-  a.bcc[c + cs] = by;
-  a.bcc[c + 2 * cs] = bz;
-  a.ecc[c] = vz * by - vy * bz;
-  a.ecc[c + cs] = vx * bz - vz * bx;
-  a.ecc[c + 2 * cs] = vy * bx - vx * by;
+  T *w = a.w + c;
+  w[IDN * cs] = T(d);
+  w[IVX * cs] = T(vx);
+  w[IVY * cs] = T(vy);
+  w[IVZ * cs] = T(vz);
+  w[IPR * cs] = T(p);
+  if (ENT && a.write_went) w[IENT * cs] = T(se);
+  a.bcc[c] = T(bx);
+  a.bcc[c + cs] = T(by);
+  a.bcc[c + 2 * cs] = T(bz);
+  a.ecc[c] = T(vz * by - vy * bz);
+  a.ecc[c + cs] = T(vx * bz - vz * bx);
+  a.ecc[c + 2 * cs] = T(vy * bx - vx * by);
   if (DT && active) {
     const double gp = e.gamma * p;
     const double aa = (gp + bsq) * id;
@@ -394,15 +401,15 @@ __device__ __forceinline__ void c2p_at(const C2pArgs &a, int k, int j, int i, do
 // cons -> prim over all cells, physics.hpp:437-494. The CFL minimum is reduced with wave64
 // shuffles and one LDS pass per block, then merged with a 64-bit atomicMin on the bit
 // pattern, issued only when the block's minimum beats the current one.
-template <bool DT, bool ENT>
-__global__ void __launch_bounds__(BX * BY) k_c2p(const C2pArgs a, const VB v) {
+template <class T, bool DT, bool ENT>
+__global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
   double mdt = 1e300;
   int nbad_local = 0;
   for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
     int k, j, i;
-    if (v.at(vb, k, j, i)) c2p_at<DT, ENT>(a, k, j, i, mdt, nbad_local);
+    if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT>(a, k, j, i, mdt, nbad_local);
   }
   if (!DT && a.flag_mode != 1) return;
   __shared__ double smin[BX * BY / 64];
@@ -500,24 +507,34 @@ VB vgrid(int nx, int ny, int nz) {
 int grid_of(const VB &v, bool conditional) { return conditional ? (v.n < COND_BLOCKS ? v.n : COND_BLOCKS) : v.n; }
 }  // namespace
 
-hipError_t launch_corner(const CornerArgs &a, hipStream_t s) {
+template <class T>
+hipError_t launch_corner(const CornerArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
   const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, g.three_d ? g.ke - g.ks + 2 : 1);
-  hipLaunchKernelGGL(k_corner, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
+  hipLaunchKernelGGL(k_corner<T>, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
   return hipGetLastError();
 }
 
-hipError_t launch_corner_list(const CornerArgs &a, const int *list, hipStream_t s) {
-  hipLaunchKernelGGL(k_corner_list, dim3(256), dim3(256), 0, s, a, list);
+template <class T>
+hipError_t launch_corner_list(const CornerArgsT<T> &a, const int *list, hipStream_t s) {
+  hipLaunchKernelGGL(k_corner_list<T>, dim3(256), dim3(256), 0, s, a, list);
   return hipGetLastError();
 }
 
-hipError_t launch_update(const UpdateArgs &a, hipStream_t s) {
+template <class T>
+hipError_t launch_update(const UpdateArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
   const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, g.ke - g.ks + 2);
-  hipLaunchKernelGGL(k_update, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
+  hipLaunchKernelGGL(k_update<T>, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
   return hipGetLastError();
 }
+
+template hipError_t launch_corner<double>(const CornerArgsT<double> &, hipStream_t);
+template hipError_t launch_corner<float>(const CornerArgsT<float> &, hipStream_t);
+template hipError_t launch_corner_list<double>(const CornerArgsT<double> &, const int *, hipStream_t);
+template hipError_t launch_corner_list<float>(const CornerArgsT<float> &, const int *, hipStream_t);
+template hipError_t launch_update<double>(const UpdateArgsT<double> &, hipStream_t);
+template hipError_t launch_update<float>(const UpdateArgsT<float> &, hipStream_t);
 
 hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s) {
   const Geom &g = a.g;
@@ -532,20 +549,24 @@ hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_c2p(const C2pArgs &a, hipStream_t s) {
+template <class T>
+hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
   const VB v = vgrid(g.ni, g.nj, g.nk);
   const dim3 grid(grid_of(v, a.only_if_bad)), block(256);
   const bool ent = a.entropy_out || a.eos.dual_energy;
   if (a.compute_dt) {
-    if (ent) hipLaunchKernelGGL((k_c2p<true, true>), grid, block, 0, s, a, v);
-    else hipLaunchKernelGGL((k_c2p<true, false>), grid, block, 0, s, a, v);
+    if (ent) hipLaunchKernelGGL((k_c2p<T, true, true>), grid, block, 0, s, a, v);
+    else hipLaunchKernelGGL((k_c2p<T, true, false>), grid, block, 0, s, a, v);
   } else {
-    if (ent) hipLaunchKernelGGL((k_c2p<false, true>), grid, block, 0, s, a, v);
-    else hipLaunchKernelGGL((k_c2p<false, false>), grid, block, 0, s, a, v);
+    if (ent) hipLaunchKernelGGL((k_c2p<T, false, true>), grid, block, 0, s, a, v);
+    else hipLaunchKernelGGL((k_c2p<T, false, false>), grid, block, 0, s, a, v);
   }
   return hipGetLastError();
 }
+
+template hipError_t launch_c2p<double>(const C2pArgsT<double> &, hipStream_t);
+template hipError_t launch_c2p<float>(const C2pArgsT<float> &, hipStream_t);
 
 hipError_t launch_flag(const C2pArgs &a, hipStream_t s) {
   const Geom &g = a.g;

@@ -41,13 +41,18 @@ typedef struct {
 /* Physics configuration: Config (physics.hpp:22-33) + the driver switches of main.cpp:112-163.
  * recon: 0 dc, 1 plm, 2 ppm, 3 wenoz, 4 ppmx (defs.hpp:35). rsolver: 0 hlle, 1 hlld,
  * 2 lhlld (physics.hpp:20). nstage: 1..4 = rk1, rk2, rk3, rk4/ssprk43 (defs.hpp:36).
- * bc[d]: 0 periodic, 1 outflow, 2 reflect (defs.hpp:37). */
+ * bc[d]: 0 periodic, 1 outflow, 2 reflect (defs.hpp:37).
+ * precision: 0 = FP64 throughout (the reference's arithmetic, bitwise); 1 = mixed: primitives,
+ * reconstruction, Riemann solve, fluxes and EMFs in FP32, con2prim, RK combine, CT and all sums
+ * in FP64 (conserved state and face B stay FP64). Mixed needs dual_energy = 0. No counterpart
+ * in the reference, which is FP64 only (defs.hpp:11 Real = double). */
 typedef struct {
   int recon, rsolver, nstage;
   double gamma, dfloor, pfloor, cfl;
   int flatten, low_mach, fofc, dual_energy;
   double de_eta, lhlld_phi;
   int bc[3];
+  int precision;
 } chq_config;
 
 /* Array identifiers for chq_upload / chq_download / chq_array_count. */
