@@ -6,5 +6,6 @@ The native library is required: nothing here falls back to a CPU or Python path.
 """
 from ._capi import LIB_PATH, NativeLibraryMissing, lib  # noqa: F401
 from .mesh import Mesh, make_config, make_mesh_desc  # noqa: F401
+from .batch import Batch  # noqa: F401
 
-__all__ = ["Mesh", "make_config", "make_mesh_desc", "lib", "LIB_PATH", "NativeLibraryMissing"]
+__all__ = ["Mesh", "Batch", "make_config", "make_mesh_desc", "lib", "LIB_PATH", "NativeLibraryMissing"]

@@ -104,6 +104,20 @@ def lib():
         "chq_deck_upload": (C.c_int, [vp, vp]),
         "chq_write_vtk": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_char_p, C.c_double, C.c_int]),
         "chq_history": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_int, dp]),
+        "chq_batch_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "chq_batch_destroy": (C.c_int, [vp]),
+        "chq_batch_add": (C.c_int, [vp, vp]),
+        "chq_batch_remove": (C.c_int, [vp, vp]),
+        "chq_batch_size": (C.c_int, [vp]),
+        "chq_batch_run": (vp, [vp, C.c_int]),
+        "chq_batch_step": (C.c_int, [vp, C.c_int]),
+        "chq_batch_step_timed": (C.c_int, [vp, C.c_int, dp]),
+        "chq_batch_sync": (C.c_int, [vp]),
+        "chq_batch_unfinished": (C.c_int, [vp, ip, ip]),
+        "chq_batch_use_graph": (C.c_int, [vp, C.c_int]),
+        "chq_batch_tune": (C.c_int, [vp, C.c_int, dp]),
+        "chq_batch_grid": (C.c_int, [vp, C.c_int, ip, C.POINTER(C.c_char_p)]),
+        "chq_batch_set_grid": (C.c_int, [vp, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -123,6 +137,9 @@ def exported_symbols():
         "chq_deck_open", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
         "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",
         "chq_write_vtk", "chq_history",
+        "chq_batch_create", "chq_batch_destroy", "chq_batch_add", "chq_batch_remove", "chq_batch_size",
+        "chq_batch_run", "chq_batch_step", "chq_batch_step_timed", "chq_batch_sync", "chq_batch_unfinished",
+        "chq_batch_use_graph", "chq_batch_tune", "chq_batch_grid", "chq_batch_set_grid",
     ]
 
 

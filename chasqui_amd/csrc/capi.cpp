@@ -9,224 +9,22 @@
 #include <string>
 #include <vector>
 
-#include "../../include/chasqui_amd.h"
-#include "kernels.hpp"
+#include "ctx.hpp"
 
 namespace chq {
 hipError_t launch_relayout(void *ref, void *dev, const Geom &g, int nc, int nk, int nj, int ni, int to_dev,
                            int elem_bytes, hipStream_t s);
 }
 
-using namespace chq;
-
 namespace {
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string &msg) {
+int chq::fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
-#define HIPCHK(expr)                                                                        \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return fail(CHQ_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
-// RK coefficient table (g0, g1, beta, cur, out), main.cpp:250-268
-struct Coef { double g0, g1, beta; int cur, out; };
-Coef rk_coef(int nstage, int s) {
-  if (nstage == 1) return {1, 0, 1, 0, 0};
-  if (nstage == 2) return s == 1 ? Coef{1, 0, 1, 0, 1} : Coef{0.5, 0.5, 0.5, 1, 0};
-  if (nstage == 3) {
-    if (s == 1) return {1, 0, 1, 0, 1};
-    if (s == 2) return {0.75, 0.25, 0.25, 1, 1};
-    return {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 1, 0};
-  }
-  if (s == 1) return {1, 0, 0.5, 0, 1};
-  if (s == 2) return {0, 1, 0.5, 1, 1};
-  if (s == 3) return {2.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 1, 1};
-  return {0, 1, 0.5, 1, 0};
-}
-}  // namespace
-
-// The arrays the flux / corner-EMF side works in, per precision: double for the FP64 path,
-// float for the mixed-precision path (the conserved state and face B are always double).
-template <class T>
-struct FieldSet {
-  T *w[2]{}, *bcc[2]{}, *ecc[2]{};   // two primitive sets (current / tentative)
-  T *flx[3]{}, *ef[6]{}, *ee[3]{};   // fluxes, face EMFs (e2x1 e3x1 e1x2 e3x2 e1x3 e2x3), edge EMFs
-};
-
-struct chq_ctx {
-  Geom g{};
-  Eos eos{};
-  chq_config cfg{};
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  char *arena = nullptr;
-  size_t arena_bytes = 0;
-  double *u[2]{}, *bf[2][3]{};
-  FieldSet<double> f64;      // FP64 path
-  FieldSet<float> f32;       // mixed precision (only one of the two sets is allocated)
-  bool mixed = false;        // primitives, fluxes and EMFs are FP32
-  int *flag = nullptr;
-  int *badlist = nullptr;    // flagged cells of the current stage (fused fofc path)
-  Ctrl *ctrl = nullptr;      // time loop
-  Ctrl *ctrl_api = nullptr;  // per-kernel entry points (always active)
-  void *staging = nullptr;   // device staging buffer for layout conversion
-  size_t staging_bytes = 0;
-  int wcur = 0;              // which primitive set (w, bcc, ecc) is current
-  bool entropy = false;      // transport IENT in the time loop (only needed with dual energy)
-  // per-kernel-class timing: event pairs recorded around launches of one class
-  int timing_class = -1;
-  std::vector<hipEvent_t> tev;   // begin/end pairs
-  size_t tev_used = 0;
-
-  void tmark(int cls, bool begin) {
-    if (cls != timing_class) return;
-    if (tev_used == tev.size()) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return;
-      tev.push_back(e);
-    }
-    if (!begin && (tev_used % 2) == 0) return;
-    (void)hipEventRecord(tev[tev_used++], stream);
-  }
-
-  // face-flux launch description for direction d from buffer set `cur`
-  FluxArgs flux_args(int d, int cur, int wset, const Ctrl *c) const {
-    FluxArgs a{};
-    a.g = g;
-    a.eos = eos;
-    a.flatten = cfg.flatten;
-    a.low_mach = cfg.low_mach;
-    a.entropy = 1;
-    a.lhlld_phi = cfg.lhlld_phi;
-    // physics.hpp:85-88
-    int il = g.is, iu = g.ie, jl = g.js, ju = g.je, kl = g.ks, ku = g.ke;
-    if (d == 0) iu = g.ie + 1; else if (g.nx1 > 1) { il = g.is - 1; iu = g.ie + 1; }
-    if (d == 1) ju = g.je + 1; else if (g.two_d) { jl = g.js - 1; ju = g.je + 1; }
-    if (d == 2) ku = g.ke + 1; else if (g.three_d) { kl = g.ks - 1; ku = g.ke + 1; }
-    a.il = il; a.iu = iu; a.jl = jl; a.ju = ju; a.kl = kl; a.ku = ku;
-    a.w = f64.w[wset];
-    a.bcc = f64.bcc[wset];
-    a.bn = bf[cur][d];
-    a.flx = f64.flx[d];
-    // et2 = e3x1 / e1x2 / e2x3 ; et1 = e2x1 / e3x2 / e1x3 (physics.hpp:93-94)
-    static const int et2_id[3] = {1, 2, 5}, et1_id[3] = {0, 3, 4};   // ef = e2x1 e3x1 e1x2 e3x2 e1x3 e2x3
-    a.et2 = f64.ef[et2_id[d]];
-    a.et1 = f64.ef[et1_id[d]];
-    a.flag = flag;
-    a.ctrl = c;
-    return a;
-  }
-  CornerArgs corner_args(int wset, const Ctrl *c, int only_if_bad) const {
-    CornerArgs a{};
-    a.g = g;
-    a.f1 = f64.flx[0]; a.f2 = f64.flx[1]; a.f3 = f64.flx[2];
-    a.e2x1 = f64.ef[0]; a.e3x1 = f64.ef[1]; a.e1x2 = f64.ef[2]; a.e3x2 = f64.ef[3]; a.e1x3 = f64.ef[4]; a.e2x3 = f64.ef[5];
-    a.ecc = f64.ecc[wset];
-    a.e1 = f64.ee[0]; a.e2 = f64.ee[1]; a.e3 = f64.ee[2];
-    a.ctrl = c;
-    a.only_if_bad = only_if_bad;
-    return a;
-  }
-  UpdateArgs update_args(int out, double g0, double g1, int nvar, const Ctrl *c, int only_if_bad) const {
-    UpdateArgs a{};
-    a.g = g;
-    a.g0 = g0; a.g1 = g1;
-    a.nvar = nvar;
-    a.f1 = f64.flx[0]; a.f2 = f64.flx[1]; a.f3 = f64.flx[2];
-    a.ua = u[0]; a.ub = u[1]; a.uo = u[out];
-    a.b1a = bf[0][0]; a.b2a = bf[0][1]; a.b3a = bf[0][2];
-    a.b1b = bf[1][0]; a.b2b = bf[1][1]; a.b3b = bf[1][2];
-    a.b1o = bf[out][0]; a.b2o = bf[out][1]; a.b3o = bf[out][2];
-    a.e1 = f64.ee[0]; a.e2 = f64.ee[1]; a.e3 = f64.ee[2];
-    a.ctrl = c;
-    a.only_if_bad = only_if_bad;
-    return a;
-  }
-  C2pArgs c2p_args(int buf, int wset, Ctrl *c) const {
-    C2pArgs a{};
-    a.g = g;
-    a.eos = eos;
-    a.b1 = bf[buf][0]; a.b2 = bf[buf][1]; a.b3 = bf[buf][2];
-    a.u = u[buf];
-    a.w = f64.w[wset]; a.bcc = f64.bcc[wset]; a.ecc = f64.ecc[wset];
-    a.write_went = 1;
-    a.entropy_out = 1;
-    a.flag = flag;
-    a.badlist = badlist;
-    a.ctrl = c;
-    return a;
-  }
-  hipError_t boundaries(int buf, int nvar, const Ctrl *c, int only_if_bad) {
-    BcArgs a{};
-    a.g = g;
-    a.nvar = nvar;
-    a.u = u[buf];
-    a.b1 = bf[buf][0]; a.b2 = bf[buf][1]; a.b3 = bf[buf][2];
-    a.ctrl = c;
-    a.only_if_bad = only_if_bad;
-    a.bc = cfg.bc[0];
-    hipError_t e = launch_bc(a, 0, stream);
-    if (e != hipSuccess) return e;
-    if (g.two_d) { a.bc = cfg.bc[1]; e = launch_bc(a, 1, stream); if (e != hipSuccess) return e; }
-    if (g.three_d) { a.bc = cfg.bc[2]; e = launch_bc(a, 2, stream); }
-    return e;
-  }
-  int ndir() const { return 1 + (g.two_d ? 1 : 0) + (g.three_d ? 1 : 0); }
-
-  template <class T> FluxArgsT<T> flux_args_t(const FieldSet<T> &f, int d, int cur, int wset, const Ctrl *c) const {
-    FluxArgs a0 = flux_args(d, cur, wset, c);
-    FluxArgsT<T> a;
-    a.g = a0.g; a.eos = a0.eos; a.flatten = a0.flatten; a.low_mach = a0.low_mach; a.entropy = a0.entropy;
-    a.lhlld_phi = a0.lhlld_phi;
-    a.il = a0.il; a.iu = a0.iu; a.jl = a0.jl; a.ju = a0.ju; a.kl = a0.kl; a.ku = a0.ku;
-    a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.bn = a0.bn; a.flx = f.flx[d];
-    static const int et2_id[3] = {1, 2, 5}, et1_id[3] = {0, 3, 4};
-    a.et2 = f.ef[et2_id[d]];
-    a.et1 = f.ef[et1_id[d]];
-    a.flag = flag;
-    a.ctrl = c;
-    return a;
-  }
-  template <class T> CornerArgsT<T> corner_args_t(const FieldSet<T> &f, int wset, const Ctrl *c, int oib) const {
-    CornerArgsT<T> a{};
-    a.g = g;
-    a.f1 = f.flx[0]; a.f2 = f.flx[1]; a.f3 = f.flx[2];
-    a.e2x1 = f.ef[0]; a.e3x1 = f.ef[1]; a.e1x2 = f.ef[2]; a.e3x2 = f.ef[3]; a.e1x3 = f.ef[4]; a.e2x3 = f.ef[5];
-    a.ecc = f.ecc[wset];
-    a.e1 = f.ee[0]; a.e2 = f.ee[1]; a.e3 = f.ee[2];
-    a.ctrl = c;
-    a.only_if_bad = oib;
-    return a;
-  }
-  template <class T> UpdateArgsT<T> update_args_t(const FieldSet<T> &f, int out, double g0, double g1, int nvar,
-                                                  const Ctrl *c, int oib) const {
-    UpdateArgs a0 = update_args(out, g0, g1, nvar, c, oib);
-    UpdateArgsT<T> a;
-    a.g = a0.g; a.g0 = g0; a.g1 = g1; a.beta = 0.0; a.beta_dt = 0.0; a.use_ctrl_dt = 0; a.nvar = nvar;
-    a.f1 = f.flx[0]; a.f2 = f.flx[1]; a.f3 = f.flx[2];
-    a.ua = a0.ua; a.ub = a0.ub; a.uo = a0.uo;
-    a.b1a = a0.b1a; a.b2a = a0.b2a; a.b3a = a0.b3a; a.b1b = a0.b1b; a.b2b = a0.b2b; a.b3b = a0.b3b;
-    a.b1o = a0.b1o; a.b2o = a0.b2o; a.b3o = a0.b3o;
-    a.e1 = f.ee[0]; a.e2 = f.ee[1]; a.e3 = f.ee[2];
-    a.ctrl = c;
-    a.only_if_bad = oib;
-    return a;
-  }
-  template <class T> C2pArgsT<T> c2p_args_t(const FieldSet<T> &f, int buf, int wset, Ctrl *c) const {
-    C2pArgs a0 = c2p_args(buf, wset, c);
-    C2pArgsT<T> a;
-    a.g = a0.g; a.eos = a0.eos; a.b1 = a0.b1; a.b2 = a0.b2; a.b3 = a0.b3; a.u = a0.u;
-    a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.ecc = f.ecc[wset];
-    a.write_went = 1; a.entropy_out = 1; a.compute_dt = 0; a.slot = 0; a.flag_mode = 0;
-    a.flag = flag; a.badlist = badlist; a.ctrl = c; a.only_if_bad = 0;
-    return a;
-  }
-};
 
 namespace {
 
@@ -267,7 +65,21 @@ bool array_desc(const chq_ctx *c, int id, void **dev, int *nc, int *face, int *e
   return true;
 }
 
+// The time loop flips the current primitive set on the device, and only in stages that ran
+// (a run that reached tlim stops flipping): read it back before the host uses it.
+int sync_wcur(chq_ctx *c) {
+  if (!c->wcur_stale) return CHQ_OK;
+  int v = 0;
+  HIPCHK(hipMemcpyAsync(&v, &c->ctrl->wcur, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->wcur = v;
+  c->wcur_stale = false;
+  return CHQ_OK;
+}
+
 int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
+  if (c->batch_stream) HIPCHK(hipStreamSynchronize(c->batch_stream));
+  if (int rc = sync_wcur(c)) return rc;
   void *dev;
   int nc, face, eb, wide;
   if (!array_desc(c, id, &dev, &nc, &face, &eb, &wide)) return fail(CHQ_EINVAL, "unknown array id");
@@ -410,6 +222,7 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
 
 int chq_destroy(chq_ctx *c) {
   if (!c) return CHQ_OK;
+  if (c->batch) (void)chq_batch_remove(static_cast<chq_batch *>(c->batch), c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->arena) (void)hipFree(c->arena);
   if (c->staging) (void)hipFree(c->staging);
@@ -455,6 +268,7 @@ int chq_download(chq_ctx *c, int id, void *host, size_t count) {
 // ---------------------------------------------------------------- per-kernel entry points
 int chq_fluxes(chq_ctx *c, int dir, int cur) {
   if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
   if (c->mixed) {   // FP32 fluxes; the entropy component is not computed on this path
     FluxArgsT<float> a = c->flux_args_t(c->f32, dir, cur, c->wcur, c->ctrl_api);
@@ -469,6 +283,7 @@ int chq_fluxes(chq_ctx *c, int dir, int cur) {
 
 int chq_corner_emf(chq_ctx *c) {
   if (!c) return fail(CHQ_EINVAL, "null context");
+  if (int rc_ = sync_wcur(c)) return rc_;
   if (c->mixed) HIPCHK(launch_corner<float>(c->corner_args_t(c->f32, c->wcur, c->ctrl_api, 0), c->stream));
   else HIPCHK(launch_corner<double>(c->corner_args(c->wcur, c->ctrl_api, 0), c->stream));
   return CHQ_OK;
@@ -498,6 +313,7 @@ int chq_boundaries(chq_ctx *c, int buf) {
 
 int chq_cons_to_prim(chq_ctx *c, int buf, int compute_dt, double *min_dt) {
   if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 1, c->stream));
   if (c->mixed) {
     C2pArgsT<float> a = c->c2p_args_t(c->f32, buf, c->wcur, c->ctrl_api);
@@ -523,6 +339,7 @@ int chq_cons_to_prim(chq_ctx *c, int buf, int compute_dt, double *min_dt) {
 
 int chq_fofc_flag(chq_ctx *c, int buf, int *count) {
   if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_fofc_flag: FP64 contexts only");
   C2pArgs a = c->c2p_args(buf, c->wcur, c->ctrl_api);
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 0, c->stream));
@@ -536,6 +353,7 @@ int chq_fofc_flag(chq_ctx *c, int buf, int *count) {
 
 int chq_fofc_repair(chq_ctx *c, int dir, int cur) {
   if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_fofc_repair: FP64 contexts only");
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
   FluxArgs a = c->flux_args(dir, cur, c->wcur, c->ctrl_api);
@@ -547,6 +365,7 @@ int chq_fofc_repair(chq_ctx *c, int dir, int cur) {
 // device when the flag kernel counted cells)
 int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt, long *nbad) {
   if (!c || (cur & ~1) || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_stage: FP64 contexts only");
   const Ctrl *k = c->ctrl_api;
   const Recon rc = Recon(c->cfg.recon);
@@ -578,6 +397,7 @@ int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt
 // ---------------------------------------------------------------- time loop
 int chq_init(chq_ctx *c, double t, double tlim, int nlim) {
   if (!c) return fail(CHQ_EINVAL, "null context");
+  if (int rc_ = sync_wcur(c)) return rc_;
   HIPCHK(c->boundaries(0, 6, c->ctrl_api, 0));
   double mdt = 0.0;
   int rc = chq_cons_to_prim(c, 0, 1, &mdt);
@@ -589,6 +409,7 @@ int chq_init(chq_ctx *c, double t, double tlim, int nlim) {
   h.cfl = c->cfg.cfl;
   h.min_dt = std::fmin(mdt, 1e300);   // main.cpp:201-202
   h.active = 0;
+  h.wcur = c->wcur;
   HIPCHK(hipMemcpyAsync(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return CHQ_OK;
@@ -710,11 +531,14 @@ static int enqueue_step(chq_ctx *c) {
     HIPCHK(launch_ctrl_end_stage(k, last, fused ? 1 : 0, c->stream));
   }
   HIPCHK(launch_ctrl_end_step(k, c->stream));
+  c->wcur_stale = fused;
   return CHQ_OK;
 }
 
 int chq_step(chq_ctx *c, int nsteps) {
   if (!c || nsteps < 0) return fail(CHQ_EINVAL, "bad argument");
+  if (c->batch) return fail(CHQ_EINVAL, "this context is advanced by a batch");
+  if (int rc_ = sync_wcur(c)) return rc_;
   for (int n = 0; n < nsteps; ++n) {
     const int rc = enqueue_step(c);
     if (rc) return rc;
@@ -724,6 +548,7 @@ int chq_step(chq_ctx *c, int nsteps) {
 
 int chq_step_timed(chq_ctx *c, int nsteps, double *ms) {
   if (!c || nsteps < 0 || !ms) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   const int rc = chq_step(c, nsteps);
   if (rc) return rc;
@@ -762,6 +587,7 @@ int chq_kernel_timing_read(chq_ctx *c, double *total_ms, int *launches) {
 
 int chq_get_status(chq_ctx *c, chq_status *st) {
   if (!c || !st) return fail(CHQ_EINVAL, "bad argument");
+  if (c->batch_stream) HIPCHK(hipStreamSynchronize(c->batch_stream));
   Ctrl h{};
   HIPCHK(hipMemcpyAsync(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -48,7 +48,7 @@ struct Ctrl {
   int cycle, nlim;
   int nbad;                      // fofc flags of the current stage
   int nbad_last;                 // flags of the last stage (selects the repaired minimum)
-  int pad;
+  int wcur;                      // current primitive set (batched runs; solo contexts track it on the host)
   long long fofc_total;
 };
 

@@ -50,4 +50,12 @@ hipError_t launch_repair(const FluxArgs &a, int dir, bool only_if_bad, hipStream
   return hipGetLastError();
 }
 
+hipError_t launchb_flux(const RunArgs<double> *R, ItemList L, int dir, int cur, Recon rc, RSolver rs, int grid, hipStream_t s) {
+  return fluxk::launchb_flux_t<double>(R, L, dir, cur, rc, rs, grid, s);
+}
+
+hipError_t launchb_repair(const RunArgs<double> *R, CondList C, int cur, int grid, hipStream_t s) {
+  return fluxk::launchb_repair_t<double>(R, C, cur, grid, s);
+}
+
 }  // namespace chq

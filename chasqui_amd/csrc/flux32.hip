@@ -21,4 +21,12 @@ hipError_t launch_repair_list(const FluxArgsT<float> a[3], int ndir, const int *
   return fluxk::launch_repair_list_t<float>(a, ndir, list, s);
 }
 
+hipError_t launchb_flux(const RunArgs<float> *R, ItemList L, int dir, int cur, Recon rc, RSolver rs, int grid, hipStream_t s) {
+  return fluxk::launchb_flux_t<float>(R, L, dir, cur, rc, rs, grid, s);
+}
+
+hipError_t launchb_repair(const RunArgs<float> *R, CondList C, int cur, int grid, hipStream_t s) {
+  return fluxk::launchb_repair_t<float>(R, C, cur, grid, s);
+}
+
 }  // namespace chq

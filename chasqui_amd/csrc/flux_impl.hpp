@@ -20,10 +20,6 @@
 namespace chq {
 namespace fluxk {
 
-constexpr int BX = 64, BY = 4;
-constexpr int SEG = 16;   // faces per thread in the y/z march
-
-template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
 
 // Riemann solve of one face from its two half-states and store (physics.hpp:167-224).
 // c = face index (device layout); sd = stride along DIR.
@@ -106,15 +102,13 @@ __device__ __forceinline__ void var_ptrs(const FluxArgsT<T> &a, const T *p[NV]) 
 }
 
 // ---------------------------------------------------------------- x faces: lane = cell
+// One wave's row segment of x faces (lane = cell).
 template <class T, Recon RC, RSolver RS, bool ENT>
-__global__ void __launch_bounds__(256) k_flux_x(const FluxArgsT<T> a, const int nseg, const int nwaves) {
+__device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nseg, const int wave) {
   constexpr int NV = ENT ? 8 : 7;
   constexpr bool W = wide_v<RC>;
   constexpr int H = W ? 3 : 2;            // lanes before the first face: cell halo + 1
-  constexpr int FPW = 64 - H - (W ? 2 : 1);   // faces per wave
-  if (!a.ctrl->active) return;
-  const int wave = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
-  if (wave >= nwaves) return;
+  constexpr int FPW = fpw_v<RC>;
   const int lane = int(threadIdx.x) & 63;
   const int seg = wave % nseg, row = wave / nseg;
   const int nrow_j = a.ju - a.jl + 1;
@@ -182,19 +176,29 @@ __global__ void __launch_bounds__(256) k_flux_x(const FluxArgsT<T> a, const int 
   solve_store<T, 0, RS, ENT, NV>(a, row0 + i, 1, Lface, Rf);
 }
 
+template <class T, Recon RC, RSolver RS, bool ENT>
+__global__ void __launch_bounds__(256) k_flux_x(const FluxArgsT<T> a, const int nseg, const int nwaves) {
+  if (!a.ctrl->active) return;
+  const int wave = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  flux_x_wave<T, RC, RS, ENT>(a, nseg, wave);
+}
+
 // ---------------------------------------------------------------- y/z faces: register march
 template <class T, int DIR, Recon RC, RSolver RS, bool ENT>
-__global__ void __launch_bounds__(BX * BY) k_flux_m(const FluxArgsT<T> a) {
+__device__ __forceinline__ void flux_m_tile(const FluxArgsT<T> &a, const int bx, const int by, const int bz) {
   constexpr int NV = ENT ? 8 : 7;
   constexpr bool W = wide_v<RC>;
-  if (!a.ctrl->active) return;
   const Geom &g = a.g;
-  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int t = int(blockIdx.y) * BY + int(threadIdx.y);     // transverse index: k (DIR 1) or j (DIR 2)
+  const int i = a.il + bx * BX + int(threadIdx.x);
+  const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
+  // transverse index (k for DIR 1, j for DIR 2) and segment along DIR
+  const int t = nt == 1 ? 0 : by * BY + int(threadIdx.y);
+  const int sg = nt == 1 ? bz * BY + int(threadIdx.y) : bz;
   const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
-  const int F0 = lo_d + int(blockIdx.z) * SEG;
+  const int F0 = lo_d + sg * SEG;
   const int F1 = min(F0 + SEG - 1, hi_d);
-  if (i > a.iu) return;
+  if (i > a.iu || F0 > hi_d) return;
   int j, k;
   if constexpr (DIR == 1) { k = a.kl + t; j = 0; if (k > a.ku) return; }
   else { j = a.jl + t; k = 0; if (j > a.ju) return; }
@@ -290,6 +294,12 @@ __global__ void __launch_bounds__(BX * BY) k_flux_m(const FluxArgsT<T> a) {
   }
 }
 
+template <class T, int DIR, Recon RC, RSolver RS, bool ENT>
+__global__ void __launch_bounds__(BX * BY) k_flux_m(const FluxArgsT<T> a) {
+  if (!a.ctrl->active) return;
+  flux_m_tile<T, DIR, RC, RS, ENT>(a, int(blockIdx.x), int(blockIdx.y), int(blockIdx.z));
+}
+
 // Donor-cell HLLE flux and face EMFs of one face from the stage-input primitives
 // (fofc.hpp:76-88).
 template <class T, int DIR>
@@ -344,15 +354,13 @@ hipError_t go_flux(const FluxArgsT<T> &a, hipStream_t s) {
   const int nx = a.iu - a.il + 1, ny = a.ju - a.jl + 1, nz = a.ku - a.kl + 1;
   if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
   if constexpr (DIR == 0) {
-    constexpr bool W = wide_v<RC>;
-    constexpr int FPW = 64 - (W ? 3 : 2) - (W ? 2 : 1);
-    const int nseg = div_up(nx, FPW);
-    const int nwaves = nseg * ny * nz;
+    int nseg, nwaves;
+    x_shape(a, fpw_v<RC>, nseg, nwaves);
     hipLaunchKernelGGL((k_flux_x<T, RC, RS, ENT>), dim3(div_up(nwaves, 4)), dim3(256), 0, s, a, nseg, nwaves);
   } else {
-    const int nt = DIR == 1 ? nz : ny, nd = DIR == 1 ? ny : nz;
-    dim3 grid(div_up(nx, BX), div_up(nt, BY), div_up(nd, SEG)), block(BX, BY, 1);
-    hipLaunchKernelGGL((k_flux_m<T, DIR, RC, RS, ENT>), grid, block, 0, s, a);
+    int gx, gy, gz;
+    m_shape<T, DIR>(a, gx, gy, gz);
+    hipLaunchKernelGGL((k_flux_m<T, DIR, RC, RS, ENT>), dim3(gx, gy, gz), dim3(BX, BY, 1), 0, s, a);
   }
   return hipGetLastError();
 }
@@ -388,6 +396,103 @@ template <class T>
 hipError_t launch_repair_list_t(const FluxArgsT<T> a[3], int ndir, const int *list, hipStream_t s) {
   hipLaunchKernelGGL(k_repair_list<T>, dim3(256), dim3(256), 0, s, a[0], a[ndir > 1 ? 1 : 0], a[ndir > 2 ? 2 : 0],
                      list, ndir);
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------- batched launches
+// Grid-stride walks over (run, tile) lists of a batch; the tile bodies are the ones above,
+// with the run's launch description for its current primitive set (Ctrl::wcur).
+template <class T, Recon RC, RSolver RS>
+__global__ void __launch_bounds__(256) kb_flux_x(const RunArgs<T> *R, const ItemList L, const int cur) {
+  const int n = *L.n;
+  for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
+    const Item m = L.item[it];
+    const Ctrl *ctl = R[m.run].ctrl;
+    if (!ctl->active) continue;
+    const FluxArgsT<T> a = R[m.run].flux[cur][ctl->wcur][0];
+    int nseg, nwaves;
+    x_shape(a, fpw_v<RC>, nseg, nwaves);
+    const int wave = m.tile * 4 + int(threadIdx.x >> 6);
+    if (wave < nwaves) flux_x_wave<T, RC, RS, false>(a, nseg, wave);
+  }
+}
+
+template <class T, int DIR, Recon RC, RSolver RS>
+__global__ void __launch_bounds__(BX * BY) kb_flux_m(const RunArgs<T> *R, const ItemList L, const int cur) {
+  const int n = *L.n;
+  for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
+    const Item m = L.item[it];
+    const Ctrl *ctl = R[m.run].ctrl;
+    if (!ctl->active) continue;
+    const FluxArgsT<T> a = R[m.run].flux[cur][ctl->wcur][DIR];
+    int gx, gy, gz;
+    m_shape<T, DIR>(a, gx, gy, gz);
+    const int bx = m.tile % gx, rest = m.tile / gx;
+    flux_m_tile<T, DIR, RC, RS, false>(a, bx, rest % gy, rest / gy);
+  }
+}
+
+// fofc repair of the flagged runs (compacted list, tiles of 256 (cell, face) entries)
+template <class T>
+__global__ void __launch_bounds__(256) kb_repair(const RunArgs<T> *R, const CondList C, const int cur) {
+  const int nrun = *C.nrun();
+  if (nrun == 0) return;
+  const int *pre = C.pre(CL_REPAIR);
+  const int total = pre[nrun];
+  for (int it = int(blockIdx.x); it < total; it += int(gridDim.x)) {
+    const int p = locate(pre, nrun, it);
+    const int r = C.run()[p];
+    const Ctrl *ctl = R[r].ctrl;
+    const int wc = ctl->wcur, per = 2 * R[r].ndir;
+    const int t = (it - pre[p]) * 256 + int(threadIdx.x);
+    if (t >= ctl->nbad * per) continue;
+    const FluxArgsT<T> *fa = R[r].flux[cur][wc];
+    const Geom &g = fa[0].g;
+    const long long c0 = R[r].badlist[t / per];
+    const int q = t % per, dir = q >> 1, side = q & 1;
+    if (dir == 0) repair_face<T, 0>(fa[0], c0 + side, 1);
+    else if (dir == 1) repair_face<T, 1>(fa[1], c0 + side * (long long)g.pitch, g.pitch);
+    else repair_face<T, 2>(fa[2], c0 + side * g.plane, g.plane);
+  }
+}
+
+template <class T, int DIR, Recon RC>
+hipError_t gob_flux_rs(const RunArgs<T> *R, ItemList L, int cur, RSolver rs, int grid, hipStream_t s) {
+  if constexpr (DIR == 0) {
+    if (rs == RSolver::hlld) hipLaunchKernelGGL((kb_flux_x<T, RC, RSolver::hlld>), dim3(grid), dim3(256), 0, s, R, L, cur);
+    else if (rs == RSolver::lhlld) hipLaunchKernelGGL((kb_flux_x<T, RC, RSolver::lhlld>), dim3(grid), dim3(256), 0, s, R, L, cur);
+    else hipLaunchKernelGGL((kb_flux_x<T, RC, RSolver::hlle>), dim3(grid), dim3(256), 0, s, R, L, cur);
+  } else {
+    const dim3 b(BX, BY, 1);
+    if (rs == RSolver::hlld) hipLaunchKernelGGL((kb_flux_m<T, DIR, RC, RSolver::hlld>), dim3(grid), b, 0, s, R, L, cur);
+    else if (rs == RSolver::lhlld) hipLaunchKernelGGL((kb_flux_m<T, DIR, RC, RSolver::lhlld>), dim3(grid), b, 0, s, R, L, cur);
+    else hipLaunchKernelGGL((kb_flux_m<T, DIR, RC, RSolver::hlle>), dim3(grid), b, 0, s, R, L, cur);
+  }
+  return hipGetLastError();
+}
+
+template <class T, int DIR>
+hipError_t gob_flux_rc(const RunArgs<T> *R, ItemList L, int cur, Recon rc, RSolver rs, int grid, hipStream_t s) {
+  switch (rc) {
+    case Recon::dc: return gob_flux_rs<T, DIR, Recon::dc>(R, L, cur, rs, grid, s);
+    case Recon::plm: return gob_flux_rs<T, DIR, Recon::plm>(R, L, cur, rs, grid, s);
+    case Recon::ppm: return gob_flux_rs<T, DIR, Recon::ppm>(R, L, cur, rs, grid, s);
+    case Recon::ppmx: return gob_flux_rs<T, DIR, Recon::ppmx>(R, L, cur, rs, grid, s);
+    default: return gob_flux_rs<T, DIR, Recon::wenoz>(R, L, cur, rs, grid, s);
+  }
+}
+
+template <class T>
+hipError_t launchb_flux_t(const RunArgs<T> *R, ItemList L, int dir, int cur, Recon rc, RSolver rs, int grid, hipStream_t s) {
+  if (dir == 0) return gob_flux_rc<T, 0>(R, L, cur, rc, rs, grid, s);
+  if (dir == 1) return gob_flux_rc<T, 1>(R, L, cur, rc, rs, grid, s);
+  return gob_flux_rc<T, 2>(R, L, cur, rc, rs, grid, s);
+}
+
+template <class T>
+hipError_t launchb_repair_t(const RunArgs<T> *R, CondList C, int cur, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(kb_repair<T>, dim3(grid), dim3(256), 0, s, R, C, cur);
   return hipGetLastError();
 }
 

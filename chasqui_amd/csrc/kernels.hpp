@@ -88,6 +88,116 @@ struct C2pArgsT {
 };
 using C2pArgs = C2pArgsT<double>;
 
+// ---------------------------------------------------------------- work decomposition
+// The memory-bound kernels walk "virtual blocks": chunks of 256 consecutive points of one
+// (j, i) plane of an nx x ny box (rows are linearised, so rows of 257 or 262 points waste no
+// lanes), times nz planes.
+struct VB {
+  int nx, ny, cpp, n;   // plane size, chunks per plane, total chunks
+  __device__ __forceinline__ bool at(int vb, int &k, int &j, int &i) const {
+    k = vb / cpp;
+    const int t = (vb - k * cpp) * 256 + int(threadIdx.x);
+    j = t / nx;
+    i = t - j * nx;
+    return t < nx * ny;
+  }
+};
+inline __host__ __device__ VB vgrid(int nx, int ny, int nz) {
+  VB v;
+  v.nx = nx;
+  v.ny = ny;
+  v.cpp = div_up(nx * ny, 256);
+  v.n = v.cpp * nz;
+  return v;
+}
+// corner EMF: the union box [is..ie+1] x [js..je+1] x [ks..ke+1] (one k plane below 3D)
+inline __host__ __device__ VB corner_vb(const Geom &g) {
+  return vgrid(g.ie - g.is + 2, g.je - g.js + 2, g.three_d ? g.ke - g.ks + 2 : 1);
+}
+// update: active cells plus the upper faces of the present directions
+inline __host__ __device__ VB update_vb(const Geom &g) {
+  return vgrid(g.ie - g.is + 2, g.two_d ? g.je - g.js + 2 : 1, g.three_d ? g.ke - g.ks + 2 : 1);
+}
+// con2prim: every cell, ghosts included
+inline __host__ __device__ VB c2p_vb(const Geom &g) { return vgrid(g.ni, g.nj, g.nk); }
+// boundaries of direction d: threads over (k, j, ghost), (k, ghost, i) or (ghost, j, i)
+inline __host__ __device__ long long bc_points(const Geom &g, int d) {
+  if (d == 0) return (long long)(g.nk + 1) * (g.nj + 1) * g.ng;
+  if (d == 1) return (long long)(g.nk + 1) * g.ng * (g.ni + 1);
+  return (long long)g.ng * (g.nj + 1) * (g.ni + 1);
+}
+
+// face-flux decomposition (flux_impl.hpp): x faces by wave64 row segments, y/z faces by
+// BX x BY thread tiles marching SEG faces along the direction
+namespace fluxk {
+constexpr int BX = 64, BY = 4;
+constexpr int SEG = 16;   // faces per thread in the y/z march
+
+template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
+inline __host__ __device__ int fpw_of(Recon rc) {
+  const bool w = !(rc == Recon::dc || rc == Recon::plm);
+  return 64 - (w ? 3 : 2) - (w ? 2 : 1);
+}
+template <Recon RC> constexpr int fpw_v = 64 - (wide_v<RC> ? 3 : 2) - (wide_v<RC> ? 2 : 1);   // faces per wave
+
+// x-face work: nseg row segments of FPW faces per row, one wave each; 4 waves per block
+template <class T>
+__host__ __device__ inline void x_shape(const FluxArgsT<T> &a, int fpw, int &nseg, int &nwaves) {
+  nseg = div_up(a.iu - a.il + 1, fpw);
+  nwaves = nseg * (a.ju - a.jl + 1) * (a.ku - a.kl + 1);
+}
+
+// y/z-face work: tiles of BX faces in i x BY rows x SEG faces along DIR. When there is one
+// transverse row (2D y faces, DIR 1 with one k plane) the BY thread rows take consecutive
+// segments instead, so no thread row idles.
+template <class T, int DIR>
+__host__ __device__ inline void m_shape(const FluxArgsT<T> &a, int &gx, int &gy, int &gz) {
+  const int nx = a.iu - a.il + 1;
+  const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
+  const int nd = DIR == 1 ? a.ju - a.jl + 1 : a.ku - a.kl + 1;
+  gx = div_up(nx, BX);
+  if (nt == 1) { gy = 1; gz = div_up(div_up(nd, SEG), BY); }
+  else { gy = div_up(nt, BY); gz = div_up(nd, SEG); }
+}
+
+}  // namespace fluxk
+
+// ---------------------------------------------------------------- batches
+// Many independent meshes advanced by the same launches. Every run keeps its own arena and
+// control block; the launches walk device-resident work lists of (run, tile) items, so a
+// captured step graph stays valid while runs join and leave (only list contents change).
+struct Item { int run, tile; };
+struct ItemList {
+  const Item *item;
+  const int *n;        // number of items (device memory)
+};
+
+// every launch description of one run, for each (stage-input buffer, primitive set)
+template <class T>
+struct RunArgs {
+  FluxArgsT<T> flux[2][2][3];    // [input buffer][primitive set][dir]
+  CornerArgsT<T> corner[2];      // [primitive set]
+  UpdateArgsT<T> update[2];      // [output buffer]; coefficients come with the launch
+  C2pArgsT<T> c2p[2][2];         // [buffer][primitive set written]
+  BcArgs bc[2][3];               // [buffer][dir]
+  Ctrl *ctrl;
+  int *badlist;                  // cells flagged by con2prim in the current stage
+  int ndir;
+  int cond_tiles[5];             // tiles of the whole-mesh fofc re-runs: update, bc x1..x3, c2p
+};
+
+// The fofc branch of a batch: the runs that flagged cells in the current stage, compacted on
+// the device, with per-class tile prefixes (classes below).
+enum : int { CL_REPAIR = 0, CL_CORNER, CL_UPDATE, CL_BC1, CL_BC2, CL_BC3, CL_C2P, CL_N };
+struct CondList {
+  int cap;             // capacity in runs
+  int *base;           // [0] = number of flagged runs, then run[cap], then pre[CL_N][cap + 1]
+  __host__ __device__ int *nrun() const { return base; }
+  __host__ __device__ int *run() const { return base + 1; }
+  __host__ __device__ int *pre(int cls) const { return base + 1 + cap + cls * (cap + 1); }
+  static size_t ints(int cap) { return size_t(1) + cap + size_t(CL_N) * (cap + 1); }
+};
+
 // launches (defined in stage.hip / flux.hip / flux32.hip)
 hipError_t launch_flux(const FluxArgsT<double> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s);
 hipError_t launch_flux(const FluxArgsT<float> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s);
@@ -105,5 +215,35 @@ hipError_t launch_ctrl_begin_step(Ctrl *c, hipStream_t s);
 hipError_t launch_ctrl_begin_stage(Ctrl *c, int reset_slots, hipStream_t s);
 hipError_t launch_ctrl_end_stage(Ctrl *c, int last, int fused, hipStream_t s);
 hipError_t launch_ctrl_end_step(Ctrl *c, hipStream_t s);
+
+// batched launches: `grid` blocks walk the list (grid-stride); stage coefficients are
+// launch arguments, everything per run comes from the RunArgs table
+hipError_t launchb_flux(const RunArgs<double> *R, ItemList L, int dir, int cur, Recon rc, RSolver rs, int grid, hipStream_t s);
+hipError_t launchb_flux(const RunArgs<float> *R, ItemList L, int dir, int cur, Recon rc, RSolver rs, int grid, hipStream_t s);
+hipError_t launchb_repair(const RunArgs<double> *R, CondList C, int cur, int grid, hipStream_t s);
+hipError_t launchb_repair(const RunArgs<float> *R, CondList C, int cur, int grid, hipStream_t s);
+template <class T> hipError_t launchb_corner(const RunArgs<T> *R, ItemList L, int grid, hipStream_t s);
+template <class T> hipError_t launchb_update(const RunArgs<T> *R, ItemList L, int out, double g0, double g1, double beta, int grid, hipStream_t s);
+template <class T> hipError_t launchb_bc(const RunArgs<T> *R, ItemList L, int dir, int buf, int grid, hipStream_t s);
+template <class T> hipError_t launchb_c2p(const RunArgs<T> *R, ItemList L, int buf, int last, int grid, hipStream_t s);
+template <class T> hipError_t launchb_fofc_scan(const RunArgs<T> *R, const int *nruns, CondList C, hipStream_t s);
+template <class T> hipError_t launchb_corner_cond(const RunArgs<T> *R, CondList C, int grid, hipStream_t s);
+template <class T> hipError_t launchb_update_cond(const RunArgs<T> *R, CondList C, int out, double g0, double g1, double beta, int grid, hipStream_t s);
+template <class T> hipError_t launchb_bc_cond(const RunArgs<T> *R, CondList C, int dir, int buf, int grid, hipStream_t s);
+template <class T> hipError_t launchb_c2p_cond(const RunArgs<T> *R, CondList C, int buf, int last, int grid, hipStream_t s);
+// control-block step of every run: arg = (table capacity << 8) | operand
+template <class T> hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg, hipStream_t s);
+enum : int { CTRL_BEGIN_STEP = 0, CTRL_BEGIN_STAGE, CTRL_END_STAGE, CTRL_END_STEP };
+
+// position of `it` in an exclusive prefix pre[0..n] (largest p with pre[p] <= it)
+__device__ __forceinline__ int locate(const int *pre, int n, int it) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 }  // namespace chq

@@ -23,21 +23,9 @@ __device__ __forceinline__ bool gate(const Ctrl *c, int only_if_bad) {
   return true;
 }
 
-// Blocks iterate over "virtual blocks" (vb -> bx, by, bz of a gx x gy x gz tile grid): a
-// normal launch has one block per tile; the launches of the fofc branch, which usually
-// exit at once, use a small grid-stride grid so an empty branch costs a few microseconds.
-// A virtual block is a chunk of 256 consecutive points of one (j, i) plane of an nx x ny
-// box (rows are linearised, so rows of 257 or 262 points waste no lanes), times nz planes.
-struct VB {
-  int nx, ny, cpp, n;   // plane size, chunks per plane, total chunks
-  __device__ __forceinline__ bool at(int vb, int &k, int &j, int &i) const {
-    k = vb / cpp;
-    const int t = (vb - k * cpp) * 256 + int(threadIdx.x);
-    j = t / nx;
-    i = t - j * nx;
-    return t < nx * ny;
-  }
-};
+// Blocks iterate over virtual blocks (VB, kernels.hpp): a normal launch has one block per
+// tile; the launches of the fofc branch, which usually exit at once, use a small
+// grid-stride grid so an empty branch costs a few microseconds.
 constexpr int COND_BLOCKS = 16384;   // grid of the conditional (fofc-branch) launches
 
 // ---------------------------------------------------------------- corner EMF
@@ -121,26 +109,30 @@ __global__ void __launch_bounds__(256) k_corner(const CornerArgsT<T> a, const VB
 // whole-domain recompute would produce from unchanged inputs. Overlapping cells write the
 // same value to a shared edge.
 template <class T>
+__device__ __forceinline__ void corner_list_entry(const CornerArgsT<T> &a, const int *list, int t) {
+  const Geom &g = a.g;
+  const long long c0 = list[t >> 3];
+  const int o = t & 7;
+  const long long r = c0 - g.ioff;
+  const int k = int(r / g.plane), j = int((r % g.plane) / g.pitch), i = int(r % g.pitch);
+  const int kk = k + ((o >> 2) & 1), jj = j + ((o >> 1) & 1), ii = i + (o & 1);
+  if (g.three_d) {
+    // the edge kernel's union box and per-component ranges (physics.hpp:266, 277, 288)
+    if (kk > g.ke + 1 || jj > g.je + 1 || ii > g.ie + 1) return;
+    corner3_at(a, kk, jj, ii);
+  } else {
+    if (kk != g.ks || jj > g.je + 1 || ii > g.ie + 1) return;
+    if (!g.two_d && jj != g.js) return;
+    corner12_at(a, jj, ii);
+  }
+}
+
+template <class T>
 __global__ void __launch_bounds__(256) k_corner_list(const CornerArgsT<T> a, const int *list) {
   if (!gate(a.ctrl, 1)) return;
-  const Geom &g = a.g;
   const int n = a.ctrl->nbad * 8;
-  for (int t = int(blockIdx.x) * 256 + int(threadIdx.x); t < n; t += int(gridDim.x) * 256) {
-    const long long c0 = list[t >> 3];
-    const int o = t & 7;
-    const long long r = c0 - g.ioff;
-    const int k = int(r / g.plane), j = int((r % g.plane) / g.pitch), i = int(r % g.pitch);
-    const int kk = k + ((o >> 2) & 1), jj = j + ((o >> 1) & 1), ii = i + (o & 1);
-    if (g.three_d) {
-      // the edge kernel's union box and per-component ranges (physics.hpp:266, 277, 288)
-      if (kk > g.ke + 1 || jj > g.je + 1 || ii > g.ie + 1) continue;
-      corner3_at(a, kk, jj, ii);
-    } else {
-      if (kk != g.ks || jj > g.je + 1 || ii > g.ie + 1) continue;
-      if (!g.two_d && jj != g.js) continue;
-      corner12_at(a, jj, ii);
-    }
-  }
+  for (int t = int(blockIdx.x) * 256 + int(threadIdx.x); t < n; t += int(gridDim.x) * 256)
+    corner_list_entry(a, list, t);
 }
 
 // ---------------------------------------------------------------- update
@@ -198,11 +190,9 @@ __global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB
 
 // ---------------------------------------------------------------- boundaries
 // physics.hpp:354-432. x1: threads over (k, j, g) with g fastest; x2: (k, g, i); x3: (g, j, i).
-__global__ void __launch_bounds__(256) k_bc1(const BcArgs a) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
+__device__ __forceinline__ void bc1_at(const BcArgs &a, const long long t) {
   const Geom &g = a.g;
   const int ng = g.ng;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n_all = (long long)(g.nk + 1) * (g.nj + 1) * ng;
   if (t >= n_all) return;
   const int gg = int(t % ng), j = int((t / ng) % (g.nj + 1)), k = int(t / ((long long)ng * (g.nj + 1)));
@@ -233,11 +223,9 @@ __global__ void __launch_bounds__(256) k_bc1(const BcArgs a) {
   if (bc == 0 && gg == 0) b1[ie + 1] = b1[is];
 }
 
-__global__ void __launch_bounds__(256) k_bc2(const BcArgs a) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
+__device__ __forceinline__ void bc2_at(const BcArgs &a, const long long t) {
   const Geom &g = a.g;
   const int ng = g.ng;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n_all = (long long)(g.nk + 1) * ng * (g.ni + 1);
   if (t >= n_all) return;
   const int i = int(t % (g.ni + 1)), gg = int((t / (g.ni + 1)) % ng), k = int(t / ((long long)ng * (g.ni + 1)));
@@ -271,11 +259,9 @@ __global__ void __launch_bounds__(256) k_bc2(const BcArgs a) {
   if (bc == 0 && gg == 0) b2[(je + 1) * sj] = b2[js * sj];
 }
 
-__global__ void __launch_bounds__(256) k_bc3(const BcArgs a) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
+__device__ __forceinline__ void bc3_at(const BcArgs &a, const long long t) {
   const Geom &g = a.g;
   const int ng = g.ng;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n_all = (long long)ng * (g.nj + 1) * (g.ni + 1);
   if (t >= n_all) return;
   const int i = int(t % (g.ni + 1)), j = int((t / (g.ni + 1)) % (g.nj + 1)), gg = int(t / ((long long)(g.nj + 1) * (g.ni + 1)));
@@ -305,6 +291,19 @@ __global__ void __launch_bounds__(256) k_bc3(const BcArgs a) {
   else if (bc == 1) { b3[fl * sk] = b3[ks * sk]; b3[fr * sk] = b3[(ke + 1) * sk]; }
   else { b3[fl * sk] = -b3[(2 * ks - fl) * sk]; b3[fr * sk] = -b3[(2 * (ke + 1) - fr) * sk]; }
   if (bc == 0 && gg == 0) b3[(ke + 1) * sk] = b3[ks * sk];
+}
+
+template <int DIR>
+__device__ __forceinline__ void bc_at(const BcArgs &a, const long long t) {
+  if (DIR == 0) bc1_at(a, t);
+  else if (DIR == 1) bc2_at(a, t);
+  else bc3_at(a, t);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_bc(const BcArgs a) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  bc_at<DIR>(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- reductions
@@ -398,22 +397,10 @@ __device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i
   }
 }
 
-// cons -> prim over all cells, physics.hpp:437-494. The CFL minimum is reduced with wave64
-// shuffles and one LDS pass per block, then merged with a 64-bit atomicMin on the bit
-// pattern, issued only when the block's minimum beats the current one.
-template <class T, bool DT, bool ENT>
-__global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
-  if (!gate(a.ctrl, a.only_if_bad)) return;
-  const Geom &g = a.g;
-  double mdt = 1e300;
-  int nbad_local = 0;
-  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
-    int k, j, i;
-    if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT>(a, k, j, i, mdt, nbad_local);
-  }
-  if (!DT && a.flag_mode != 1) return;
-  __shared__ double smin[BX * BY / 64];
-  __shared__ int ssum[BX * BY / 64];
+// Block merge of the CFL minimum and the flag count into the control block (the block's
+// minimum goes out only when it beats the current one). Every thread of the block calls it.
+template <bool DT, class T>
+__device__ __forceinline__ void c2p_block_merge(const C2pArgsT<T> &a, double mdt, int nbad_local, double *smin, int *ssum) {
   const int lane = int(threadIdx.x) & 63, wid = int(threadIdx.x) >> 6;
   const double wm = wave_min(mdt);
   const int ws = wave_sum(nbad_local);
@@ -429,6 +416,25 @@ __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
     }
     if (a.flag_mode == 1 && s) atomicAdd(&a.ctrl->nbad, s);
   }
+  __syncthreads();   // smin / ssum are reused by the next tile of a grid-stride loop
+}
+
+// cons -> prim over all cells, physics.hpp:437-494. The CFL minimum is reduced with wave64
+// shuffles and one LDS pass per block, then merged with a 64-bit atomicMin on the bit
+// pattern, issued only when the block's minimum beats the current one.
+template <class T, bool DT, bool ENT>
+__global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  double mdt = 1e300;
+  int nbad_local = 0;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+    int k, j, i;
+    if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT>(a, k, j, i, mdt, nbad_local);
+  }
+  if (!DT && a.flag_mode != 1) return;
+  __shared__ double smin[BX * BY / 64];
+  __shared__ int ssum[BX * BY / 64];
+  c2p_block_merge<DT>(a, mdt, nbad_local, smin, ssum);
 }
 
 // standalone FlagBadCells over the active cells (fofc.hpp:17-44); count into ctrl->nbad
@@ -467,50 +473,260 @@ __device__ __forceinline__ unsigned long long bits_of(double x) { return (unsign
 __device__ __forceinline__ double from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
 
 // main.cpp:246-248
-__global__ void k_begin_step(Ctrl *c) {
+__device__ __forceinline__ void ctrl_begin_step(Ctrl *c) {
   if (!(c->t < c->tlim && (c->nlim < 0 || c->cycle < c->nlim))) { c->active = 0; return; }
   c->active = 1;
   double dt = c->cfl * c->min_dt;
   if (c->t + dt > c->tlim) dt = c->tlim - c->t;
   c->dt = dt;
 }
-__global__ void k_begin_stage(Ctrl *c, int reset_slots) {
+__device__ __forceinline__ void ctrl_begin_stage(Ctrl *c, int reset_slots) {
   c->nbad = 0;
   if (reset_slots) { c->mdt_bits[0] = bits_of(1e300); c->mdt_bits[1] = bits_of(1e300); }
 }
 // the CFL minimum of the final (possibly fofc-repaired) con2prim of the last stage
-// (fused order: slot 1 holds the minimum of the repaired state when cells were flagged)
-__global__ void k_end_stage(Ctrl *c, int last, int fused) {
+// (fused order: slot 1 holds the minimum of the repaired state when cells were flagged).
+// The fused order alternates the primitive sets every stage that ran.
+__device__ __forceinline__ void ctrl_end_stage(Ctrl *c, int last, int fused) {
   if (!c->active) return;
+  if (fused) c->wcur ^= 1;
   c->fofc_total += c->nbad;
   c->nbad_last = c->nbad;
   if (last) c->min_dt = fmin(from_bits(c->mdt_bits[(fused && c->nbad) ? 1 : 0]), 1e300);
 }
 // main.cpp:291-292
-__global__ void k_end_step(Ctrl *c) {
+__device__ __forceinline__ void ctrl_end_step(Ctrl *c) {
   if (!c->active) return;
   c->t += c->dt;
   c->cycle += 1;
+}
+__global__ void k_begin_step(Ctrl *c) { ctrl_begin_step(c); }
+__global__ void k_begin_stage(Ctrl *c, int reset_slots) { ctrl_begin_stage(c, reset_slots); }
+__global__ void k_end_stage(Ctrl *c, int last, int fused) { ctrl_end_stage(c, last, fused); }
+__global__ void k_end_step(Ctrl *c) { ctrl_end_step(c); }
+
+// ---------------------------------------------------------------- batched kernels
+// Grid-stride walks over device-resident (run, tile) lists; each item runs the same tile body
+// as the single-mesh kernels, with that run's launch description (RunArgs) and control block.
+// The primitive set a run reads is its Ctrl::wcur; con2prim writes the other one.
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_corner(const RunArgs<T> *R, const ItemList L) {
+  const int n = *L.n;
+  for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
+    const Item m = L.item[it];
+    const Ctrl *ctl = R[m.run].ctrl;
+    if (!ctl->active) continue;
+    const CornerArgsT<T> a = R[m.run].corner[ctl->wcur];
+    const Geom &g = a.g;
+    int k, j, i;
+    if (!corner_vb(g).at(m.tile, k, j, i)) continue;
+    if (g.three_d) corner3_at(a, g.ks + k, g.js + j, g.is + i);
+    else corner12_at(a, g.js + j, g.is + i);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void update_tile(const RunArgs<T> &run, const Ctrl *ctl, int tile, int out, double g0,
+                                            double g1, double beta) {
+  UpdateArgsT<T> a = run.update[out];
+  a.g0 = g0;
+  a.g1 = g1;
+  const double bdt = beta * ctl->dt;
+  const Geom &g = a.g;
+  int k, j, i;
+  if (update_vb(g).at(tile, k, j, i)) update_at(a, bdt, g.ks + k, g.js + j, g.is + i);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_update(const RunArgs<T> *R, const ItemList L, int out, double g0, double g1,
+                                                 double beta) {
+  const int n = *L.n;
+  for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
+    const Item m = L.item[it];
+    const Ctrl *ctl = R[m.run].ctrl;
+    if (!ctl->active) continue;
+    update_tile(R[m.run], ctl, m.tile, out, g0, g1, beta);
+  }
+}
+
+template <class T, int DIR>
+__global__ void __launch_bounds__(256) kb_bc(const RunArgs<T> *R, const ItemList L, int buf) {
+  const int n = *L.n;
+  for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
+    const Item m = L.item[it];
+    if (!R[m.run].ctrl->active) continue;
+    const BcArgs a = R[m.run].bc[buf][DIR];
+    bc_at<DIR>(a, (long long)m.tile * 256 + threadIdx.x);
+  }
+}
+
+// one con2prim tile of a run (the block's CFL minimum and flags merge into that run's
+// control block); `cond` selects the fofc re-run (no flags, second minimum slot)
+template <class T, bool LAST>
+__device__ __forceinline__ void c2p_tile(const RunArgs<T> &run, const Ctrl *ctl, int tile, int buf, bool cond,
+                                         double *smin, int *ssum) {
+  C2pArgsT<T> a = run.c2p[buf][ctl->wcur ^ 1];
+  a.compute_dt = LAST;
+  a.entropy_out = LAST;
+  a.slot = cond ? 1 : 0;
+  if (cond) a.flag_mode = 0;
+  double mdt = 1e300;
+  int nb = 0;
+  int k, j, i;
+  if (c2p_vb(a.g).at(tile, k, j, i)) c2p_at<T, LAST, LAST>(a, k, j, i, mdt, nb);
+  if (LAST) c2p_block_merge<LAST>(a, mdt, nb, smin, ssum);
+}
+
+template <class T, bool LAST>
+__global__ void __launch_bounds__(256) kb_c2p(const RunArgs<T> *R, const ItemList L, int buf) {
+  __shared__ double smin[BX * BY / 64];
+  __shared__ int ssum[BX * BY / 64];
+  const int n = *L.n;
+  for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
+    const Item m = L.item[it];
+    const Ctrl *ctl = R[m.run].ctrl;
+    if (!ctl->active) continue;
+    c2p_tile<T, LAST>(R[m.run], ctl, m.tile, buf, false, smin, ssum);
+  }
+}
+
+// Compaction of the runs that flagged cells this stage (one block): run list and, per class,
+// the exclusive prefix of their tiles.
+template <class T>
+__global__ void __launch_bounds__(1024) kb_fofc_scan(const RunArgs<T> *R, const int *nruns, const CondList C) {
+  constexpr int NC = CL_N + 1;   // flagged-run count + the classes
+  __shared__ int s_tot[NC];
+  __shared__ int s_wave[16][NC];
+  const int tid = int(threadIdx.x), lane = tid & 63, wid = tid >> 6;
+  if (tid < NC) s_tot[tid] = 0;
+  __syncthreads();
+  const int nr = *nruns;
+  for (int base = 0; base < nr; base += 1024) {
+    const int r = base + tid;
+    int cnt[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) cnt[q] = 0;
+    if (r < nr) {
+      const Ctrl *c = R[r].ctrl;
+      if (c->active && c->nbad > 0) {
+        const int nb = c->nbad;
+        cnt[0] = 1;
+        cnt[1 + CL_REPAIR] = div_up(nb * 2 * R[r].ndir, 256);
+        cnt[1 + CL_CORNER] = div_up(nb * 8, 256);
+        cnt[1 + CL_UPDATE] = R[r].cond_tiles[0];
+        cnt[1 + CL_BC1] = R[r].cond_tiles[1];
+        cnt[1 + CL_BC2] = R[r].cond_tiles[2];
+        cnt[1 + CL_BC3] = R[r].cond_tiles[3];
+        cnt[1 + CL_C2P] = R[r].cond_tiles[4];
+      }
+    }
+    // inclusive wave scans, then the wave offsets
+    int inc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      int v = cnt[q];
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+      }
+      inc[q] = v;
+      if (lane == 63) s_wave[wid][q] = v;
+    }
+    __syncthreads();
+    if (cnt[0]) {
+      int off[NC];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        int o = s_tot[q];
+        for (int w = 0; w < wid; ++w) o += s_wave[w][q];
+        off[q] = o + inc[q] - cnt[q];
+      }
+      const int pos = off[0];
+      C.run()[pos] = r;
+#pragma unroll
+      for (int q = 0; q < CL_N; ++q) C.pre(q)[pos] = off[1 + q];
+    }
+    __syncthreads();
+    if (tid < NC) {
+      int add = 0;
+      for (int w = 0; w < 16; ++w) add += s_wave[w][tid];
+      s_tot[tid] += add;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int m = s_tot[0];
+    *C.nrun() = m;
+    for (int q = 0; q < CL_N; ++q) C.pre(q)[m] = s_tot[1 + q];
+  }
+}
+
+// walk of one class of the compacted fofc list: fn(run, tile)
+template <class F>
+__device__ __forceinline__ void cond_walk(const CondList &C, int cls, F fn) {
+  const int nrun = *C.nrun();
+  if (nrun == 0) return;
+  const int *pre = C.pre(cls);
+  const int total = pre[nrun];
+  for (int it = int(blockIdx.x); it < total; it += int(gridDim.x)) {
+    const int p = locate(pre, nrun, it);
+    fn(C.run()[p], it - pre[p]);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_corner_cond(const RunArgs<T> *R, const CondList C) {
+  cond_walk(C, CL_CORNER, [&](int r, int tile) {
+    const Ctrl *ctl = R[r].ctrl;
+    const CornerArgsT<T> a = R[r].corner[ctl->wcur];
+    const int t = tile * 256 + int(threadIdx.x);
+    if (t < ctl->nbad * 8) corner_list_entry(a, R[r].badlist, t);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_update_cond(const RunArgs<T> *R, const CondList C, int out, double g0,
+                                                      double g1, double beta) {
+  cond_walk(C, CL_UPDATE, [&](int r, int tile) { update_tile(R[r], R[r].ctrl, tile, out, g0, g1, beta); });
+}
+
+template <class T, int DIR>
+__global__ void __launch_bounds__(256) kb_bc_cond(const RunArgs<T> *R, const CondList C, int buf) {
+  cond_walk(C, CL_BC1 + DIR, [&](int r, int tile) {
+    const BcArgs a = R[r].bc[buf][DIR];
+    bc_at<DIR>(a, (long long)tile * 256 + threadIdx.x);
+  });
+}
+
+template <class T, bool LAST>
+__global__ void __launch_bounds__(256) kb_c2p_cond(const RunArgs<T> *R, const CondList C, int buf) {
+  __shared__ double smin[BX * BY / 64];
+  __shared__ int ssum[BX * BY / 64];
+  cond_walk(C, CL_C2P, [&](int r, int tile) { c2p_tile<T, LAST>(R[r], R[r].ctrl, tile, buf, true, smin, ssum); });
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg) {
+  const int r = int(blockIdx.x) * 256 + int(threadIdx.x);
+  if (r >= *nruns) return;
+  Ctrl *c = R[r].ctrl;
+  if (op == CTRL_BEGIN_STEP) ctrl_begin_step(c);
+  else if (op == CTRL_BEGIN_STAGE) ctrl_begin_stage(c, arg);
+  else if (op == CTRL_END_STAGE) ctrl_end_stage(c, arg, 1);
+  else ctrl_end_step(c);
 }
 
 }  // namespace
 
 namespace {
-VB vgrid(int nx, int ny, int nz) {
-  VB v;
-  v.nx = nx;
-  v.ny = ny;
-  v.cpp = div_up(nx * ny, 256);
-  v.n = v.cpp * nz;
-  return v;
-}
 int grid_of(const VB &v, bool conditional) { return conditional ? (v.n < COND_BLOCKS ? v.n : COND_BLOCKS) : v.n; }
 }  // namespace
 
 template <class T>
 hipError_t launch_corner(const CornerArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
-  const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, g.three_d ? g.ke - g.ks + 2 : 1);
+  const VB v = corner_vb(g);
   hipLaunchKernelGGL(k_corner<T>, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
   return hipGetLastError();
 }
@@ -524,7 +740,7 @@ hipError_t launch_corner_list(const CornerArgsT<T> &a, const int *list, hipStrea
 template <class T>
 hipError_t launch_update(const UpdateArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
-  const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, g.ke - g.ks + 2);
+  const VB v = update_vb(g);
   hipLaunchKernelGGL(k_update<T>, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
   return hipGetLastError();
 }
@@ -537,22 +753,17 @@ template hipError_t launch_update<double>(const UpdateArgsT<double> &, hipStream
 template hipError_t launch_update<float>(const UpdateArgsT<float> &, hipStream_t);
 
 hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s) {
-  const Geom &g = a.g;
-  long long n = 0;
-  if (dir == 0) n = (long long)(g.nk + 1) * (g.nj + 1) * g.ng;
-  else if (dir == 1) n = (long long)(g.nk + 1) * g.ng * (g.ni + 1);
-  else n = (long long)g.ng * (g.nj + 1) * (g.ni + 1);
-  const int nb = int((n + 255) / 256);
-  if (dir == 0) hipLaunchKernelGGL(k_bc1, dim3(nb), dim3(256), 0, s, a);
-  else if (dir == 1) hipLaunchKernelGGL(k_bc2, dim3(nb), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_bc3, dim3(nb), dim3(256), 0, s, a);
+  const int nb = int((bc_points(a.g, dir) + 255) / 256);
+  if (dir == 0) hipLaunchKernelGGL(k_bc<0>, dim3(nb), dim3(256), 0, s, a);
+  else if (dir == 1) hipLaunchKernelGGL(k_bc<1>, dim3(nb), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_bc<2>, dim3(nb), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
-  const VB v = vgrid(g.ni, g.nj, g.nk);
+  const VB v = c2p_vb(g);
   const dim3 grid(grid_of(v, a.only_if_bad)), block(256);
   const bool ent = a.entropy_out || a.eos.dual_energy;
   if (a.compute_dt) {
@@ -591,5 +802,84 @@ hipError_t launch_ctrl_end_step(Ctrl *c, hipStream_t s) {
   hipLaunchKernelGGL(k_end_step, dim3(1), dim3(1), 0, s, c);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- batched launches
+template <class T>
+hipError_t launchb_corner(const RunArgs<T> *R, ItemList L, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(kb_corner<T>, dim3(grid), dim3(256), 0, s, R, L);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_update(const RunArgs<T> *R, ItemList L, int out, double g0, double g1, double beta, int grid,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(kb_update<T>, dim3(grid), dim3(256), 0, s, R, L, out, g0, g1, beta);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_bc(const RunArgs<T> *R, ItemList L, int dir, int buf, int grid, hipStream_t s) {
+  if (dir == 0) hipLaunchKernelGGL((kb_bc<T, 0>), dim3(grid), dim3(256), 0, s, R, L, buf);
+  else if (dir == 1) hipLaunchKernelGGL((kb_bc<T, 1>), dim3(grid), dim3(256), 0, s, R, L, buf);
+  else hipLaunchKernelGGL((kb_bc<T, 2>), dim3(grid), dim3(256), 0, s, R, L, buf);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_c2p(const RunArgs<T> *R, ItemList L, int buf, int last, int grid, hipStream_t s) {
+  if (last) hipLaunchKernelGGL((kb_c2p<T, true>), dim3(grid), dim3(256), 0, s, R, L, buf);
+  else hipLaunchKernelGGL((kb_c2p<T, false>), dim3(grid), dim3(256), 0, s, R, L, buf);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_fofc_scan(const RunArgs<T> *R, const int *nruns, CondList C, hipStream_t s) {
+  hipLaunchKernelGGL(kb_fofc_scan<T>, dim3(1), dim3(1024), 0, s, R, nruns, C);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_corner_cond(const RunArgs<T> *R, CondList C, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(kb_corner_cond<T>, dim3(grid), dim3(256), 0, s, R, C);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_update_cond(const RunArgs<T> *R, CondList C, int out, double g0, double g1, double beta, int grid,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(kb_update_cond<T>, dim3(grid), dim3(256), 0, s, R, C, out, g0, g1, beta);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_bc_cond(const RunArgs<T> *R, CondList C, int dir, int buf, int grid, hipStream_t s) {
+  if (dir == 0) hipLaunchKernelGGL((kb_bc_cond<T, 0>), dim3(grid), dim3(256), 0, s, R, C, buf);
+  else if (dir == 1) hipLaunchKernelGGL((kb_bc_cond<T, 1>), dim3(grid), dim3(256), 0, s, R, C, buf);
+  else hipLaunchKernelGGL((kb_bc_cond<T, 2>), dim3(grid), dim3(256), 0, s, R, C, buf);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_c2p_cond(const RunArgs<T> *R, CondList C, int buf, int last, int grid, hipStream_t s) {
+  if (last) hipLaunchKernelGGL((kb_c2p_cond<T, true>), dim3(grid), dim3(256), 0, s, R, C, buf);
+  else hipLaunchKernelGGL((kb_c2p_cond<T, false>), dim3(grid), dim3(256), 0, s, R, C, buf);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg, hipStream_t s) {
+  // one thread per run slot of the table (capacity fixed by the caller through `arg` >> 8)
+  const int cap = arg >> 8;
+  hipLaunchKernelGGL(kb_ctrl<T>, dim3(div_up(cap > 0 ? cap : 1, 256)), dim3(256), 0, s, R, nruns, op, arg & 0xff);
+  return hipGetLastError();
+}
+
+#define CHQ_BATCH_INST(T)                                                                                     \
+  template hipError_t launchb_corner<T>(const RunArgs<T> *, ItemList, int, hipStream_t);                     \
+  template hipError_t launchb_update<T>(const RunArgs<T> *, ItemList, int, double, double, double, int,      \
+                                        hipStream_t);                                                         \
+  template hipError_t launchb_bc<T>(const RunArgs<T> *, ItemList, int, int, int, hipStream_t);               \
+  template hipError_t launchb_c2p<T>(const RunArgs<T> *, ItemList, int, int, int, hipStream_t);              \
+  template hipError_t launchb_fofc_scan<T>(const RunArgs<T> *, const int *, CondList, hipStream_t);          \
+  template hipError_t launchb_corner_cond<T>(const RunArgs<T> *, CondList, int, hipStream_t);                \
+  template hipError_t launchb_update_cond<T>(const RunArgs<T> *, CondList, int, double, double, double, int, \
+                                             hipStream_t);                                                    \
+  template hipError_t launchb_bc_cond<T>(const RunArgs<T> *, CondList, int, int, int, hipStream_t);          \
+  template hipError_t launchb_c2p_cond<T>(const RunArgs<T> *, CondList, int, int, int, hipStream_t);         \
+  template hipError_t launchb_ctrl<T>(const RunArgs<T> *, const int *, int, int, hipStream_t);
+CHQ_BATCH_INST(double)
+CHQ_BATCH_INST(float)
+#undef CHQ_BATCH_INST
 
 }  // namespace chq

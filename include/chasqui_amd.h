@@ -160,6 +160,39 @@ int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, dou
  * emag eint divB_max divB_L1 */
 int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]);
 
+/* ---- batches: many independent runs advanced by shared launches (no counterpart in the
+ * reference, which advances one mesh with a host-driven loop, main.cpp:246-297) ----
+ * Every run is a context created, filled and initialised (chq_init) as for chq_step; the batch
+ * then advances all of them together: one launch per kernel class and RK stage walks a
+ * device-resident list of (run, tile) items, each run with its own dt, t, cycle, tlim, CFL,
+ * boundary conditions and size (1D, 2D and 3D runs mix). A batch holds runs of one scheme
+ * (reconstruction, Riemann solver, integrator, precision; dual energy off), fixed by its
+ * first run. Results are bitwise those of chq_step on the same context. The RK step is
+ * captured in a hipGraph and replayed; runs joining or leaving only rewrite device lists.
+ * While in a batch a context is not stepped by chq_step; its arrays and status may be read
+ * (reads wait for the batch's stream). */
+typedef struct chq_batch chq_batch;
+int chq_batch_create(int capacity, chq_batch **out);
+int chq_batch_destroy(chq_batch *batch);
+int chq_batch_add(chq_batch *batch, chq_ctx *ctx);
+int chq_batch_remove(chq_batch *batch, chq_ctx *ctx);   /* the last run moves into its slot */
+int chq_batch_size(const chq_batch *batch);
+chq_ctx *chq_batch_run(chq_batch *batch, int index);
+int chq_batch_step(chq_batch *batch, int nsteps);      /* asynchronous */
+int chq_batch_step_timed(chq_batch *batch, int nsteps, double *ms);
+int chq_batch_sync(chq_batch *batch);
+/* runs that have not reached tlim / nlim; flags[i] (optional, size() entries) = 1 if run i
+ * has not */
+int chq_batch_unfinished(chq_batch *batch, int *count, int *flags);
+int chq_batch_use_graph(chq_batch *batch, int on);     /* default on */
+/* Autotuner: times every kernel class over candidate grid sizes on the current runs (stage-1
+ * configuration; touches only arrays a step rewrites) and keeps the fastest; best_ms = sum of
+ * the best per-class launch times. Call between steps. */
+int chq_batch_tune(chq_batch *batch, int reps, double *best_ms);
+/* grid size (blocks) of kernel class `cls` (0..10) and its name */
+int chq_batch_grid(chq_batch *batch, int cls, int *grid, const char **name);
+int chq_batch_set_grid(chq_batch *batch, int cls, int grid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,160 @@
+"""GPU: batches of uneven runs against the same runs stepped alone.
+
+A batch advances every run with shared launches (device work lists, one launch per kernel
+class and stage, hipGraph replay); no reduction spans runs, so each run must be bitwise what
+`Mesh.step` gives it alone (and the solo path is pinned bitwise to the oracle elsewhere).
+Runs differ in dimensionality, size, boundaries, CFL, tlim/nlim (so they finish at different
+steps), and one of them has fofc firing every stage.
+"""
+import numpy as np
+import pytest
+
+from helpers import deck
+
+pytestmark = pytest.mark.gpu
+
+SCHEME = ["mhd.reconstruct=ppm", "mhd.rsolver=hlld", "time.integrator=rk3"]
+
+RUNS = [
+    ("blast_3d.in", ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=16"]),
+    ("blast_3d.in", ["mesh.nx1=20", "mesh.nx2=12", "mesh.nx3=8", "mesh.bc_x1=reflect", "mesh.bc_x2=outflow",
+                     "time.nlim=5"]),
+    ("blast_3d.in", ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12", "mhd.pfloor=0.1"]),          # fofc fires
+    ("orszag_tang.in", ["mesh.nx1=40", "mesh.nx2=24", "time.cfl=0.3"]),
+    ("orszag_tang.in", ["mesh.nx1=32", "mesh.nx2=32", "time.tlim=0.01"]),                     # finishes early
+    ("brio_wu.in", ["mesh.nx1=300"]),
+    ("cpaw_1d.in", ["mesh.nx1=64", "time.tlim=0.05"]),
+]
+
+
+def _mesh(name, ov, prec):
+    from chasqui_amd.deck import Deck
+    d = Deck(deck(name), SCHEME + ov + [f"mhd.precision={prec}"])
+    m = d.make_mesh()
+    d.close()
+    return m
+
+
+def _state(m):
+    out = {n: m.download(n) for n in ("u0", "b1f0", "b2f0", "b3f0", "w", "bcc", "ecc")}
+    st = m.status()
+    out["status"] = (st["t"], st["cycle"], st["min_dt"], st["fofc_total"], st["dt"])
+    return out
+
+
+def _same(a, b, label):
+    for k in a:
+        if k == "status":
+            assert a[k] == b[k], f"{label}: status {a[k]} != {b[k]}"
+        elif k == "u0":   # the entropy slot is rebuilt by pow() from rho and p in both
+            assert np.array_equal(a[k][:5], b[k][:5]), f"{label}: {k}"
+            assert np.array_equal(a[k][5], b[k][5]), f"{label}: u0[IENT]"
+        else:
+            assert np.array_equal(a[k], b[k]), f"{label}: {k}"
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_solo(prec, graph):
+    from chasqui_amd import Batch
+    steps = 12
+    print(f"case batch prec={prec} graph={graph}", flush=True)
+    solo = []
+    for name, ov in RUNS:
+        m = _mesh(name, ov, prec)
+        m.step(steps)
+        solo.append(_state(m))
+        m.close()
+    b = Batch(16)
+    b.use_graph(graph)
+    meshes = [_mesh(name, ov, prec) for name, ov in RUNS]
+    for m in meshes:
+        b.add(m)
+    b.step(steps)
+    b.sync()
+    for i, m in enumerate(meshes):
+        _same(_state(m), solo[i], f"run {i} {RUNS[i]}")
+    assert solo[2]["status"][3] > 0, "the fofc case must flag cells"
+    cnt, flags = b.unfinished()
+    assert flags[4] == 0 and flags[6] == 0 and flags[1] == 0
+    b.close()
+    for m in meshes:
+        m.close()
+
+
+def test_batch_membership_changes():
+    """Runs join and leave between replays of the captured step; the others are unaffected."""
+    from chasqui_amd import Batch
+    print("case membership", flush=True)
+    A, B, Cc = RUNS[0], RUNS[3], RUNS[5]
+    ref = {}
+    for key, (name, ov), n in (("a", A, 9), ("b", B, 5), ("c", Cc, 6)):
+        m = _mesh(name, ov, "fp64")
+        m.step(n)
+        ref[key] = _state(m)
+        m.close()
+    b = Batch(4)
+    ma, mb, mc = _mesh(*A, "fp64"), _mesh(*B, "fp64"), _mesh(*Cc, "fp64")
+    b.add(ma)
+    b.add(mb)
+    b.step(3)
+    b.add(mc)        # joins after 3 steps
+    b.step(2)
+    b.remove(mb)     # leaves after 5 steps
+    b.step(4)        # a: 9 steps, c: 6 steps
+    b.sync()
+    _same(_state(ma), ref["a"], "a")
+    _same(_state(mb), ref["b"], "b")
+    _same(_state(mc), ref["c"], "c")
+    # a context that left a batch steps on its own again
+    mb.step(1)
+    b.close()
+    for m in (ma, mb, mc):
+        m.close()
+
+
+def test_batch_autotune_keeps_results():
+    """The autotuner times kernel classes on scratch arrays only: tuning between steps must not
+    change any run's trajectory."""
+    from chasqui_amd import Batch
+    print("case autotune", flush=True)
+    runs = RUNS[:4]
+    solo = []
+    for name, ov in runs:
+        m = _mesh(name, ov, "fp64")
+        m.step(6)
+        solo.append(_state(m))
+        m.close()
+    b = Batch(8)
+    meshes = [_mesh(name, ov, "fp64") for name, ov in runs]
+    for m in meshes:
+        b.add(m)
+    b.step(2)
+    best = b.tune(reps=2)
+    assert best > 0
+    grids = b.grids()
+    print("tuned grids", grids, flush=True)
+    b.step(4)
+    b.sync()
+    for i, m in enumerate(meshes):
+        _same(_state(m), solo[i], f"run {i}")
+    b.close()
+    for m in meshes:
+        m.close()
+
+
+def test_batch_rejects_mixed_schemes():
+    from chasqui_amd import Batch
+    b = Batch(2)
+    m1 = _mesh("cpaw_1d.in", ["mesh.nx1=32"], "fp64")
+    from chasqui_amd.deck import Deck
+    d = Deck(deck("cpaw_1d.in"), ["mesh.nx1=32", "mhd.reconstruct=wenoz"])
+    m2 = d.make_mesh()
+    b.add(m1)
+    with pytest.raises(RuntimeError, match="one scheme"):
+        b.add(m2)
+    with pytest.raises(RuntimeError, match="advanced by a batch"):
+        m1.step(1)
+    b.close()
+    m1.close()
+    m2.close()
