@@ -118,6 +118,166 @@ def run(n, precision, steps, warmup, breakdown=False):
     return ms, flux_ms, flux_launches, st, kernels
 
 
+OT_DECK = """
+[mesh]
+nx1 = 64
+nx2 = 64
+x1min = 0.0
+x1max = 1.0
+x2min = 0.0
+x2max = 1.0
+[time]
+integrator = rk3
+cfl = 0.4
+tlim = 0.5
+ndiag = 0
+[mhd]
+gamma = 1.6666666666666667
+reconstruct = ppmx
+rsolver = hlld
+[problem]
+name = orszag_tang
+"""
+
+BW_DECK = """
+[mesh]
+nx1 = 400
+x1min = -0.5
+x1max = 0.5
+bc_x1 = outflow
+[time]
+integrator = rk3
+cfl = 0.4
+tlim = 0.1
+ndiag = 0
+[mhd]
+gamma = 2.0
+reconstruct = ppm
+rsolver = hlld
+[problem]
+name = shock_tube
+"""
+
+
+def ensemble_overrides(nruns, seed):
+    """Batched Orszag-Tang ensemble: sizes drawn uniformly from {64 ... 512}^2, per-run seeded
+    +-5% perturbation of the pressure and the field amplitude."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [64, 96, 128, 192, 256, 384, 512]
+    p0, b0 = 5.0 / (12.0 * np.pi), 1.0 / np.sqrt(4.0 * np.pi)
+    out = []
+    for _ in range(nruns):
+        n = int(rng.choice(sizes))
+        fp, fb = 1.0 + 0.05 * rng.uniform(-1, 1), 1.0 + 0.05 * rng.uniform(-1, 1)
+        out.append([f"mesh.nx1={n}", f"mesh.nx2={n}", f"problem.pres={float(p0 * fp)!r}",
+                    f"problem.b0={float(b0 * fb)!r}"])
+    return out
+
+
+def ensemble(nruns, steps, precision, seed=7, tune=True):
+    """Config: `nruns` independent 2D Orszag-Tang runs (PPMX + HLLD + RK3) advanced as one batch
+    (one launch per kernel class and stage, hipGraph replay), against the same runs stepped one
+    after another on their own. Returns zone-updates/s of both and the speedup."""
+    from chasqui_amd import Batch
+    from chasqui_amd.deck import Deck
+    ovs = ensemble_overrides(nruns, seed)
+    cells = 0
+    meshes = []
+    for ov in ovs:
+        d = Deck(OT_DECK, ov + [f"mhd.precision={precision}"])
+        m = d.make_mesh()
+        d.close()
+        meshes.append(m)
+        cells += int(ov[0].split("=")[1]) ** 2
+    # sequential: each run alone, K steps (one warm-up step each, untimed)
+    seq_ms = 0.0
+    for m in meshes:
+        m.step(1)
+        m.sync()
+        seq_ms += m.step_timed(steps)
+    b = Batch(nruns)
+    for m in meshes:
+        b.add(m)
+    b.step(1)
+    b.sync()
+    tuned = None
+    if tune:
+        tuned = b.tune(reps=3)
+    b.step(1)
+    b.sync()
+    ms = b.step_timed(steps)
+    grids = b.grids()
+    b.close()
+    for m in meshes:
+        m.close()
+    zu = float(cells) * steps
+    return {"runs": nruns, "cells": cells, "steps": steps, "precision": precision,
+            "batched_zone_updates_per_s": zu / (ms * 1e-3), "batched_ms_per_step": ms / steps,
+            "sequential_zone_updates_per_s": zu / (seq_ms * 1e-3), "sequential_ms_per_step": seq_ms / steps,
+            "speedup_vs_sequential": seq_ms / ms, "tuned_sum_ms": tuned, "grids": grids}
+
+
+def serve_jobs(njobs, seed=11):
+    """Served stream: seeded mix of 1D Brio-Wu shock tubes (400-4000 cells), 2D Orszag-Tang
+    (128^2-1024^2) and 3D blasts (64^3-192^3), reconstruction from {ppm, wenoz}. Each run is
+    capped at a step budget (1D 200, 2D 40, 3D 20 steps) so the stream finishes in minutes."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(njobs):
+        kind = rng.choice(["1d", "2d", "3d"])
+        rec = str(rng.choice(["ppm", "wenoz"]))
+        if kind == "1d":
+            n = int(rng.integers(400, 4001))
+            jobs.append((BW_DECK, [f"mesh.nx1={n}", f"mhd.reconstruct={rec}", "time.nlim=200"]))
+        elif kind == "2d":
+            n = int(rng.choice([128, 256, 384, 512, 768, 1024]))
+            jobs.append((OT_DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mhd.reconstruct={rec}", "time.nlim=40"]))
+        else:
+            n = int(rng.choice([64, 96, 128, 160, 192]))
+            jobs.append((DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}", f"mhd.reconstruct={rec}",
+                                "time.nlim=20", "time.tlim=1e9"]))
+    return jobs
+
+
+def serve(njobs, rate_factor=4.0, seed=11, budget_gb=200.0, slots=64, chunk=4):
+    """Sequential plain-FP64 runs (one after another, alone) against the continuous-batching
+    server fed by seeded Poisson arrivals at `rate_factor` x the sequential completion rate,
+    mixed precision with FP64 fallback. Returns runs/s of both and the server's p50/p99
+    time-to-solution."""
+    import numpy as np
+    from chasqui_amd.deck import Deck
+    from chasqui_amd.serve import Server
+    jobs = serve_jobs(njobs, seed)
+    t0 = time.perf_counter()
+    for text, ov in jobs:
+        d = Deck(text, ov + ["mhd.precision=fp64"])
+        m = d.make_mesh()
+        d.close()
+        m.run(chunk=chunk)
+        m.close()
+    seq_s = time.perf_counter() - t0
+    seq_rate = njobs / seq_s
+    rng = np.random.default_rng(seed + 1)
+    gaps = rng.exponential(1.0 / (rate_factor * seq_rate), size=njobs)
+    arrivals = [(float(t), text, ov, "mixed") for t, (text, ov) in zip(np.cumsum(gaps) - gaps[0], jobs)]
+    srv = Server(int(budget_gb * 2 ** 30), slots=slots, chunk=chunk)
+    t0 = time.perf_counter()
+    recs = srv.play(arrivals)
+    span = time.perf_counter() - t0
+    srv.close()
+    tts = np.array([r["tts"] for r in recs])
+    return {"jobs": njobs, "sequential_fp64_runs_per_s": seq_rate, "sequential_s": seq_s,
+            "served_runs_per_s": njobs / span, "served_makespan_s": span,
+            "speedup_vs_sequential_fp64": (njobs / span) / seq_rate,
+            "arrival_rate_per_s": rate_factor * seq_rate,
+            "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
+            "fell_back": int(sum(r["fell_back"] for r in recs)),
+            "failed": int(sum(r["state"] != "done" for r in recs)),
+            "budget_gb": budget_gb, "slots_per_batch": slots, "chunk_steps": chunk}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +287,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision side measurement")
+    ap.add_argument("--ensemble-runs", type=int, default=64,
+                    help="runs of the batched Orszag-Tang ensemble measured beside the headline (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=0, help="jobs of the served stream (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one mesh per device; "
@@ -176,6 +339,10 @@ def main():
                         "ms_per_step": mms / args.steps, "speedup_vs_fp64": mval / value,
                         "flux_avg_launch_ms": mflux / max(mlaunch, 1),
                         "fofc_corrections_in_run": mst["fofc_total"], "kernels": mk}
+    if args.ensemble_runs > 0:
+        out["batch"] = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
+    if args.serve_jobs > 0:
+        out["serve"] = serve(args.serve_jobs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     print(json.dumps(out))

@@ -118,6 +118,17 @@ def lib():
         "chq_batch_tune": (C.c_int, [vp, C.c_int, dp]),
         "chq_batch_grid": (C.c_int, [vp, C.c_int, ip, C.POINTER(C.c_char_p)]),
         "chq_batch_set_grid": (C.c_int, [vp, C.c_int, C.c_int]),
+        "chq_bytes_for": (C.c_size_t, [C.POINTER(MeshDesc), C.POINTER(Config)]),
+        "chq_server_create": (C.c_int, [C.c_size_t, C.c_int, C.c_int, C.POINTER(vp)]),
+        "chq_server_destroy": (C.c_int, [vp]),
+        "chq_server_submit": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int, ip]),
+        "chq_server_pump": (C.c_int, [vp, C.c_int, ip, ip]),
+        "chq_server_job": (C.c_int, [vp, C.c_int, dp]),
+        "chq_server_job_ctx": (vp, [vp, C.c_int]),
+        "chq_server_job_error": (C.c_char_p, [vp, C.c_int]),
+        "chq_server_release": (C.c_int, [vp, C.c_int]),
+        "chq_server_stats": (C.c_int, [vp, dp]),
+        "chq_server_clock": (C.c_double, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -140,6 +151,9 @@ def exported_symbols():
         "chq_batch_create", "chq_batch_destroy", "chq_batch_add", "chq_batch_remove", "chq_batch_size",
         "chq_batch_run", "chq_batch_step", "chq_batch_step_timed", "chq_batch_sync", "chq_batch_unfinished",
         "chq_batch_use_graph", "chq_batch_tune", "chq_batch_grid", "chq_batch_set_grid",
+        "chq_bytes_for", "chq_server_create", "chq_server_destroy", "chq_server_submit", "chq_server_pump",
+        "chq_server_job", "chq_server_job_ctx", "chq_server_job_error", "chq_server_release",
+        "chq_server_stats", "chq_server_clock",
     ]
 
 

@@ -28,7 +28,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 
 HIP_SOURCES = ["flux.hip", "flux32.hip", "stage.hip", "layout.hip"]
 EXTRA = {"flux32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
-HOST_SOURCES = ["capi.cpp", "batch.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]
+HOST_SOURCES = ["capi.cpp", "batch.cpp", "server.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]
 
 
 def _compile(src):

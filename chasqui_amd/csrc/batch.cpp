@@ -40,6 +40,7 @@ struct BatchBase {
   virtual int rebuild() = 0;
   virtual int enqueue_step() = 0;
   virtual int tune(int reps, double *best_ms) = 0;
+  virtual int live(std::vector<int> &out) = 0;   // per run: not finished
 };
 
 }  // namespace
@@ -57,6 +58,7 @@ struct chq_batch {
   bool use_graph = true;
   hipGraphExec_t graph = nullptr;
   bool graph_ok = false;
+  bool dirty = false;   // membership changed since the device tables were written
   std::unique_ptr<BatchBase> impl;
   chq_batch() {
     for (int q = 0; q < G_N; ++q) grid[q] = q < K_NLIST ? 4096 : 1024;
@@ -81,8 +83,10 @@ struct Batch final : BatchBase {
   Item *d_items[K_NLIST]{};
   size_t item_cap[K_NLIST]{};
   int *d_cond = nullptr;               // CondList storage
+  int *d_live = nullptr;               // per run: not finished (written at the end of every step)
   std::vector<RunArgs<T>> h_runs;
   std::vector<Item> h_items[K_NLIST];
+  std::vector<int> h_live;
   int h_count[1 + K_NLIST]{};
 
   explicit Batch(chq_batch *bb) : b(bb) {}
@@ -91,12 +95,14 @@ struct Batch final : BatchBase {
     if (d_count) (void)hipFree(d_count);
     for (auto *p : d_items) if (p) (void)hipFree(p);
     if (d_cond) (void)hipFree(d_cond);
+    if (d_live) (void)hipFree(d_live);
   }
 
   int alloc() {
     HIPCHK(hipMalloc(&d_runs, sizeof(RunArgs<T>) * size_t(b->capacity)));
     HIPCHK(hipMalloc(&d_count, sizeof(int) * (1 + K_NLIST)));
     HIPCHK(hipMalloc(&d_cond, sizeof(int) * CondList::ints(b->capacity)));
+    HIPCHK(hipMalloc(&d_live, sizeof(int) * size_t(b->capacity)));
     // the scan kernel writes every field the conditional kernels read; start from zero runs
     std::vector<int> z(CondList::ints(b->capacity), 0);
     HIPCHK(hipMemcpy(d_cond, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -210,7 +216,25 @@ struct Batch final : BatchBase {
                               b->stream));
     }
     if (n) HIPCHK(hipMemcpyAsync(d_runs, h_runs.data(), sizeof(RunArgs<T>) * size_t(n), hipMemcpyHostToDevice, b->stream));
+    // live flags in table order from the runs' control blocks (moved runs keep theirs)
+    h_live.assign(size_t(n), 1);
+    for (int r = 0; r < n; ++r) {
+      Ctrl h{};
+      HIPCHK(hipMemcpy(&h, b->runs[r]->ctrl, sizeof(h), hipMemcpyDeviceToHost));
+      h_live[r] = (h.t < h.tlim && (h.nlim < 0 || h.cycle < h.nlim)) ? 1 : 0;
+    }
+    if (n) HIPCHK(hipMemcpyAsync(d_live, h_live.data(), sizeof(int) * size_t(n), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(d_count, h_count, sizeof(h_count), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return CHQ_OK;
+  }
+
+  // live flags; runs never stepped in this batch get theirs from their control block
+  int live(std::vector<int> &out) override {
+    const int n = int(b->runs.size());
+    out.assign(size_t(n), 0);
+    if (!n) return CHQ_OK;
+    HIPCHK(hipMemcpyAsync(out.data(), d_live, sizeof(int) * size_t(n), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return CHQ_OK;
   }
@@ -218,7 +242,7 @@ struct Batch final : BatchBase {
   ItemList list(int k) const { return ItemList{d_items[k], d_count + 1 + k}; }
   CondList cond() const { return CondList{b->capacity, d_cond}; }
   int ctrl(int op, int arg) {
-    HIPCHK(launchb_ctrl<T>(d_runs, d_count, op, (b->capacity << 8) | arg, b->stream));
+    HIPCHK(launchb_ctrl<T>(d_runs, d_count, b->capacity, op, arg, d_live, b->stream));
     return CHQ_OK;
   }
 
@@ -339,8 +363,6 @@ int capture(chq_batch *b) {
   return CHQ_OK;
 }
 
-bool finished(const Ctrl &h) { return !(h.t < h.tlim && (h.nlim < 0 || h.cycle < h.nlim)); }
-
 }  // namespace
 
 extern "C" {
@@ -403,12 +425,7 @@ int chq_batch_add(chq_batch *b, chq_ctx *c) {
   b->runs.push_back(c);
   c->batch = b;
   c->batch_stream = b->stream;
-  if (int e = b->impl->rebuild()) {
-    b->runs.pop_back();
-    c->batch = nullptr;
-    c->batch_stream = nullptr;
-    return e;
-  }
+  b->dirty = true;
   return CHQ_OK;
 }
 
@@ -422,13 +439,25 @@ int chq_batch_remove(chq_batch *b, chq_ctx *c) {
   c->batch = nullptr;
   c->batch_stream = nullptr;
   c->wcur_stale = true;
-  return b->impl->rebuild();
+  b->dirty = true;
+  return CHQ_OK;
 }
 
 int chq_batch_size(const chq_batch *b) { return b ? int(b->runs.size()) : 0; }
 
+namespace {
+// device tables follow the membership lazily: joins and leaves between two steps cost one rewrite
+int refresh(chq_batch *b) {
+  if (!b->dirty || !b->impl) return CHQ_OK;
+  if (int e = b->impl->rebuild()) return e;
+  b->dirty = false;
+  return CHQ_OK;
+}
+}  // namespace
+
 int chq_batch_step(chq_batch *b, int nsteps) {
   if (!b || nsteps < 0) return fail(CHQ_EINVAL, "bad argument");
+  if (int e = refresh(b)) return e;
   if (b->runs.empty() || nsteps == 0) return CHQ_OK;
   if (b->use_graph && !b->graph_ok)
     if (int e = capture(b)) return e;
@@ -442,6 +471,7 @@ int chq_batch_step(chq_batch *b, int nsteps) {
 
 int chq_batch_step_timed(chq_batch *b, int nsteps, double *ms) {
   if (!b || !ms) return fail(CHQ_EINVAL, "bad argument");
+  if (int e = refresh(b)) return e;
   if (b->use_graph && !b->graph_ok && !b->runs.empty())
     if (int e = capture(b)) return e;
   HIPCHK(hipEventRecord(b->ev0, b->stream));
@@ -462,15 +492,15 @@ int chq_batch_sync(chq_batch *b) {
 
 int chq_batch_unfinished(chq_batch *b, int *count, int *flags) {
   if (!b || !count) return fail(CHQ_EINVAL, "bad argument");
-  HIPCHK(hipStreamSynchronize(b->stream));
+  *count = 0;
+  if (!b->impl) return CHQ_OK;
+  if (int e = refresh(b)) return e;
+  std::vector<int> live;
+  if (int e = b->impl->live(live)) return e;
   int n = 0;
-  for (size_t r = 0; r < b->runs.size(); ++r) {
-    Ctrl h{};
-    HIPCHK(hipMemcpyAsync(&h, b->runs[r]->ctrl, sizeof(h), hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    const int live = finished(h) ? 0 : 1;
-    if (flags) flags[r] = live;
-    n += live;
+  for (size_t r = 0; r < live.size(); ++r) {
+    if (flags) flags[r] = live[r];
+    n += live[r];
   }
   *count = n;
   return CHQ_OK;
@@ -490,6 +520,7 @@ int chq_batch_use_graph(chq_batch *b, int on) {
 int chq_batch_tune(chq_batch *b, int reps, double *best_ms) {
   if (!b || reps < 1) return fail(CHQ_EINVAL, "bad argument");
   if (b->runs.empty()) return fail(CHQ_EINVAL, "empty batch");
+  if (int e = refresh(b)) return e;
   return b->impl->tune(reps, best_ms);
 }
 

@@ -108,6 +108,37 @@ int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
   return CHQ_OK;
 }
 
+// Mesh::Allocate, mesh.hpp:37-48, plus the padded device layout
+void fill_geom(const chq_mesh_desc *m, Geom &g) {
+  g = Geom{};
+  g.nx1 = m->nx1; g.nx2 = m->nx2; g.nx3 = m->nx3; g.ng = m->ng;
+  g.ni = g.nx1 + 2 * g.ng;
+  g.nj = (g.nx2 > 1) ? g.nx2 + 2 * g.ng : 1;
+  g.nk = (g.nx3 > 1) ? g.nx3 + 2 * g.ng : 1;
+  g.is = g.ng; g.ie = g.ng + g.nx1 - 1;
+  g.js = (g.nx2 > 1) ? g.ng : 0; g.je = (g.nx2 > 1) ? g.ng + g.nx2 - 1 : 0;
+  g.ks = (g.nx3 > 1) ? g.ng : 0; g.ke = (g.nx3 > 1) ? g.ng + g.nx3 - 1 : 0;
+  g.two_d = g.nx2 > 1; g.three_d = g.nx3 > 1;
+  g.dx1 = (m->x1max - m->x1min) / g.nx1;
+  g.dx2 = (m->x2max - m->x2min) / g.nx2;
+  g.dx3 = (m->x3max - m->x3min) / g.nx3;
+  g.idx1 = 1.0 / g.dx1; g.idx2 = 1.0 / g.dx2; g.idx3 = 1.0 / g.dx3;   // physics.hpp:309
+  g.ioff = (16 - g.is % 16) % 16;
+  g.pitch = ((g.ioff + g.ni + 1 + 15) / 16) * 16;
+  g.plane = (long long)g.pitch * (g.nj + 1);
+  g.cstride = g.plane * (g.nk + 1);
+}
+
+// device arena of a context: conserved state (12), face B (6), flags (2) as double-sized
+// components, then the field set (w 12, bcc 6, ecc 6, fluxes 18, face EMFs 6, edge EMFs 3) in
+// double (FP64) or float (mixed), padded to 256 B, then two 256-B control blocks
+size_t arena_bytes_for(const Geom &g, bool mixed) {
+  const size_t comp = size_t(g.cstride) * sizeof(double);
+  const size_t nfield = 12 + 6 + 6 + 18 + 6 + 3;
+  const size_t field_bytes = nfield * size_t(g.cstride) * (mixed ? sizeof(float) : sizeof(double));
+  return (12 + 6 + 2) * comp + ((field_bytes + 255) / 256) * 256 + 2 * 256;
+}
+
 int check_cfg(const chq_mesh_desc *m, const chq_config *c) {
   if (!m || !c) return fail(CHQ_EINVAL, "null mesh or config");
   if (m->nx1 < 1 || m->nx2 < 1 || m->nx3 < 1) return fail(CHQ_EINVAL, "mesh sizes must be >= 1");
@@ -143,23 +174,7 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   if (!c) return fail(CHQ_ENOMEM, "host allocation failed");
   c->cfg = *cfg;
   Geom &g = c->g;
-  // Mesh::Allocate, mesh.hpp:37-48
-  g.nx1 = m->nx1; g.nx2 = m->nx2; g.nx3 = m->nx3; g.ng = m->ng;
-  g.ni = g.nx1 + 2 * g.ng;
-  g.nj = (g.nx2 > 1) ? g.nx2 + 2 * g.ng : 1;
-  g.nk = (g.nx3 > 1) ? g.nx3 + 2 * g.ng : 1;
-  g.is = g.ng; g.ie = g.ng + g.nx1 - 1;
-  g.js = (g.nx2 > 1) ? g.ng : 0; g.je = (g.nx2 > 1) ? g.ng + g.nx2 - 1 : 0;
-  g.ks = (g.nx3 > 1) ? g.ng : 0; g.ke = (g.nx3 > 1) ? g.ng + g.nx3 - 1 : 0;
-  g.two_d = g.nx2 > 1; g.three_d = g.nx3 > 1;
-  g.dx1 = (m->x1max - m->x1min) / g.nx1;
-  g.dx2 = (m->x2max - m->x2min) / g.nx2;
-  g.dx3 = (m->x3max - m->x3min) / g.nx3;
-  g.idx1 = 1.0 / g.dx1; g.idx2 = 1.0 / g.dx2; g.idx3 = 1.0 / g.dx3;   // physics.hpp:309
-  g.ioff = (16 - g.is % 16) % 16;
-  g.pitch = ((g.ioff + g.ni + 1 + 15) / 16) * 16;
-  g.plane = (long long)g.pitch * (g.nj + 1);
-  g.cstride = g.plane * (g.nk + 1);
+  fill_geom(m, g);
   c->eos.gamma = cfg->gamma;
   c->eos.gm1 = cfg->gamma - 1.0;
   c->eos.dfloor = cfg->dfloor;
@@ -175,13 +190,8 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
     return cleanup(fail(CHQ_EHIP, std::string("stream/event: ") + hipGetErrorString(e)));
-  // arena: conserved state (12), face B (6), flags (2) as double-sized components, then the
-  // field set (w 12, bcc 6, ecc 6, fluxes 18, face EMFs 6, edge EMFs 3) in double (FP64) or
-  // float (mixed), then two control blocks
   const size_t comp = size_t(g.cstride) * sizeof(double);
-  const size_t nfield = 12 + 6 + 6 + 18 + 6 + 3;
-  const size_t field_bytes = nfield * size_t(g.cstride) * (c->mixed ? sizeof(float) : sizeof(double));
-  c->arena_bytes = (12 + 6 + 2) * comp + ((field_bytes + 255) / 256) * 256 + 2 * 256;
+  c->arena_bytes = arena_bytes_for(g, c->mixed);
   e = hipMalloc((void **)&c->arena, c->arena_bytes);
   if (e != hipSuccess) { c->arena = nullptr; return cleanup(fail(CHQ_ENOMEM, "device arena of " + std::to_string(c->arena_bytes) + " bytes: " + hipGetErrorString(e))); }
   e = hipMemsetAsync(c->arena, 0, c->arena_bytes, c->stream);
@@ -247,6 +257,13 @@ int chq_geometry(const chq_ctx *c, int *iv, double *dx) {
 }
 
 size_t chq_device_bytes(const chq_ctx *c) { return c ? c->arena_bytes + c->staging_bytes : 0; }
+
+size_t chq_bytes_for(const chq_mesh_desc *m, const chq_config *cfg) {
+  if (check_cfg(m, cfg)) return 0;
+  Geom g;
+  fill_geom(m, g);
+  return arena_bytes_for(g, cfg->precision == 1);
+}
 
 size_t chq_array_count(const chq_ctx *c, int id) {
   void *dev;

@@ -231,8 +231,9 @@ template <class T> hipError_t launchb_corner_cond(const RunArgs<T> *R, CondList 
 template <class T> hipError_t launchb_update_cond(const RunArgs<T> *R, CondList C, int out, double g0, double g1, double beta, int grid, hipStream_t s);
 template <class T> hipError_t launchb_bc_cond(const RunArgs<T> *R, CondList C, int dir, int buf, int grid, hipStream_t s);
 template <class T> hipError_t launchb_c2p_cond(const RunArgs<T> *R, CondList C, int buf, int last, int grid, hipStream_t s);
-// control-block step of every run: arg = (table capacity << 8) | operand
-template <class T> hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg, hipStream_t s);
+// control-block step of every run (one thread per table slot, `cap` slots); the end of a step
+// also writes live[r] = run r has not reached tlim / nlim
+template <class T> hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int cap, int op, int arg, int *live, hipStream_t s);
 enum : int { CTRL_BEGIN_STEP = 0, CTRL_BEGIN_STAGE, CTRL_END_STAGE, CTRL_END_STEP };
 
 // position of `it` in an exclusive prefix pre[0..n] (largest p with pre[p] <= it)

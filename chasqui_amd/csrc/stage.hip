@@ -706,15 +706,19 @@ __global__ void __launch_bounds__(256) kb_c2p_cond(const RunArgs<T> *R, const Co
   cond_walk(C, CL_C2P, [&](int r, int tile) { c2p_tile<T, LAST>(R[r], R[r].ctrl, tile, buf, true, smin, ssum); });
 }
 
+// live[r] (end of step) = 1 while run r has not reached tlim / nlim
 template <class T>
-__global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg) {
+__global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg, int *live) {
   const int r = int(blockIdx.x) * 256 + int(threadIdx.x);
   if (r >= *nruns) return;
   Ctrl *c = R[r].ctrl;
   if (op == CTRL_BEGIN_STEP) ctrl_begin_step(c);
   else if (op == CTRL_BEGIN_STAGE) ctrl_begin_stage(c, arg);
   else if (op == CTRL_END_STAGE) ctrl_end_stage(c, arg, 1);
-  else ctrl_end_step(c);
+  else {
+    ctrl_end_step(c);
+    live[r] = (c->t < c->tlim && (c->nlim < 0 || c->cycle < c->nlim)) ? 1 : 0;
+  }
 }
 
 }  // namespace
@@ -858,10 +862,8 @@ hipError_t launchb_c2p_cond(const RunArgs<T> *R, CondList C, int buf, int last, 
   return hipGetLastError();
 }
 template <class T>
-hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg, hipStream_t s) {
-  // one thread per run slot of the table (capacity fixed by the caller through `arg` >> 8)
-  const int cap = arg >> 8;
-  hipLaunchKernelGGL(kb_ctrl<T>, dim3(div_up(cap > 0 ? cap : 1, 256)), dim3(256), 0, s, R, nruns, op, arg & 0xff);
+hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int cap, int op, int arg, int *live, hipStream_t s) {
+  hipLaunchKernelGGL(kb_ctrl<T>, dim3(div_up(cap > 0 ? cap : 1, 256)), dim3(256), 0, s, R, nruns, op, arg, live);
   return hipGetLastError();
 }
 
@@ -877,7 +879,7 @@ hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg, 
                                              hipStream_t);                                                    \
   template hipError_t launchb_bc_cond<T>(const RunArgs<T> *, CondList, int, int, int, hipStream_t);          \
   template hipError_t launchb_c2p_cond<T>(const RunArgs<T> *, CondList, int, int, int, hipStream_t);         \
-  template hipError_t launchb_ctrl<T>(const RunArgs<T> *, const int *, int, int, hipStream_t);
+  template hipError_t launchb_ctrl<T>(const RunArgs<T> *, const int *, int, int, int, int *, hipStream_t);
 CHQ_BATCH_INST(double)
 CHQ_BATCH_INST(float)
 #undef CHQ_BATCH_INST

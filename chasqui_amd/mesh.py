@@ -43,8 +43,19 @@ class Mesh:
         L = lib()
         h = C.c_void_p()
         check(L.chq_create(C.byref(mesh_desc), C.byref(config), C.byref(h)), "chq_create")
+        self._init_handle(L, h, owned=True)
+
+    @classmethod
+    def borrow(cls, handle):
+        """A view of a context owned elsewhere (a served run): closing it does not destroy it."""
+        m = cls.__new__(cls)
+        m._init_handle(lib(), C.c_void_p(handle), owned=False)
+        return m
+
+    def _init_handle(self, L, h, owned):
         self.h = h
         self.L = L
+        self.owned = owned
         iv = (C.c_int * 16)()
         dx = (C.c_double * 3)()
         check(L.chq_geometry(self.h, iv, dx), "chq_geometry")
@@ -55,7 +66,8 @@ class Mesh:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.chq_destroy(self.h)
+            if self.owned:
+                self.L.chq_destroy(self.h)
             self.h = None
 
     def __del__(self):

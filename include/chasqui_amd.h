@@ -83,6 +83,9 @@ int chq_destroy(chq_ctx *ctx);
 int chq_geometry(const chq_ctx *ctx, int *iv, double *dx3);
 /* bytes of device memory held by the context */
 size_t chq_device_bytes(const chq_ctx *ctx);
+/* device bytes chq_create would allocate for this mesh and configuration (0 if invalid); the
+ * serving loop admits runs against an HBM budget with it */
+size_t chq_bytes_for(const chq_mesh_desc *mesh, const chq_config *cfg);
 
 /* Host <-> device copies in the reference layout (pgen::Initialize's deep_copy,
  * pgen.hpp:302-306; outputs, output.hpp:33-34; restart, restart.hpp:146-149). `count` is
@@ -192,6 +195,30 @@ int chq_batch_tune(chq_batch *batch, int reps, double *best_ms);
 /* grid size (blocks) of kernel class `cls` (0..10) and its name */
 int chq_batch_grid(chq_batch *batch, int cls, int *grid, const char **name);
 int chq_batch_set_grid(chq_batch *batch, int cls, int grid);
+
+/* ---- serving: continuous batching of arriving runs (no counterpart in the reference) ----
+ * Runs arrive as decks (chq_deck_open text + overrides) and wait in arrival order; each loop
+ * iteration (chq_server_pump) admits what fits the HBM budget (chq_bytes_for) and the batch
+ * slots (first fit, the first run that does not fit keeps its memory reserved), advances
+ * every batch (one per scheme, each on its own stream) `chunk_steps` steps, and retires the
+ * runs that reached tlim / nlim. A finished run keeps its device state until
+ * chq_server_release; read it through chq_server_job_ctx. Mixed-precision runs whose state
+ * stops being finite are re-run in FP64. */
+typedef struct chq_server chq_server;
+int chq_server_create(size_t hbm_budget, int slots_per_batch, int chunk_steps, chq_server **out);
+int chq_server_destroy(chq_server *srv);
+/* precision: -1 the deck's, 0 FP64, 1 mixed */
+int chq_server_submit(chq_server *srv, const char *deck_text, const char *overrides, int precision, int *job_id);
+int chq_server_pump(chq_server *srv, int max_done, int *done_ids, int *n_done);
+/* info = state (0 queued 1 running 2 done 3 failed 4 released), precision, fell back to FP64,
+ * cycle, t, submit / admit / done times (s, chq_server_clock), device bytes, fofc corrections */
+int chq_server_job(chq_server *srv, int job_id, double info[10]);
+chq_ctx *chq_server_job_ctx(chq_server *srv, int job_id);
+const char *chq_server_job_error(chq_server *srv, int job_id);
+int chq_server_release(chq_server *srv, int job_id);
+/* stats = queued, running, finished, bytes in use, budget, batches, batch steps issued */
+int chq_server_stats(chq_server *srv, double stats[7]);
+double chq_server_clock(void);
 
 #ifdef __cplusplus
 }
