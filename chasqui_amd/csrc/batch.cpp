@@ -86,7 +86,6 @@ struct Batch final : BatchBase {
   int *d_live = nullptr;               // per run: not finished (written at the end of every step)
   std::vector<RunArgs<T>> h_runs;
   std::vector<Item> h_items[K_NLIST];
-  std::vector<int> h_live;
   int h_count[1 + K_NLIST]{};
 
   explicit Batch(chq_batch *bb) : b(bb) {}
@@ -216,14 +215,8 @@ struct Batch final : BatchBase {
                               b->stream));
     }
     if (n) HIPCHK(hipMemcpyAsync(d_runs, h_runs.data(), sizeof(RunArgs<T>) * size_t(n), hipMemcpyHostToDevice, b->stream));
-    // live flags in table order from the runs' control blocks (moved runs keep theirs)
-    h_live.assign(size_t(n), 1);
-    for (int r = 0; r < n; ++r) {
-      Ctrl h{};
-      HIPCHK(hipMemcpy(&h, b->runs[r]->ctrl, sizeof(h), hipMemcpyDeviceToHost));
-      h_live[r] = (h.t < h.tlim && (h.nlim < 0 || h.cycle < h.nlim)) ? 1 : 0;
-    }
-    if (n) HIPCHK(hipMemcpyAsync(d_live, h_live.data(), sizeof(int) * size_t(n), hipMemcpyHostToDevice, b->stream));
+    // live flags in the new table order, from the runs' control blocks
+    if (int e = ctrl(CTRL_LIVE, 0)) return e;
     HIPCHK(hipMemcpyAsync(d_count, h_count, sizeof(h_count), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return CHQ_OK;

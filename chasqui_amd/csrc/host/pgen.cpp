@@ -12,6 +12,8 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "deck.hpp"
 
@@ -38,6 +40,24 @@ void HostGrid::build() {
 
 namespace {
 
+// fn(k, j) over the nk x nj rows, split across host threads for large meshes (every row
+// writes only its own elements, so the result does not depend on the split)
+void for_rows(int nk, int nj, int ni, const std::function<void(int, int)> &fn) {
+  const long long rows = (long long)nk * nj;
+  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (rows * ni < (1 << 17) || nt == 1) {
+    for (int k = 0; k < nk; ++k)
+      for (int j = 0; j < nj; ++j) fn(k, j);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (long long r = t; r < rows; r += nt) fn(int(r / nj), int(r % nj));
+    });
+  for (auto &x : th) x.join();
+}
+
 int parse_bc(const std::string &s) {
   if (s == "periodic") return 0;
   if (s == "outflow") return 1;
@@ -55,8 +75,7 @@ struct Fields {
 void curl_potential(Fields &h, const std::function<double(int, double, double, double)> &A) {
   const HostGrid &m = h.g;
   const double idx = 1.0 / m.dx1, idy = 1.0 / m.dx2, idz = 1.0 / m.dx3;
-  for (int k = 0; k < m.nk + 1; ++k)
-    for (int j = 0; j < m.nj + 1; ++j)
+  for_rows(m.nk + 1, m.nj + 1, m.ni + 1, [&](int k, int j) {
       for (int i = 0; i < m.ni + 1; ++i) {
         const double xf = m.x1f(i), yf = m.x2f(j), zf = m.x3f(k);
         const double xv = m.x1v(i), yv = m.x2v(j), zv = m.x3v(k);
@@ -72,6 +91,7 @@ void curl_potential(Fields &h, const std::function<double(int, double, double, d
         h.b2[f] = by;
         h.b3[f] = bz;
       }
+  });
 }
 
 // circularly polarized Alfven wave, pgen.hpp:54-91 (+ angle2, 3D propagation direction)
@@ -98,8 +118,7 @@ void cpaw(Params &p, Fields &h) {
     });
     if (!m.two_d)   // the uniform B1 of a 1D run cannot come from the potential
       for (size_t f = 0; f < m.nface(); ++f) h.b1[f] += b0 * cth;
-    for (int k = 0; k < m.nk; ++k)
-      for (int j = 0; j < m.nj; ++j)
+    for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
         for (int i = 0; i < m.ni; ++i) {
           const double xp = m.x1v(i) * cth + m.x2v(j) * sth;
           const double vyp = fac * std::sin(kw * xp), vz = fac * std::cos(kw * xp);
@@ -109,6 +128,7 @@ void cpaw(Params &p, Fields &h) {
           h.W(3, k, j, i) = vz;
           h.W(4, k, j, i) = pres;
         }
+    });
     return;
   }
   if (!m.three_d) throw std::runtime_error("cpaw: problem.angle2 != 0 needs a 3D mesh");
@@ -123,8 +143,7 @@ void cpaw(Params &p, Fields &h) {
     const double a3 = b0 * yp + (amp / kw) * std::cos(kw * xp);
     return a2 * e2[c] + a3 * e3[c];
   });
-  for (int k = 0; k < m.nk; ++k)
-    for (int j = 0; j < m.nj; ++j)
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
       for (int i = 0; i < m.ni; ++i) {
         const double xp = (m.x1v(i) * e1[0] + m.x2v(j) * e1[1]) + m.x3v(k) * e1[2];
         const double v2 = fac * std::sin(kw * xp), v3 = fac * std::cos(kw * xp);
@@ -132,6 +151,7 @@ void cpaw(Params &p, Fields &h) {
         for (int c = 0; c < 3; ++c) h.W(1 + c, k, j, i) = (vflow * e1[c] + v2 * e2[c]) + v3 * e3[c];
         h.W(4, k, j, i) = pres;
       }
+  });
 }
 
 // Orszag-Tang vortex, pgen.hpp:94-111 (+ problem.pres, problem.b0)
@@ -143,8 +163,7 @@ void orszag_tang(Params &p, Fields &h) {
     if (c != 2) return 0.0;
     return b0 / (2.0 * kPi) * std::cos(2.0 * kPi * y) + b0 / (4.0 * kPi) * std::cos(4.0 * kPi * x);
   });
-  for (int k = 0; k < m.nk; ++k)
-    for (int j = 0; j < m.nj; ++j)
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
       for (int i = 0; i < m.ni; ++i) {
         h.W(0, k, j, i) = 25.0 / (36.0 * kPi);
         h.W(1, k, j, i) = -std::sin(2.0 * kPi * m.x2v(j));
@@ -152,6 +171,7 @@ void orszag_tang(Params &p, Fields &h) {
         h.W(3, k, j, i) = 0.0;
         h.W(4, k, j, i) = pres;
       }
+  });
 }
 
 // splitmix64 of (seed, all-cell index) mapped to [-1, 1): a counter-based generator, so the
@@ -178,8 +198,7 @@ void blast(Params &p, Fields &h) {
   if (std::fabs(bx) < 1e-14 * b0) bx = 0.0;
   if (std::fabs(by) < 1e-14 * b0) by = 0.0;
   for (size_t f = 0; f < m.nface(); ++f) { h.b1[f] = bx; h.b2[f] = by; h.b3[f] = 0.0; }
-  for (int k = 0; k < m.nk; ++k)
-    for (int j = 0; j < m.nj; ++j)
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
       for (int i = 0; i < m.ni; ++i) {
         const double x = m.x1v(i), y = m.two_d ? m.x2v(j) : 0.0, z = m.three_d ? m.x3v(k) : 0.0;
         const double r = std::sqrt(x * x + y * y + z * z);
@@ -191,6 +210,7 @@ void blast(Params &p, Fields &h) {
         h.W(3, k, j, i) = 0.0;
         h.W(4, k, j, i) = (r < rad) ? pin : pout;
       }
+  });
 }
 
 // Brio-Wu shock tube, pgen.hpp:158-184 (density noise is not provided)
@@ -298,8 +318,7 @@ void Deck::initialize() {
   else throw std::runtime_error("unknown problem: " + problem);
   // conserved variables from primitives and face fields, pgen.hpp:286-301
   const double gm1 = cfg.gamma - 1.0;
-  for (int k = 0; k < g.nk; ++k)
-    for (int j = 0; j < g.nj; ++j)
+  for_rows(g.nk, g.nj, g.ni, [&](int k, int j) {
       for (int i = 0; i < g.ni; ++i) {
         const size_t c = g.c(k, j, i);
         const double bx = 0.5 * (b1[g.f(k, j, i)] + b1[g.f(k, j, i + 1)]);
@@ -314,6 +333,7 @@ void Deck::initialize() {
         w[5 * nc + c] = pr / std::pow(d, cfg.gamma);
         u0[5 * nc + c] = d * w[5 * nc + c];
       }
+  });
 }
 
 }  // namespace chq
@@ -420,8 +440,9 @@ size_t chq_deck_print(const chq_deck *k, char *buf, size_t len) {
 
 int chq_deck_upload(const chq_deck *k, chq_ctx *ctx) {
   if (!k || !ctx) return CHQ_EINVAL;
+  // the primitives are not uploaded: chq_init derives them from u0 and the face fields with
+  // con2prim over every cell, as the reference's driver does before its loop (main.cpp:200-201)
   int rc = chq_upload(ctx, CHQ_U0, k->d.u0.data(), k->d.u0.size());
-  if (!rc) rc = chq_upload(ctx, CHQ_W, k->d.w.data(), k->d.w.size());
   if (!rc) rc = chq_upload(ctx, CHQ_B1F0, k->d.b1.data(), k->d.b1.size());
   if (!rc) rc = chq_upload(ctx, CHQ_B2F0, k->d.b2.data(), k->d.b2.size());
   if (!rc) rc = chq_upload(ctx, CHQ_B3F0, k->d.b3.data(), k->d.b3.size());

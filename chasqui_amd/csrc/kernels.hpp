@@ -234,7 +234,7 @@ template <class T> hipError_t launchb_c2p_cond(const RunArgs<T> *R, CondList C, 
 // control-block step of every run (one thread per table slot, `cap` slots); the end of a step
 // also writes live[r] = run r has not reached tlim / nlim
 template <class T> hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int cap, int op, int arg, int *live, hipStream_t s);
-enum : int { CTRL_BEGIN_STEP = 0, CTRL_BEGIN_STAGE, CTRL_END_STAGE, CTRL_END_STEP };
+enum : int { CTRL_BEGIN_STEP = 0, CTRL_BEGIN_STAGE, CTRL_END_STAGE, CTRL_END_STEP, CTRL_LIVE };
 
 // position of `it` in an exclusive prefix pre[0..n] (largest p with pre[p] <= it)
 __device__ __forceinline__ int locate(const int *pre, int n, int it) {

@@ -63,6 +63,7 @@ struct chq_server {
   std::vector<int> failed;            // runs that could not be created, not yet reported
   int next_id = 1;
   long long steps_issued = 0;
+  double t_launch = 0.0, t_admit = 0.0, t_poll = 0.0, t_retire = 0.0;   // host seconds per phase
 };
 
 namespace {
@@ -176,17 +177,24 @@ int chq_server_submit(chq_server *s, const char *deck_text, const char *override
 int chq_server_pump(chq_server *s, int max_done, int *done_ids, int *n_done) {
   if (!s || !n_done) return fail(CHQ_EINVAL, "bad argument");
   *n_done = 0;
+  // launch the running batches first, then do the admission work (deck setup, uploads,
+  // con2prim on the new runs' own streams) on the host while the device steps
+  double t0 = now_s();
+  for (auto &kv : s->batches)
+    if (chq_batch_size(kv.second) > 0) {
+      if (int e = chq_batch_step(kv.second, s->chunk)) return e;
+      s->steps_issued += s->chunk;
+    }
+  double t1 = now_s();
+  s->t_launch += t1 - t0;
   if (int e = admit(s)) return e;
   for (int id : s->failed) {
     if (done_ids && *n_done < max_done) done_ids[*n_done] = id;
     ++*n_done;
   }
   s->failed.clear();
-  for (auto &kv : s->batches)
-    if (chq_batch_size(kv.second) > 0) {
-      if (int e = chq_batch_step(kv.second, s->chunk)) return e;
-      s->steps_issued += s->chunk;
-    }
+  double t2 = now_s();
+  s->t_admit += t2 - t1;
   std::vector<int> flags;
   for (auto &kv : s->batches) {
     chq_batch *b = kv.second;
@@ -194,7 +202,9 @@ int chq_server_pump(chq_server *s, int max_done, int *done_ids, int *n_done) {
     if (!n) continue;
     flags.assign(size_t(n), 1);
     int live = 0;
+    const double tp = now_s();
     if (int e = chq_batch_unfinished(b, &live, flags.data())) return e;
+    s->t_poll += now_s() - tp;
     if (live == n) continue;
     std::vector<chq_ctx *> gone;
     for (int r = 0; r < n; ++r)
@@ -223,6 +233,7 @@ int chq_server_pump(chq_server *s, int max_done, int *done_ids, int *n_done) {
       }
     }
   }
+  s->t_retire += now_s() - t2;
   return CHQ_OK;
 }
 
@@ -276,8 +287,9 @@ int chq_server_release(chq_server *s, int job_id) {
   return CHQ_OK;
 }
 
-// stats[0..6] = queued, running, finished (not released), in_use bytes, budget, batches, steps
-int chq_server_stats(chq_server *s, double stats[7]) {
+// stats[0..10] = queued, running, finished (not released), in_use bytes, budget, batches,
+// batch steps, host seconds launching steps / admitting / waiting for polls / retiring
+int chq_server_stats(chq_server *s, double stats[11]) {
   if (!s || !stats) return fail(CHQ_EINVAL, "bad argument");
   int q = 0, r = 0, d = 0;
   for (auto &kv : s->jobs) {
@@ -292,6 +304,10 @@ int chq_server_stats(chq_server *s, double stats[7]) {
   stats[4] = double(s->budget);
   stats[5] = double(s->batches.size());
   stats[6] = double(s->steps_issued);
+  stats[7] = s->t_launch;
+  stats[8] = s->t_admit;
+  stats[9] = s->t_poll;
+  stats[10] = s->t_retire;
   return CHQ_OK;
 }
 

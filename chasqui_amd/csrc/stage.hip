@@ -716,7 +716,7 @@ __global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *n
   else if (op == CTRL_BEGIN_STAGE) ctrl_begin_stage(c, arg);
   else if (op == CTRL_END_STAGE) ctrl_end_stage(c, arg, 1);
   else {
-    ctrl_end_step(c);
+    if (op == CTRL_END_STEP) ctrl_end_step(c);
     live[r] = (c->t < c->tlim && (c->nlim < 0 || c->cycle < c->nlim)) ? 1 : 0;
   }
 }

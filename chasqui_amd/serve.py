@@ -69,10 +69,11 @@ class Server:
         check(self.L.chq_server_release(self.h, jid), "chq_server_release")
 
     def stats(self):
-        s = (C.c_double * 7)()
+        s = (C.c_double * 11)()
         check(self.L.chq_server_stats(self.h, s), "chq_server_stats")
         return {"queued": int(s[0]), "running": int(s[1]), "finished": int(s[2]), "bytes_in_use": int(s[3]),
-                "budget": int(s[4]), "batches": int(s[5]), "batch_steps": int(s[6])}
+                "budget": int(s[4]), "batches": int(s[5]), "batch_steps": int(s[6]), "host_launch_s": s[7],
+                "host_admit_s": s[8], "host_poll_wait_s": s[9], "host_retire_s": s[10]}
 
     def clock(self):
         return self.L.chq_server_clock()

@@ -154,7 +154,8 @@ const char *chq_deck_get(const chq_deck *deck, const char *block, const char *ke
 int chq_deck_initial(const chq_deck *deck, int array, void *host, size_t count);
 /* Params::Print (params.hpp:82-90) into buf; returns the length needed (with the NUL) */
 size_t chq_deck_print(const chq_deck *deck, char *buf, size_t len);
-/* upload the initial state (u0, w, b?f0) into a context created from chq_deck_mesh/config */
+/* upload the initial state (u0, b?f0) into a context created from chq_deck_mesh/config; the
+ * primitives follow from chq_init (con2prim over every cell, main.cpp:200-201) */
 int chq_deck_upload(const chq_deck *deck, chq_ctx *ctx);
 
 /* output::WriteVTK (output.hpp:32-85): legacy VTK, PLUTO layout, of the current primitives */
@@ -216,8 +217,9 @@ int chq_server_job(chq_server *srv, int job_id, double info[10]);
 chq_ctx *chq_server_job_ctx(chq_server *srv, int job_id);
 const char *chq_server_job_error(chq_server *srv, int job_id);
 int chq_server_release(chq_server *srv, int job_id);
-/* stats = queued, running, finished, bytes in use, budget, batches, batch steps issued */
-int chq_server_stats(chq_server *srv, double stats[7]);
+/* stats = queued, running, finished, bytes in use, budget, batches, batch steps issued, host
+ * seconds spent launching steps, admitting, waiting for polls, retiring */
+int chq_server_stats(chq_server *srv, double stats[11]);
 double chq_server_clock(void);
 
 #ifdef __cplusplus

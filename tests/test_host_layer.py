@@ -18,6 +18,10 @@ CASES = [
     ("orszag_tang.in", ["mesh.nx1=20", "mesh.nx2=24"]),
     ("orszag_tang.in", ["mesh.nx1=16", "mesh.nx2=16", "problem.pres=0.14", "problem.b0=0.3"]),
     ("brio_wu.in", ["mesh.nx1=100"]),
+    # large enough for the host layer to split the setup across threads
+    ("blast_3d.in", ["mesh.nx1=64", "mesh.nx2=64", "mesh.nx3=64"]),
+    ("orszag_tang.in", ["mesh.nx1=384", "mesh.nx2=384"]),
+    ("cpaw_3d_oblique.in", ["mesh.nx1=48", "mesh.nx2=48", "mesh.nx3=48"]),
 ]
 
 
