@@ -219,9 +219,9 @@ def ensemble(nruns, steps, precision, seed=7, tune=True):
 
 
 def serve_jobs(njobs, seed=11):
-    """Served stream: seeded mix of 1D Brio-Wu shock tubes (400-4000 cells), 2D Orszag-Tang
-    (128^2-1024^2) and 3D blasts (64^3-192^3), reconstruction from {ppm, wenoz}. Each run is
-    capped at a step budget (1D 200, 2D 40, 3D 20 steps) so the stream finishes in minutes."""
+    """Served stream: seeded mix of 1D Brio-Wu shock tubes (400-4000 cells) run to their
+    tlim = 0.1, 2D Orszag-Tang (128^2-1024^2) and 3D blasts (64^3-192^3) capped at 100 and 50
+    steps so the stream finishes in minutes; reconstruction drawn from {ppm, wenoz}."""
     import numpy as np
     rng = np.random.default_rng(seed)
     jobs = []
@@ -230,14 +230,14 @@ def serve_jobs(njobs, seed=11):
         rec = str(rng.choice(["ppm", "wenoz"]))
         if kind == "1d":
             n = int(rng.integers(400, 4001))
-            jobs.append((BW_DECK, [f"mesh.nx1={n}", f"mhd.reconstruct={rec}", "time.nlim=200"]))
+            jobs.append((BW_DECK, [f"mesh.nx1={n}", f"mhd.reconstruct={rec}"]))
         elif kind == "2d":
             n = int(rng.choice([128, 256, 384, 512, 768, 1024]))
-            jobs.append((OT_DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mhd.reconstruct={rec}", "time.nlim=40"]))
+            jobs.append((OT_DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mhd.reconstruct={rec}", "time.nlim=100"]))
         else:
             n = int(rng.choice([64, 96, 128, 160, 192]))
             jobs.append((DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}", f"mhd.reconstruct={rec}",
-                                "time.nlim=20", "time.tlim=1e9"]))
+                                "time.nlim=50", "time.tlim=1e9"]))
     return jobs
 
 

@@ -93,6 +93,8 @@ def lib():
         "chq_kernel_timing": (C.c_int, [vp, C.c_int]),
         "chq_kernel_timing_read": (C.c_int, [vp, dp, ip]),
         "chq_deck_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+        "chq_deck_open_lazy": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+        "chq_deck_prepare": (C.c_int, [vp]),
         "chq_deck_close": (C.c_int, [vp]),
         "chq_deck_last_error": (C.c_char_p, []),
         "chq_deck_mesh": (C.c_int, [vp, C.POINTER(MeshDesc)]),
@@ -145,7 +147,7 @@ def exported_symbols():
         "chq_corner_emf", "chq_update", "chq_boundaries", "chq_cons_to_prim", "chq_fofc_flag",
         "chq_fofc_repair", "chq_stage", "chq_init", "chq_step", "chq_step_timed",
         "chq_get_status", "chq_sync", "chq_stream", "chq_kernel_timing", "chq_kernel_timing_read",
-        "chq_deck_open", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
+        "chq_deck_open", "chq_deck_open_lazy", "chq_deck_prepare", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
         "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",
         "chq_write_vtk", "chq_history",
         "chq_batch_create", "chq_batch_destroy", "chq_batch_add", "chq_batch_remove", "chq_batch_size",

@@ -38,8 +38,8 @@ def _compile(src):
     obj = os.path.join(OBJ, src.replace("/", "_") + ".o")
     if src.endswith(".hip"):
         cmd = [HIPCC] + DEVICE + COMMON + EXTRA.get(src, []) + ["-c", path, "-o", obj]
-    else:
-        cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
+    else:   # host code: no device pass
+        cmd = [HIPCC, "--offload-host-only"] + COMMON + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -215,9 +215,9 @@ struct Batch final : BatchBase {
                               b->stream));
     }
     if (n) HIPCHK(hipMemcpyAsync(d_runs, h_runs.data(), sizeof(RunArgs<T>) * size_t(n), hipMemcpyHostToDevice, b->stream));
-    // live flags in the new table order, from the runs' control blocks
-    if (int e = ctrl(CTRL_LIVE, 0)) return e;
     HIPCHK(hipMemcpyAsync(d_count, h_count, sizeof(h_count), hipMemcpyHostToDevice, b->stream));
+    // live flags in the new table order, from the runs' control blocks (after the run count)
+    if (int e = ctrl(CTRL_LIVE, 0)) return e;
     HIPCHK(hipStreamSynchronize(b->stream));
     return CHQ_OK;
   }

@@ -52,8 +52,8 @@ void for_rows(int nk, int nj, int ni, const std::function<void(int, int)> &fn) {
   }
   std::vector<std::thread> th;
   for (unsigned t = 0; t < nt; ++t)
-    th.emplace_back([&, t] {
-      for (long long r = t; r < rows; r += nt) fn(int(r / nj), int(r % nj));
+    th.emplace_back([&, t] {   // a contiguous block of rows per thread
+      for (long long r = rows * t / nt; r < rows * (t + 1) / nt; ++r) fn(int(r / nj), int(r % nj));
     });
   for (auto &x : th) x.join();
 }
@@ -67,7 +67,7 @@ int parse_bc(const std::string &s) {
 
 struct Fields {
   const HostGrid &g;
-  std::vector<double> &w, &b1, &b2, &b3;
+  HostArray &w, &b1, &b2, &b3;
   double &W(int n, int k, int j, int i) { return w[size_t(n) * g.ncell() + g.c(k, j, i)]; }
 };
 
@@ -197,7 +197,14 @@ void blast(Params &p, Fields &h) {
   double bx = b0 * std::cos(ang), by = b0 * std::sin(ang);
   if (std::fabs(bx) < 1e-14 * b0) bx = 0.0;
   if (std::fabs(by) < 1e-14 * b0) by = 0.0;
-  for (size_t f = 0; f < m.nface(); ++f) { h.b1[f] = bx; h.b2[f] = by; h.b3[f] = 0.0; }
+  for_rows(m.nk + 1, m.nj + 1, m.ni + 1, [&](int k, int j) {
+    for (int i = 0; i < m.ni + 1; ++i) {
+      const size_t f = m.f(k, j, i);
+      h.b1[f] = bx;
+      h.b2[f] = by;
+      h.b3[f] = 0.0;
+    }
+  });
   for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
       for (int i = 0; i < m.ni; ++i) {
         const double x = m.x1v(i), y = m.two_d ? m.x2v(j) : 0.0, z = m.three_d ? m.x3v(k) : 0.0;
@@ -223,16 +230,15 @@ void shock_tube(Params &p, Fields &h) {
   const double x0 = p.get_real("problem", "x0", 0.0);
   const double ul = p.get_real("problem", "ul", 0.0), ur = p.get_real("problem", "ur", 0.0);
   if (p.get_real("problem", "perturb", 0.0) != 0.0) throw std::runtime_error("shock_tube: problem.perturb is not supported");
-  for (int k = 0; k < m.nk + 1; ++k)
-    for (int j = 0; j < m.nj + 1; ++j)
+  for_rows(m.nk + 1, m.nj + 1, m.ni + 1, [&](int k, int j) {
       for (int i = 0; i < m.ni + 1; ++i) {
         const size_t f = m.f(k, j, i);
         h.b1[f] = bx;
         h.b2[f] = (m.x1v(i) < x0) ? byl : byr;
         h.b3[f] = 0.0;
       }
-  for (int k = 0; k < m.nk; ++k)
-    for (int j = 0; j < m.nj; ++j)
+  });
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
       for (int i = 0; i < m.ni; ++i) {
         const bool left = m.x1v(i) < x0;
         h.W(0, k, j, i) = (left ? dl : dr) * (1.0 + 0.0);
@@ -241,6 +247,7 @@ void shock_tube(Params &p, Fields &h) {
         h.W(3, k, j, i) = 0.0;
         h.W(4, k, j, i) = left ? pl : pr;
       }
+  });
 }
 
 }  // namespace
@@ -305,11 +312,12 @@ void Deck::configure() {
 
 void Deck::initialize() {
   const size_t nc = g.ncell(), nf = g.nface();
-  w.assign(6 * nc, 0.0);
-  u0.assign(6 * nc, 0.0);
-  b1.assign(nf, 0.0);
-  b2.assign(nf, 0.0);
-  b3.assign(nf, 0.0);
+  // every element is written below (no zero fill)
+  w.resize(6 * nc);
+  u0.resize(6 * nc);
+  b1.resize(nf);
+  b2.resize(nf);
+  b3.resize(nf);
   Fields h{g, w, b1, b2, b3};
   if (problem == "cpaw") cpaw(p, h);
   else if (problem == "orszag_tang") orszag_tang(p, h);
@@ -351,7 +359,8 @@ extern "C" {
 
 const char *chq_deck_last_error(void) { return g_deck_err.c_str(); }
 
-int chq_deck_open(const char *text, const char *overrides, chq_deck **out) {
+namespace {
+int open_deck(const char *text, const char *overrides, bool lazy, chq_deck **out) {
   if (!out || !text) { g_deck_err = "null argument"; return CHQ_EINVAL; }
   *out = nullptr;
   chq_deck *k = new chq_deck();
@@ -367,7 +376,10 @@ int chq_deck_open(const char *text, const char *overrides, chq_deck **out) {
       }
     }
     k->d.configure();
-    k->d.initialize();
+    if (!lazy) {
+      k->d.initialize();
+      k->d.ready = true;
+    }
   } catch (const std::exception &e) {
     g_deck_err = e.what();
     delete k;
@@ -375,6 +387,34 @@ int chq_deck_open(const char *text, const char *overrides, chq_deck **out) {
   }
   *out = k;
   return CHQ_OK;
+}
+
+// the initial state of a lazily opened deck is built on first use
+int ensure_state(const chq_deck *k) {
+  if (k->d.ready) return CHQ_OK;
+  try {
+    auto &d = const_cast<chq::Deck &>(k->d);
+    d.initialize();
+    d.ready = true;
+  } catch (const std::exception &e) {
+    g_deck_err = e.what();
+    return CHQ_EINVAL;
+  }
+  return CHQ_OK;
+}
+}  // namespace
+
+int chq_deck_open(const char *text, const char *overrides, chq_deck **out) {
+  return open_deck(text, overrides, false, out);
+}
+
+int chq_deck_open_lazy(const char *text, const char *overrides, chq_deck **out) {
+  return open_deck(text, overrides, true, out);
+}
+
+int chq_deck_prepare(chq_deck *k) {
+  if (!k) return CHQ_EINVAL;
+  return ensure_state(k);
 }
 
 int chq_deck_close(chq_deck *k) {
@@ -404,7 +444,8 @@ int chq_deck_run(const chq_deck *k, double *tlim, int *nlim, int *ndiag) {
 
 int chq_deck_initial(const chq_deck *k, int array, void *host, size_t count) {
   if (!k || !host) return CHQ_EINVAL;
-  const std::vector<double> *v = nullptr;
+  if (int e = ensure_state(k)) return e;
+  const chq::HostArray *v = nullptr;
   switch (array) {
     case CHQ_U0: v = &k->d.u0; break;
     case CHQ_W: v = &k->d.w; break;
@@ -440,6 +481,7 @@ size_t chq_deck_print(const chq_deck *k, char *buf, size_t len) {
 
 int chq_deck_upload(const chq_deck *k, chq_ctx *ctx) {
   if (!k || !ctx) return CHQ_EINVAL;
+  if (int e = ensure_state(k)) return e;
   // the primitives are not uploaded: chq_init derives them from u0 and the face fields with
   // con2prim over every cell, as the reference's driver does before its loop (main.cpp:200-201)
   int rc = chq_upload(ctx, CHQ_U0, k->d.u0.data(), k->d.u0.size());

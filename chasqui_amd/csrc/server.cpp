@@ -5,6 +5,9 @@
 // frees their slot for the next arrival. The result stays on the device until the client
 // releases the run, which returns its memory to the budget.
 //
+// The loop keeps the device one chunk ahead of the host: every pump collects the chunk the
+// previous pump launched, launches the next one, and does the admission work while it runs.
+//
 // Admission is first-fit over the queue in arrival order, with the head's memory reserved: a
 // later run may pass a head that does not fit only if the head would still fit beside it, so
 // the head is admitted as soon as the running work ahead of it drains.
@@ -13,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <future>
 #include <map>
 #include <string>
 #include <tuple>
@@ -34,7 +38,9 @@ enum : int { JOB_QUEUED = 0, JOB_RUNNING = 1, JOB_DONE = 2, JOB_FAILED = 3, JOB_
 
 struct Job {
   int id = 0;
+  std::string text, overrides;   // kept to set the run up again (FP64 fallback)
   chq_deck *deck = nullptr;
+  std::shared_future<int> prep;   // host problem setup running ahead of admission
   chq_mesh_desc mesh{};
   chq_config cfg{};
   double tlim = 0.0;
@@ -61,6 +67,8 @@ struct chq_server {
   std::map<Key, chq_batch *> batches;
   std::map<chq_ctx *, int> running;   // context -> job
   std::vector<int> failed;            // runs that could not be created, not yet reported
+  std::vector<chq_ctx *> to_free;     // released runs, destroyed when the device is idle
+  bool inflight = false;              // a chunk launched by the last pump is running
   int next_id = 1;
   long long steps_issued = 0;
   double t_launch = 0.0, t_admit = 0.0, t_poll = 0.0, t_retire = 0.0;   // host seconds per phase
@@ -75,15 +83,31 @@ bool state_ok(chq_ctx *c) {
   return std::isfinite(st.t) && std::isfinite(st.min_dt) && st.min_dt > 0.0;
 }
 
+// start the host problem setup of the next queued runs on worker threads, at most `ahead`
+// prepared and not yet admitted at a time (each holds its initial state in host memory)
+void prepare_ahead(chq_server *s, int ahead) {
+  int live = 0;
+  for (int id : s->queue) {
+    Job &j = s->jobs[id];
+    if (j.prep.valid()) { ++live; continue; }
+    if (live >= ahead) break;
+    chq_deck *d = j.deck;
+    j.prep = std::async(std::launch::async, [d] { return chq_deck_prepare(d); }).share();
+    ++live;
+  }
+}
+
 int admit_one(chq_server *s, Job &j) {
   chq_batch *&b = s->batches[key_of(j.cfg)];
   if (!b) {
     if (int e = chq_batch_create(s->slots, &b)) { s->batches.erase(key_of(j.cfg)); return e; }
   }
   if (chq_batch_size(b) >= s->slots) return 1;   // no free slot: stays queued
+  if (j.prep.valid()) j.prep.wait();             // the worker's setup must be complete
   chq_ctx *c = nullptr;
   if (int e = chq_create(&j.mesh, &j.cfg, &c)) return e;
   int e = chq_deck_upload(j.deck, c);
+  if (e == CHQ_EINVAL) e = fail(CHQ_EINVAL, std::string("problem setup: ") + chq_deck_last_error());
   if (!e) e = chq_init(c, 0.0, j.tlim, j.nlim);
   if (!e) e = chq_batch_add(b, c);
   if (e) { chq_destroy(c); return e; }
@@ -92,6 +116,8 @@ int admit_one(chq_server *s, Job &j) {
   j.t_admit = now_s();
   s->in_use += j.bytes;
   s->running[c] = j.id;
+  chq_deck_close(j.deck);   // the initial state is on the device now
+  j.deck = nullptr;
   return CHQ_OK;
 }
 
@@ -141,7 +167,9 @@ int chq_server_create(size_t hbm_budget, int slots_per_batch, int chunk_steps, c
 int chq_server_destroy(chq_server *s) {
   if (!s) return CHQ_OK;
   for (auto &kv : s->batches) chq_batch_destroy(kv.second);
+  for (chq_ctx *c : s->to_free) chq_destroy(c);
   for (auto &kv : s->jobs) {
+    if (kv.second.prep.valid()) kv.second.prep.wait();
     if (kv.second.ctx) chq_destroy(kv.second.ctx);
     if (kv.second.deck) chq_deck_close(kv.second.deck);
   }
@@ -153,7 +181,9 @@ int chq_server_destroy(chq_server *s) {
 int chq_server_submit(chq_server *s, const char *deck_text, const char *overrides, int precision, int *job_id) {
   if (!s || !deck_text || !job_id) return fail(CHQ_EINVAL, "bad argument");
   Job j;
-  if (chq_deck_open(deck_text, overrides, &j.deck)) return fail(CHQ_EINVAL, std::string("deck: ") + chq_deck_last_error());
+  // parse now; the problem setup runs at admission, on the host while batches step
+  if (chq_deck_open_lazy(deck_text, overrides, &j.deck))
+    return fail(CHQ_EINVAL, std::string("deck: ") + chq_deck_last_error());
   int ndiag = 0;
   chq_deck_mesh(j.deck, &j.mesh);
   chq_deck_config(j.deck, &j.cfg);
@@ -163,6 +193,8 @@ int chq_server_submit(chq_server *s, const char *deck_text, const char *override
   j.bytes = chq_bytes_for(&j.mesh, &j.cfg);
   if (j.bytes == 0) { chq_deck_close(j.deck); return fail(CHQ_EINVAL, "invalid mesh or configuration"); }
   if (j.bytes > s->budget) { chq_deck_close(j.deck); return fail(CHQ_ENOMEM, "run larger than the HBM budget"); }
+  j.text = deck_text;
+  j.overrides = overrides ? overrides : "";
   j.id = s->next_id++;
   j.t_submit = now_s();
   s->queue.push_back(j.id);
@@ -171,69 +203,85 @@ int chq_server_submit(chq_server *s, const char *deck_text, const char *override
   return CHQ_OK;
 }
 
-// One iteration of the loop: admit what fits, advance every batch `chunk` steps (batches run
-// concurrently on their own streams), then retire the runs that finished. done_ids receives
-// up to max_done finished (or failed) job ids; n_done their number.
+// One iteration of the loop. The device runs one chunk (`chunk` steps of every batch, batches
+// concurrently on their own streams) per call, ahead of the host: a call first collects the
+// chunk launched by the previous call (retiring the runs that finished), frees released runs
+// while the device is idle, launches the next chunk, and then does the admission work (deck
+// setup, uploads, con2prim of new runs on their own streams) while that chunk runs, returning
+// without waiting for it. done_ids receives up to max_done finished (or failed) job ids.
 int chq_server_pump(chq_server *s, int max_done, int *done_ids, int *n_done) {
   if (!s || !n_done) return fail(CHQ_EINVAL, "bad argument");
   *n_done = 0;
-  // launch the running batches first, then do the admission work (deck setup, uploads,
-  // con2prim on the new runs' own streams) on the host while the device steps
-  double t0 = now_s();
+  auto report = [&](int id) {
+    if (done_ids && *n_done < max_done) done_ids[*n_done] = id;
+    ++*n_done;
+  };
+  const double t0 = now_s();
+  if (s->inflight) {
+    std::vector<int> flags;
+    for (auto &kv : s->batches) {
+      chq_batch *b = kv.second;
+      const int n = chq_batch_size(b);
+      if (!n) continue;
+      flags.assign(size_t(n), 1);
+      int live = 0;
+      const double tp = now_s();
+      if (int e = chq_batch_unfinished(b, &live, flags.data())) return e;
+      s->t_poll += now_s() - tp;
+      if (live == n) continue;
+      std::vector<chq_ctx *> gone;
+      for (int r = 0; r < n; ++r)
+        if (!flags[size_t(r)]) gone.push_back(chq_batch_run(b, r));
+      const double t = now_s();
+      for (chq_ctx *c : gone) {
+        if (int e = chq_batch_remove(b, c)) return e;
+        auto rj = s->running.find(c);
+        if (rj == s->running.end()) continue;
+        Job &j = s->jobs[rj->second];
+        s->running.erase(rj);
+        if (j.cfg.precision == 1 && !state_ok(c)) {   // FP64 fallback: run again from the start
+          chq_destroy(c);
+          j.ctx = nullptr;
+          s->in_use -= j.bytes;
+          j.cfg.precision = 0;
+          j.bytes = chq_bytes_for(&j.mesh, &j.cfg);
+          j.fell_back = 1;
+          j.prep = std::shared_future<int>();
+        if (chq_deck_open_lazy(j.text.c_str(), j.overrides.c_str(), &j.deck)) {
+            j.state = JOB_FAILED;
+            j.error = std::string("deck: ") + chq_deck_last_error();
+            j.t_done = t;
+            report(j.id);
+            continue;
+          }
+          j.state = JOB_QUEUED;
+          s->queue.push_front(j.id);
+        } else {
+          j.state = JOB_DONE;
+          j.t_done = t;
+          report(j.id);
+        }
+      }
+    }
+    s->inflight = false;
+  }
+  for (chq_ctx *c : s->to_free) chq_destroy(c);   // the device is idle here
+  s->to_free.clear();
+  const double t1 = now_s();
+  s->t_retire += t1 - t0;
   for (auto &kv : s->batches)
     if (chq_batch_size(kv.second) > 0) {
       if (int e = chq_batch_step(kv.second, s->chunk)) return e;
       s->steps_issued += s->chunk;
+      s->inflight = true;
     }
-  double t1 = now_s();
-  s->t_launch += t1 - t0;
+  const double t2 = now_s();
+  s->t_launch += t2 - t1;
   if (int e = admit(s)) return e;
-  for (int id : s->failed) {
-    if (done_ids && *n_done < max_done) done_ids[*n_done] = id;
-    ++*n_done;
-  }
+  for (int id : s->failed) report(id);
   s->failed.clear();
-  double t2 = now_s();
-  s->t_admit += t2 - t1;
-  std::vector<int> flags;
-  for (auto &kv : s->batches) {
-    chq_batch *b = kv.second;
-    const int n = chq_batch_size(b);
-    if (!n) continue;
-    flags.assign(size_t(n), 1);
-    int live = 0;
-    const double tp = now_s();
-    if (int e = chq_batch_unfinished(b, &live, flags.data())) return e;
-    s->t_poll += now_s() - tp;
-    if (live == n) continue;
-    std::vector<chq_ctx *> gone;
-    for (int r = 0; r < n; ++r)
-      if (!flags[size_t(r)]) gone.push_back(chq_batch_run(b, r));
-    const double t = now_s();
-    for (chq_ctx *c : gone) {
-      if (int e = chq_batch_remove(b, c)) return e;
-      auto rj = s->running.find(c);
-      if (rj == s->running.end()) continue;
-      Job &j = s->jobs[rj->second];
-      s->running.erase(rj);
-      if (j.cfg.precision == 1 && !state_ok(c)) {   // FP64 fallback: run again from the start
-        chq_destroy(c);
-        j.ctx = nullptr;
-        s->in_use -= j.bytes;
-        j.cfg.precision = 0;
-        j.bytes = chq_bytes_for(&j.mesh, &j.cfg);
-        j.fell_back = 1;
-        j.state = JOB_QUEUED;
-        s->queue.push_front(j.id);
-      } else {
-        j.state = JOB_DONE;
-        j.t_done = t;
-        if (done_ids && *n_done < max_done) done_ids[*n_done] = j.id;
-        ++*n_done;
-      }
-    }
-  }
-  s->t_retire += now_s() - t2;
+  prepare_ahead(s, 4);
+  s->t_admit += now_s() - t2;
   return CHQ_OK;
 }
 
@@ -277,11 +325,13 @@ int chq_server_release(chq_server *s, int job_id) {
   if (it == s->jobs.end()) return fail(CHQ_EINVAL, "unknown job");
   Job &j = it->second;
   if (j.state != JOB_DONE && j.state != JOB_FAILED) return fail(CHQ_EINVAL, "job has not finished");
-  if (j.ctx) {
-    chq_destroy(j.ctx);
+  if (j.ctx) {   // freeing would wait for the running chunk: defer to the next pump
+    if (s->inflight) s->to_free.push_back(j.ctx);
+    else chq_destroy(j.ctx);
     j.ctx = nullptr;
     s->in_use -= j.bytes;
   }
+  if (j.prep.valid()) j.prep.wait();
   if (j.deck) { chq_deck_close(j.deck); j.deck = nullptr; }
   j.state = JOB_RELEASED;
   return CHQ_OK;

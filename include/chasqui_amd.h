@@ -143,6 +143,13 @@ typedef struct chq_deck chq_deck;
  * (main.cpp:112-185) and the problem generator (pgen.hpp:269-307: cpaw, orszag_tang, blast,
  * shock_tube). `overrides`: newline-separated block.key=value lines, or NULL. */
 int chq_deck_open(const char *text, const char *overrides, chq_deck **out);
+/* the same, with the initial state built on first use (chq_deck_initial / chq_deck_upload), so
+ * a serving loop can parse at arrival and set up at admission; problem-setup errors then
+ * surface there */
+int chq_deck_open_lazy(const char *text, const char *overrides, chq_deck **out);
+/* build a lazily opened deck's initial state now (e.g. on a worker thread); not to be called
+ * concurrently with other calls on the same deck */
+int chq_deck_prepare(chq_deck *deck);
 int chq_deck_close(chq_deck *deck);
 const char *chq_deck_last_error(void);
 int chq_deck_mesh(const chq_deck *deck, chq_mesh_desc *mesh);
@@ -199,12 +206,13 @@ int chq_batch_set_grid(chq_batch *batch, int cls, int grid);
 
 /* ---- serving: continuous batching of arriving runs (no counterpart in the reference) ----
  * Runs arrive as decks (chq_deck_open text + overrides) and wait in arrival order; each loop
- * iteration (chq_server_pump) admits what fits the HBM budget (chq_bytes_for) and the batch
- * slots (first fit, the first run that does not fit keeps its memory reserved), advances
- * every batch (one per scheme, each on its own stream) `chunk_steps` steps, and retires the
- * runs that reached tlim / nlim. A finished run keeps its device state until
- * chq_server_release; read it through chq_server_job_ctx. Mixed-precision runs whose state
- * stops being finite are re-run in FP64. */
+ * iteration (chq_server_pump) retires the runs that reached tlim / nlim in the chunk the
+ * previous iteration launched, launches the next chunk (`chunk_steps` steps of every batch,
+ * one batch per scheme, each on its own stream) and, while it runs, admits what fits the HBM
+ * budget (chq_bytes_for) and the batch slots (first fit; the first run that does not fit
+ * keeps its memory reserved). It returns without waiting for the chunk. A finished run keeps
+ * its device state until chq_server_release; read it through chq_server_job_ctx.
+ * Mixed-precision runs whose state stops being finite are re-run in FP64. */
 typedef struct chq_server chq_server;
 int chq_server_create(size_t hbm_budget, int slots_per_batch, int chunk_steps, chq_server **out);
 int chq_server_destroy(chq_server *srv);
