@@ -8,6 +8,8 @@
 //
 // Launch shape: blocks of 64 x 4 threads (i fastest), one wave64 per row segment, so all
 // loads and stores of a wave are contiguous 512 B segments.
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "numerics.hpp"
 
@@ -96,7 +98,9 @@ template <class T>
 __global__ void __launch_bounds__(256) k_corner(const CornerArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
-  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+  const bool one_per_tile = int(gridDim.x) == v.n;   // normal launch: XCD-contiguous tiles
+  for (int b = blockIdx.x; b < v.n; b += gridDim.x) {
+    const int vb = one_per_tile ? xcd_tile(b, v.n) : b;
     int k, j, i;
     if (!v.at(vb, k, j, i)) continue;
     if (g.three_d) corner3_at(a, g.ks + k, g.js + j, g.is + i);
@@ -145,32 +149,38 @@ __device__ __forceinline__ void update_at(const UpdateArgsT<T> &a, const double 
   const double g0 = a.g0, g1 = a.g1;
   const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
   const double idx1 = g.idx1, idx2 = g.idx2, idx3 = g.idx3;
+  // g0 * xa + g1 * xb; the mixed-precision path skips the read of xb when g1 == 0 (the first
+  // RK stage), the FP64 path keeps the reference's expression (it fixes the sign of zeros)
+  const bool skip_b = std::is_same<T, float>::value && g1 == 0.0;
+  auto comb = [&](const double *xa, const double *xb, long long o) {
+    return skip_b ? g0 * xa[o] : g0 * xa[o] + g1 * xb[o];
+  };
   if (i <= g.ie && j <= g.je && k <= g.ke) {
     for (int n = 0; n < a.nvar; ++n) {
       const long long o = n * cs + c;
       double dv = (double(a.f1[o + 1]) - double(a.f1[o])) * idx1;
       if (g.two_d) dv += (double(a.f2[o + sj]) - double(a.f2[o])) * idx2;
       if (g.three_d) dv += (double(a.f3[o + sk]) - double(a.f3[o])) * idx3;
-      a.uo[o] = g0 * a.ua[o] + g1 * a.ub[o] - bdt * dv;
+      a.uo[o] = comb(a.ua, a.ub, o) - bdt * dv;
     }
   }
   if (j <= g.je && k <= g.ke) {   // B1 on x1-faces
     double cr = 0.0;
     if (g.two_d) cr += (double(a.e3[c + sj]) - double(a.e3[c])) * idx2;
     if (g.three_d) cr -= (double(a.e2[c + sk]) - double(a.e2[c])) * idx3;
-    a.b1o[c] = g0 * a.b1a[c] + g1 * a.b1b[c] - bdt * cr;
+    a.b1o[c] = comb(a.b1a, a.b1b, c) - bdt * cr;
   }
   if (i <= g.ie && k <= g.ke && (g.two_d || j == g.js)) {   // B2 on x2-faces
     double cr = -(double(a.e3[c + 1]) - double(a.e3[c])) * idx1;
     if (g.three_d) cr += (double(a.e1[c + sk]) - double(a.e1[c])) * idx3;
-    const double v = g0 * a.b2a[c] + g1 * a.b2b[c] - bdt * cr;
+    const double v = comb(a.b2a, a.b2b, c) - bdt * cr;
     a.b2o[c] = v;
     if (!g.two_d) a.b2o[c + sj] = v;
   }
   if (i <= g.ie && j <= g.je && (g.three_d || k == g.ks)) {   // B3 on x3-faces
     double cr = (double(a.e2[c + 1]) - double(a.e2[c])) * idx1;
     if (g.two_d) cr -= (double(a.e1[c + sj]) - double(a.e1[c])) * idx2;
-    const double v = g0 * a.b3a[c] + g1 * a.b3b[c] - bdt * cr;
+    const double v = comb(a.b3a, a.b3b, c) - bdt * cr;
     a.b3o[c] = v;
     if (!g.three_d) a.b3o[c + sk] = v;
   }
@@ -181,7 +191,9 @@ __global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
   const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
-  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+  const bool one_per_tile = int(gridDim.x) == v.n;   // normal launch: XCD-contiguous tiles
+  for (int b = blockIdx.x; b < v.n; b += gridDim.x) {
+    const int vb = one_per_tile ? xcd_tile(b, v.n) : b;
     int k, j, i;
     if (!v.at(vb, k, j, i)) continue;
     update_at(a, bdt, g.ks + k, g.js + j, g.is + i);
@@ -427,7 +439,9 @@ __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   double mdt = 1e300;
   int nbad_local = 0;
-  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+  const bool one_per_tile = int(gridDim.x) == v.n;   // normal launch: XCD-contiguous tiles
+  for (int b = blockIdx.x; b < v.n; b += gridDim.x) {
+    const int vb = one_per_tile ? xcd_tile(b, v.n) : b;
     int k, j, i;
     if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT>(a, k, j, i, mdt, nbad_local);
   }
