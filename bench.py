@@ -32,10 +32,11 @@ MODEL_BYTES_PER_ZONE_STAGE = 1040  # README.md:45-47 (130 doubles per cell per R
 # writes 5 fluxes and 2 face EMFs
 FLUX_BYTES_PER_FACE = (5 + 2 + 1 + 5 + 2) * 8
 
-# HBM bytes per flux-kernel launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (profiles/flux_blast256_pmc.md), scaled by the copy-kernel calibration factor stated there;
-# None until measured for the current kernel.
-FLUX_TRAFFIC_PER_LAUNCH = None
+# HBM bytes per flux-kernel launch (mean of the x, y and z launches), measured with rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE and scaled by the reading of a plain 2 GiB device copy
+# (profiles/blast256_fp64_hbm_traffic.md): read 1241 / 1837 / 1837 MiB, write 935 / 992 / 992
+# MiB per x / y / z launch.
+FLUX_TRAFFIC_PER_LAUNCH = (1241 + 935 + 2 * (1837 + 992)) / 3 * 2 ** 20
 
 DECK = """
 [mesh]
@@ -241,7 +242,7 @@ def serve_jobs(njobs, seed=11):
     return jobs
 
 
-def serve(njobs, rate_factor=4.0, seed=11, budget_gb=200.0, slots=64, chunk=4):
+def serve(njobs, rate_factor=8.0, seed=11, budget_gb=200.0, slots=64, chunk=8):
     """Sequential plain-FP64 runs (one after another, alone) against the continuous-batching
     server fed by seeded Poisson arrivals at `rate_factor` x the sequential completion rate,
     mixed precision with FP64 fallback. Returns runs/s of both and the server's p50/p99
@@ -293,6 +294,12 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched Orszag-Tang ensemble measured beside the headline (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=0, help="jobs of the served stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=8.0,
+                    help="Poisson arrival rate of the served stream, in multiples of the sequential FP64 rate")
+    ap.add_argument("--serve-chunk", type=int, default=8, help="steps per serving-loop iteration")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 2 GiB (5 times): the reference reading for the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=["batch", "serve"], default=None,
                     help="print only the batched-ensemble or served-stream measurement")
     args = ap.parse_args()
@@ -301,11 +308,20 @@ def main():
               "sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(2 ** 28, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "launches": 5}))
+        return
     if args.only == "batch":
         print(json.dumps({p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}))
         return
     if args.only == "serve":
-        print(json.dumps(serve(args.serve_jobs)))
+        print(json.dumps(serve(args.serve_jobs, rate_factor=args.serve_rate, chunk=args.serve_chunk)))
         return
     n = args.n
     ms, flux_ms, flux_launches, st, breakdown = run(n, "fp64", args.steps, args.warmup, breakdown=True)
@@ -353,7 +369,7 @@ def main():
     if args.ensemble_runs > 0:
         out["batch"] = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
     if args.serve_jobs > 0:
-        out["serve"] = serve(args.serve_jobs)
+        out["serve"] = serve(args.serve_jobs, rate_factor=args.serve_rate, chunk=args.serve_chunk)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     print(json.dumps(out))

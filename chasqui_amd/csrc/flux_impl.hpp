@@ -240,13 +240,25 @@ __device__ __forceinline__ void flux_m_tile(const FluxArgsT<T> &a, const int bx,
     }
   }
   T Lprev[NV];
+  // wide stencils: q_{c+2} of the next iteration is loaded one iteration ahead, so its latency
+  // overlaps the current cell's arithmetic (the same addresses as loading it in place)
+  T qpre[NV];
+  if constexpr (W) {
+    const long long cc = base + (long long)c * sd;
+#pragma unroll
+    for (int n = 0; n < NV; ++n) qpre[n] = src[n][cc + 2 * sd];
+  }
   for (; c <= F1; ++c) {
     // finish cell c: needs q_{c+2} (wide) or q_{c+1} (narrow, already in wp1)
     const long long cc = base + (long long)c * sd;
     T qn[NV];
     if constexpr (W) {
 #pragma unroll
-      for (int n = 0; n < NV; ++n) qn[n] = src[n][cc + 2 * sd];
+      for (int n = 0; n < NV; ++n) qn[n] = qpre[n];
+      if (c < F1) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) qpre[n] = src[n][cc + 3 * sd];
+      }
     } else {
 #pragma unroll
       for (int n = 0; n < NV; ++n) qn[n] = wp1[n];

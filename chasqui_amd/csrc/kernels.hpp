@@ -102,17 +102,6 @@ struct VB {
     return t < nx * ny;
   }
 };
-// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), and each
-// XCD has its own L2. For a launch of exactly one block per tile, this maps block b to a tile
-// so that every XCD walks one contiguous eighth of the tiles in order: the j and k neighbours a
-// stencil reads were (or are being) fetched into the same L2.
-constexpr int kXcds = 8;
-inline __host__ __device__ int xcd_tile(int b, int n) {
-  const int q = n / kXcds, r = n % kXcds;
-  const int x = b % kXcds, i = b / kXcds;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
 inline __host__ __device__ VB vgrid(int nx, int ny, int nz) {
   VB v;
   v.nx = nx;

@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <functional>
 #include <future>
 #include <map>
 #include <string>
@@ -34,13 +35,16 @@ double now_s() {
   return std::chrono::duration<double>(clk::now() - t0).count();
 }
 
-enum : int { JOB_QUEUED = 0, JOB_RUNNING = 1, JOB_DONE = 2, JOB_FAILED = 3, JOB_RELEASED = 4 };
+enum : int { JOB_QUEUED = 0, JOB_RUNNING = 1, JOB_DONE = 2, JOB_FAILED = 3, JOB_RELEASED = 4,
+             JOB_STARTING = 5 /* admitted, device setup on a worker thread; reported as running */ };
 
 struct Job {
   int id = 0;
   std::string text, overrides;   // kept to set the run up again (FP64 fallback)
   chq_deck *deck = nullptr;
   std::shared_future<int> prep;   // host problem setup running ahead of admission
+  std::shared_future<int> start;  // device setup (context, upload, con2prim) after admission
+  chq_ctx *started = nullptr;     // the context the start task built
   chq_mesh_desc mesh{};
   chq_config cfg{};
   double tlim = 0.0;
@@ -68,6 +72,8 @@ struct chq_server {
   std::map<chq_ctx *, int> running;   // context -> job
   std::vector<int> failed;            // runs that could not be created, not yet reported
   std::vector<chq_ctx *> to_free;     // released runs, destroyed when the device is idle
+  std::vector<int> starting;          // admitted runs whose device setup is in progress
+  std::map<Key, int> starting_per_key;
   bool inflight = false;              // a chunk launched by the last pump is running
   int next_id = 1;
   long long steps_issued = 0;
@@ -97,27 +103,73 @@ void prepare_ahead(chq_server *s, int ahead) {
   }
 }
 
+// Admission reserves the run's memory and slot and hands its device setup (context, upload of
+// the initial state, con2prim) to a worker thread on the context's own stream; the run joins
+// its batch at the first pump after the setup is done, so the host never waits for uploads
+// while the device steps.
 int admit_one(chq_server *s, Job &j) {
-  chq_batch *&b = s->batches[key_of(j.cfg)];
+  const Key key = key_of(j.cfg);
+  chq_batch *&b = s->batches[key];
   if (!b) {
-    if (int e = chq_batch_create(s->slots, &b)) { s->batches.erase(key_of(j.cfg)); return e; }
+    if (int e = chq_batch_create(s->slots, &b)) { s->batches.erase(key); return e; }
   }
-  if (chq_batch_size(b) >= s->slots) return 1;   // no free slot: stays queued
-  if (j.prep.valid()) j.prep.wait();             // the worker's setup must be complete
-  chq_ctx *c = nullptr;
-  if (int e = chq_create(&j.mesh, &j.cfg, &c)) return e;
-  int e = chq_deck_upload(j.deck, c);
-  if (e == CHQ_EINVAL) e = fail(CHQ_EINVAL, std::string("problem setup: ") + chq_deck_last_error());
-  if (!e) e = chq_init(c, 0.0, j.tlim, j.nlim);
-  if (!e) e = chq_batch_add(b, c);
-  if (e) { chq_destroy(c); return e; }
-  j.ctx = c;
-  j.state = JOB_RUNNING;
-  j.t_admit = now_s();
+  if (chq_batch_size(b) + s->starting_per_key[key] >= s->slots) return 1;   // no free slot
+  if (s->starting.size() >= 4) return 1;                                   // setups in flight
+  Job *jp = &j;   // map nodes do not move
+  std::shared_future<int> prep = j.prep;
+  j.start = std::async(std::launch::async, [jp, prep]() -> int {
+              if (prep.valid()) prep.wait();
+              chq_ctx *c = nullptr;
+              if (int e = chq_create(&jp->mesh, &jp->cfg, &c)) { jp->error = chq_last_error(); return e; }
+              int e = chq_deck_upload(jp->deck, c);
+              if (e == CHQ_EINVAL) jp->error = std::string("problem setup: ") + chq_deck_last_error();
+              else if (e) jp->error = chq_last_error();
+              if (!e && (e = chq_init(c, 0.0, jp->tlim, jp->nlim))) jp->error = chq_last_error();
+              if (e) { chq_destroy(c); return e; }
+              jp->started = c;
+              return CHQ_OK;
+            }).share();
+  j.state = JOB_STARTING;
   s->in_use += j.bytes;
-  s->running[c] = j.id;
-  chq_deck_close(j.deck);   // the initial state is on the device now
-  j.deck = nullptr;
+  s->starting.push_back(j.id);
+  s->starting_per_key[key] += 1;
+  return CHQ_OK;
+}
+
+// runs whose device setup finished join their batch (or fail); wait = block until all have
+int join_started(chq_server *s, bool wait, const std::function<void(int)> &report) {
+  for (auto it = s->starting.begin(); it != s->starting.end();) {
+    Job &j = s->jobs[*it];
+    if (!wait && j.start.wait_for(std::chrono::seconds(0)) != std::future_status::ready) { ++it; continue; }
+    const int e = j.start.get();
+    s->starting_per_key[key_of(j.cfg)] -= 1;
+    it = s->starting.erase(it);
+    if (j.deck) { chq_deck_close(j.deck); j.deck = nullptr; }   // the initial state is on the device
+    if (e) {
+      j.state = JOB_FAILED;
+      j.t_done = now_s();
+      s->in_use -= j.bytes;
+      report(j.id);
+      continue;
+    }
+    chq_batch *b = s->batches[key_of(j.cfg)];
+    if (int e2 = chq_batch_add(b, j.started)) {
+      chq_destroy(j.started);
+      j.started = nullptr;
+      j.state = JOB_FAILED;
+      j.error = chq_last_error();
+      j.t_done = now_s();
+      s->in_use -= j.bytes;
+      report(j.id);
+      (void)e2;
+      continue;
+    }
+    j.ctx = j.started;
+    j.started = nullptr;
+    j.state = JOB_RUNNING;
+    j.t_admit = now_s();
+    s->running[j.ctx] = j.id;
+  }
   return CHQ_OK;
 }
 
@@ -169,6 +221,10 @@ int chq_server_destroy(chq_server *s) {
   for (auto &kv : s->batches) chq_batch_destroy(kv.second);
   for (chq_ctx *c : s->to_free) chq_destroy(c);
   for (auto &kv : s->jobs) {
+    if (kv.second.start.valid()) {
+      kv.second.start.wait();
+      if (kv.second.started) chq_destroy(kv.second.started);
+    }
     if (kv.second.prep.valid()) kv.second.prep.wait();
     if (kv.second.ctx) chq_destroy(kv.second.ctx);
     if (kv.second.deck) chq_deck_close(kv.second.deck);
@@ -267,6 +323,7 @@ int chq_server_pump(chq_server *s, int max_done, int *done_ids, int *n_done) {
   }
   for (chq_ctx *c : s->to_free) chq_destroy(c);   // the device is idle here
   s->to_free.clear();
+  if (int e = join_started(s, false, report)) return e;   // finished setups join this chunk
   const double t1 = now_s();
   s->t_retire += t1 - t0;
   for (auto &kv : s->batches)
@@ -293,7 +350,7 @@ int chq_server_job(chq_server *s, int job_id, double info[10]) {
   const Job &j = it->second;
   chq_status st{};
   if (j.ctx && chq_get_status(j.ctx, &st)) return CHQ_EHIP;
-  info[0] = j.state;
+  info[0] = j.state == JOB_STARTING ? JOB_RUNNING : j.state;
   info[1] = j.cfg.precision;
   info[2] = j.fell_back;
   info[3] = st.cycle;
@@ -344,7 +401,7 @@ int chq_server_stats(chq_server *s, double stats[11]) {
   int q = 0, r = 0, d = 0;
   for (auto &kv : s->jobs) {
     q += kv.second.state == JOB_QUEUED;
-    r += kv.second.state == JOB_RUNNING;
+    r += kv.second.state == JOB_RUNNING || kv.second.state == JOB_STARTING;
     d += kv.second.state == JOB_DONE || kv.second.state == JOB_FAILED;
   }
   stats[0] = q;

@@ -98,9 +98,7 @@ template <class T>
 __global__ void __launch_bounds__(256) k_corner(const CornerArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
-  const bool one_per_tile = int(gridDim.x) == v.n;   // normal launch: XCD-contiguous tiles
-  for (int b = blockIdx.x; b < v.n; b += gridDim.x) {
-    const int vb = one_per_tile ? xcd_tile(b, v.n) : b;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
     int k, j, i;
     if (!v.at(vb, k, j, i)) continue;
     if (g.three_d) corner3_at(a, g.ks + k, g.js + j, g.is + i);
@@ -191,9 +189,7 @@ __global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
   const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
-  const bool one_per_tile = int(gridDim.x) == v.n;   // normal launch: XCD-contiguous tiles
-  for (int b = blockIdx.x; b < v.n; b += gridDim.x) {
-    const int vb = one_per_tile ? xcd_tile(b, v.n) : b;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
     int k, j, i;
     if (!v.at(vb, k, j, i)) continue;
     update_at(a, bdt, g.ks + k, g.js + j, g.is + i);
@@ -439,9 +435,7 @@ __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   double mdt = 1e300;
   int nbad_local = 0;
-  const bool one_per_tile = int(gridDim.x) == v.n;   // normal launch: XCD-contiguous tiles
-  for (int b = blockIdx.x; b < v.n; b += gridDim.x) {
-    const int vb = one_per_tile ? xcd_tile(b, v.n) : b;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
     int k, j, i;
     if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT>(a, k, j, i, mdt, nbad_local);
   }
