@@ -196,6 +196,132 @@ __global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB
   }
 }
 
+
+// ---------------------------------------------------------------- 3D k-marching forms
+// 3D corner EMF and update with the (j, i) plane in 256-point chunks and each thread marching
+// a few planes in k: the k-1 (corner) or k+1 (update) values a stencil needs were loaded by the
+// same thread in the previous iteration, so every input plane is read once per segment instead
+// of being fetched again by the blocks of the next plane. Same expressions, same order as
+// corner3_at / update_at.
+// planes per thread, measured at 256^3 (corner: 4 beats 8, 16, 32, 64; mixed update: 8)
+constexpr int KSEG_CORNER = 4, KSEG_UPDATE = 8;
+
+template <class T>
+__global__ void __launch_bounds__(256) k_corner_march(const CornerArgsT<T> a, const VB v, const int nk, const int kseg) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  int kk, jj, ii;
+  if (!v.at(int(blockIdx.x), kk, jj, ii)) return;   // v spans one plane
+  const int i = g.is + ii, j = g.js + jj;
+  const int k0 = g.ks + int(blockIdx.y) * kseg;
+  const int k1 = min(k0 + kseg - 1, g.ks + nk - 1);
+  const long long sj = g.pitch, sk = g.plane, cs = g.cstride;
+  const T *ex = a.ecc, *ey = a.ecc + cs, *ez = a.ecc + 2 * cs;
+  const bool do1 = i <= g.ie, do2 = j <= g.je;
+  // plane k-1 values carried through the march
+  T p_f2 = T(0), p_e1x2 = T(0), p_exm = T(0), p_ex = T(0);   // E1: f2, e1x2, ex(j-1), ex
+  T p_f1 = T(0), p_e2x1 = T(0), p_eym = T(0), p_ey = T(0);   // E2: f1, e2x1, ey(i-1), ey
+  {
+    const long long c = g.at(k0 - 1, j, i);
+    if (do1) { p_f2 = a.f2[c]; p_e1x2 = a.e1x2[c]; p_exm = ex[c - sj]; p_ex = ex[c]; }
+    if (do2) { p_f1 = a.f1[c]; p_e2x1 = a.e2x1[c]; p_eym = ey[c - 1]; p_ey = ey[c]; }
+  }
+  for (int k = k0; k <= k1; ++k) {
+    const long long c = g.at(k, j, i);
+    if (do1) {   // E1 at (k-1/2, j-1/2, i)
+      const T f2 = a.f2[c], e1x2 = a.e1x2[c], e1x3m = a.e1x3[c - sj], e1x3 = a.e1x3[c];
+      const T exm = ex[c - sj], exc = ex[c], f3m = a.f3[c - sj], f3 = a.f3[c];
+      const T l3 = upw(p_f2, e1x3m - p_exm, e1x3 - p_ex);
+      const T r3 = upw(f2, e1x3m - exm, e1x3 - exc);
+      const T l2 = upw(f3m, p_e1x2 - p_exm, e1x2 - exm);
+      const T r2 = upw(f3, p_e1x2 - p_ex, e1x2 - exc);
+      a.e1[c] = T(0.25) * (((l3 + r3) + (l2 + r2)) + ((p_e1x2 + e1x2) + (e1x3m + e1x3)));
+      p_f2 = f2; p_e1x2 = e1x2; p_exm = exm; p_ex = exc;
+    }
+    if (do2) {   // E2 at (k-1/2, j, i-1/2)
+      const T f1 = a.f1[c], e2x1 = a.e2x1[c], e2x3m = a.e2x3[c - 1], e2x3 = a.e2x3[c];
+      const T eym = ey[c - 1], eyc = ey[c], f3m = a.f3[c - 1], f3 = a.f3[c];
+      const T l3 = upw(p_f1, e2x3m - p_eym, e2x3 - p_ey);
+      const T r3 = upw(f1, e2x3m - eym, e2x3 - eyc);
+      const T l1 = upw(f3m, p_e2x1 - p_eym, e2x1 - eym);
+      const T r1 = upw(f3, p_e2x1 - p_ey, e2x1 - eyc);
+      a.e2[c] = T(0.25) * (((l3 + r3) + (l1 + r1)) + ((e2x3m + e2x3) + (p_e2x1 + e2x1)));
+      p_f1 = f1; p_e2x1 = e2x1; p_eym = eym; p_ey = eyc;
+    }
+    if (k <= g.ke) {   // E3 at (k, j-1/2, i-1/2): this plane only
+      const T l2 = upw(a.f1[c - sj], a.e3x2[c - 1] - ez[c - sj - 1], a.e3x2[c] - ez[c - sj]);
+      const T r2 = upw(a.f1[c], a.e3x2[c - 1] - ez[c - 1], a.e3x2[c] - ez[c]);
+      const T l1 = upw(a.f2[c - 1], a.e3x1[c - sj] - ez[c - sj - 1], a.e3x1[c] - ez[c - 1]);
+      const T r1 = upw(a.f2[c], a.e3x1[c - sj] - ez[c - sj], a.e3x1[c] - ez[c]);
+      a.e3[c] = T(0.25) * (((l2 + r2) + (l1 + r1)) + ((a.e3x2[c - 1] + a.e3x2[c]) + (a.e3x1[c - sj] + a.e3x1[c])));
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_update_march(const UpdateArgsT<T> a, const VB v, const int nk, const int kseg) {
+  if (!gate(a.ctrl, a.only_if_bad)) return;
+  const Geom &g = a.g;
+  int kk, jj, ii;
+  if (!v.at(int(blockIdx.x), kk, jj, ii)) return;
+  const int i = g.is + ii, j = g.js + jj;
+  const int k0 = g.ks + int(blockIdx.y) * kseg;
+  const int k1 = min(k0 + kseg - 1, g.ks + nk - 1);
+  const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
+  const double g0 = a.g0, g1 = a.g1;
+  const long long sj = g.pitch, sk = g.plane, cs = g.cstride;
+  const double idx1 = g.idx1, idx2 = g.idx2, idx3 = g.idx3;
+  const bool skip_b = std::is_same<T, float>::value && g1 == 0.0;
+  auto comb = [&](const double *xa, const double *xb, long long o) {
+    return skip_b ? g0 * xa[o] : g0 * xa[o] + g1 * xb[o];
+  };
+  const bool cell_ij = i <= g.ie && j <= g.je;
+  // plane k values of f3 and of the edge EMFs e1, e2, loaded as the previous iteration's k+1
+  T f3c[NVAR];
+  T e1c = T(0), e2c = T(0);
+  {
+    const long long c = g.at(k0, j, i);
+    if (cell_ij && k0 <= g.ke)
+      for (int n = 0; n < a.nvar; ++n) f3c[n] = a.f3[n * cs + c];
+    e1c = a.e1[c];
+    e2c = a.e2[c];
+  }
+  for (int k = k0; k <= k1; ++k) {
+    const long long c = g.at(k, j, i);
+    T e1n = T(0), e2n = T(0);
+    if (k <= g.ke) { e1n = a.e1[c + sk]; e2n = a.e2[c + sk]; }
+    if (cell_ij && k <= g.ke) {
+      for (int n = 0; n < a.nvar; ++n) {
+        const long long o = n * cs + c;
+        const T f3n = a.f3[o + sk];
+        double dv = (double(a.f1[o + 1]) - double(a.f1[o])) * idx1;
+        dv += (double(a.f2[o + sj]) - double(a.f2[o])) * idx2;
+        dv += (double(f3n) - double(f3c[n])) * idx3;
+        a.uo[o] = comb(a.ua, a.ub, o) - bdt * dv;
+        f3c[n] = f3n;
+      }
+    }
+    if (j <= g.je && k <= g.ke) {   // B1 on x1-faces
+      double cr = 0.0;
+      cr += (double(a.e3[c + sj]) - double(a.e3[c])) * idx2;
+      cr -= (double(e2n) - double(e2c)) * idx3;
+      a.b1o[c] = comb(a.b1a, a.b1b, c) - bdt * cr;
+    }
+    if (i <= g.ie && k <= g.ke) {   // B2 on x2-faces
+      double cr = -(double(a.e3[c + 1]) - double(a.e3[c])) * idx1;
+      cr += (double(e1n) - double(e1c)) * idx3;
+      a.b2o[c] = comb(a.b2a, a.b2b, c) - bdt * cr;
+    }
+    if (i <= g.ie && j <= g.je) {   // B3 on x3-faces
+      double cr = (double(a.e2[c + 1]) - double(e2c)) * idx1;
+      cr -= (double(a.e1[c + sj]) - double(e1c)) * idx2;
+      a.b3o[c] = comb(a.b3a, a.b3b, c) - bdt * cr;
+    }
+    e1c = e1n;
+    e2c = e2n;
+  }
+}
+
 // ---------------------------------------------------------------- boundaries
 // physics.hpp:354-432. x1: threads over (k, j, g) with g fastest; x2: (k, g, i); x3: (g, j, i).
 __device__ __forceinline__ void bc1_at(const BcArgs &a, const long long t) {
@@ -738,6 +864,13 @@ int grid_of(const VB &v, bool conditional) { return conditional ? (v.n < COND_BL
 template <class T>
 hipError_t launch_corner(const CornerArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
+  if (g.three_d && !a.only_if_bad) {   // union box [ks..ke+1] in KSEG-plane segments
+    const int nk = g.ke - g.ks + 2;
+    const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, 1);
+    hipLaunchKernelGGL(k_corner_march<T>, dim3(v.n, div_up(nk, KSEG_CORNER)), dim3(256), 0, s, a, v, nk,
+                       KSEG_CORNER);
+    return hipGetLastError();
+  }
   const VB v = corner_vb(g);
   hipLaunchKernelGGL(k_corner<T>, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
   return hipGetLastError();
@@ -752,6 +885,15 @@ hipError_t launch_corner_list(const CornerArgsT<T> &a, const int *list, hipStrea
 template <class T>
 hipError_t launch_update(const UpdateArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
+  // the k march pays off for the float field set; with FP64 fluxes the plane-ordered tiles of
+  // k_update stream HBM better (measured at 256^3: 3.34 vs 3.79 ms per step)
+  if (g.three_d && !a.only_if_bad && std::is_same<T, float>::value) {
+    const int nk = g.ke - g.ks + 2;
+    const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, 1);
+    hipLaunchKernelGGL(k_update_march<T>, dim3(v.n, div_up(nk, KSEG_UPDATE)), dim3(256), 0, s, a, v, nk,
+                       KSEG_UPDATE);
+    return hipGetLastError();
+  }
   const VB v = update_vb(g);
   hipLaunchKernelGGL(k_update<T>, dim3(grid_of(v, a.only_if_bad)), dim3(256), 0, s, a, v);
   return hipGetLastError();
