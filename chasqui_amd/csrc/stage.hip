@@ -887,7 +887,7 @@ hipError_t launch_update(const UpdateArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
   // the k march pays off for the float field set; with FP64 fluxes the plane-ordered tiles of
   // k_update stream HBM better (measured at 256^3: 3.34 vs 3.79 ms per step)
-  if (g.three_d && !a.only_if_bad && std::is_same<T, float>::value) {
+  if (g.three_d && std::is_same<T, float>::value) {   // also the fofc re-run (gated in the kernel)
     const int nk = g.ke - g.ks + 2;
     const VB v = vgrid(g.ie - g.is + 2, g.je - g.js + 2, 1);
     hipLaunchKernelGGL(k_update_march<T>, dim3(v.n, div_up(nk, KSEG_UPDATE)), dim3(256), 0, s, a, v, nk,
