@@ -27,10 +27,11 @@ sys.path.insert(0, HERE)
 METRIC = "zone-updates/s at 256³ FP64 & % HBM roofline; served runs/s + p99 latency"
 HBM_PEAK = 8.0e12                 # MI355X HBM3E, bytes/s
 MODEL_BYTES_PER_ZONE_STAGE = 1040  # README.md:45-47 (130 doubles per cell per RK stage)
-# algorithmic bytes per face of the FP64 flux kernel with the entropy slot not transported:
-# reads rho, v(3), p, B_t1, B_t2 (stencil elements counted once) and the face-normal B,
-# writes 5 fluxes and 2 face EMFs
-FLUX_BYTES_PER_FACE = (5 + 2 + 1 + 5 + 2) * 8
+# Algorithmic bytes of one flux launch (one direction): the reference's face-flux traffic, 52.0
+# doubles per active cell per RK stage over the three directions (reads of the 6 primitives
+# and 2 cell-B stencils plus the face-normal B, writes of 6 fluxes and 2 face EMFs), i.e.
+# 52.0 / 3 doubles per active cell per launch.
+FLUX_DOUBLES_PER_CELL_STAGE = 52.0
 
 # HBM bytes per flux-kernel launch (mean of the x, y and z launches), measured with rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE and scaled by the reading of a plain 2 GiB device copy
@@ -73,24 +74,30 @@ seed = 42
 """
 
 
-def faces_per_flux_launch(n):
-    return (n + 1) * (n + 2) * (n + 2)
-
-
 def cpu_baseline(n, steps):
-    """The CPU oracle (plain C++/OpenMP restatement of the reference) on this host."""
+    """The CPU oracle (plain C++/OpenMP restatement of the reference) on this host's cores, on
+    a bounded sample of the same workload (`steps` RK3 steps of the same 256^3 blast)."""
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "threads")
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     po.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     o = po.OracleSim(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}"])
     t0 = time.perf_counter()
     taken = o.step(steps)
     wall = time.perf_counter() - t0
     o.close()
     return {"value": n ** 3 * taken / wall, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{taken} RK3 step(s) of the same {n}^3 blast on {threads} OpenMP threads, "
-                      f"{wall:.1f} s wall (oracle/, -O3 -ffp-contract=off)"}
+            "sample": f"{taken} RK3 steps of the same {n}^3 blast on {threads} OpenMP threads "
+                      f"({os.cpu_count()} logical CPUs, {model}), {wall:.1f} s wall "
+                      f"(oracle/, -O3 -ffp-contract=off, OMP_PROC_BIND=spread)"}
 
 
 def run(n, precision, steps, warmup, breakdown=False):
@@ -288,7 +295,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="cells per side (256 = the headline workload)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=6, help="RK3 steps of the CPU baseline sample (~10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision side measurement")
     ap.add_argument("--ensemble-runs", type=int, default=64,
@@ -328,12 +335,12 @@ def main():
     zone_updates = float(n) ** 3 * args.steps
     value = zone_updates / (ms * 1e-3)
     avg_flux_s = flux_ms * 1e-3 / max(flux_launches, 1)
-    flux_bytes = FLUX_BYTES_PER_FACE * faces_per_flux_launch(n)
+    flux_bytes = FLUX_DOUBLES_PER_CELL_STAGE / 3 * 8 * float(n) ** 3
     achieved = flux_bytes / avg_flux_s
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK,
                 "traffic": FLUX_TRAFFIC_PER_LAUNCH,
-                "kernel": "k_flux (face flux, PPM + flattening + HLLD, one direction per launch)",
+                "kernel": "k_flux_x / k_flux_m (face flux, PPM + flattening + HLLD, one direction per launch)",
                 "bytes_per_launch": flux_bytes, "avg_launch_ms": avg_flux_s * 1e3, "launches": flux_launches}
     model = value * 3 * MODEL_BYTES_PER_ZONE_STAGE
     out = {

@@ -85,6 +85,9 @@ def lib():
         "chq_stage": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_long)]),
         "chq_init": (C.c_int, [vp, C.c_double, C.c_double, C.c_int]),
+        "chq_init_resume": (C.c_int, [vp, C.c_double, C.c_int, C.c_double, C.c_int]),
+        "chq_restart_write": (C.c_int, [vp, C.c_char_p, C.c_double, C.c_int]),
+        "chq_restart_read": (C.c_int, [vp, C.c_char_p, dp, ip]),
         "chq_step": (C.c_int, [vp, C.c_int]),
         "chq_step_timed": (C.c_int, [vp, C.c_int, dp]),
         "chq_get_status": (C.c_int, [vp, C.POINTER(Status)]),
@@ -149,7 +152,7 @@ def exported_symbols():
         "chq_get_status", "chq_sync", "chq_stream", "chq_kernel_timing", "chq_kernel_timing_read",
         "chq_deck_open", "chq_deck_open_lazy", "chq_deck_prepare", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
         "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",
-        "chq_write_vtk", "chq_history",
+        "chq_write_vtk", "chq_history", "chq_init_resume", "chq_restart_write", "chq_restart_read",
         "chq_batch_create", "chq_batch_destroy", "chq_batch_add", "chq_batch_remove", "chq_batch_size",
         "chq_batch_run", "chq_batch_step", "chq_batch_step_timed", "chq_batch_sync", "chq_batch_unfinished",
         "chq_batch_use_graph", "chq_batch_tune", "chq_batch_grid", "chq_batch_set_grid",

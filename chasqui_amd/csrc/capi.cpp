@@ -412,7 +412,9 @@ int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt
 }
 
 // ---------------------------------------------------------------- time loop
-int chq_init(chq_ctx *c, double t, double tlim, int nlim) {
+int chq_init(chq_ctx *c, double t, double tlim, int nlim) { return chq_init_resume(c, t, 0, tlim, nlim); }
+
+int chq_init_resume(chq_ctx *c, double t, int cycle, double tlim, int nlim) {
   if (!c) return fail(CHQ_EINVAL, "null context");
   if (int rc_ = sync_wcur(c)) return rc_;
   HIPCHK(c->boundaries(0, 6, c->ctrl_api, 0));
@@ -426,6 +428,7 @@ int chq_init(chq_ctx *c, double t, double tlim, int nlim) {
   h.cfl = c->cfg.cfl;
   h.min_dt = std::fmin(mdt, 1e300);   // main.cpp:201-202
   h.active = 0;
+  h.cycle = cycle;
   h.wcur = c->wcur;
   HIPCHK(hipMemcpyAsync(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -1,10 +1,11 @@
 // chasqui_amd — command-line driver of the MI355X stage pipeline (restates main.cpp:97-329
 // of the reference over the C ABI):
 //
-//   chasqui_amd -i input.in [block.key=value ...]
+//   chasqui_amd -i input.in [-r restart.rst] [block.key=value ...]
 //
 // Same deck syntax and overrides, same stdout summary (parameter echo, cycle lines every
-// ndiag, zone-cycles/second, fofc count, L1 errors), same .hst / .vtk / _errors.dat files.
+// ndiag, zone-cycles/second, fofc count, L1 errors), same .hst / .vtk / .rst / _errors.dat
+// files, restart from a .rst dump.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -28,14 +29,14 @@ int die(const std::string &msg) {
 }  // namespace
 
 int main(int argc, char *argv[]) {
-  std::string input, overrides;
+  std::string input, overrides, rstfile;
   for (int a = 1; a < argc; ++a) {
     if (std::strcmp(argv[a], "-i") == 0 && a + 1 < argc) { input = argv[++a]; continue; }
-    if (std::strcmp(argv[a], "-r") == 0) return die("restart files (-r) are not supported");
+    if (std::strcmp(argv[a], "-r") == 0 && a + 1 < argc) { rstfile = argv[++a]; continue; }
     if (std::strchr(argv[a], '=')) overrides += std::string(argv[a]) + "\n";
   }
   if (input.empty()) {
-    std::cerr << "usage: chasqui_amd -i input.in [block.key=value ...]\n";
+    std::cerr << "usage: chasqui_amd -i input.in [-r restart.rst] [block.key=value ...]\n";
     return 1;
   }
   std::ifstream fin(input);
@@ -55,6 +56,7 @@ int main(int argc, char *argv[]) {
   const std::string basename = chq_deck_get(deck, "output", "basename");
   const double dt_vtk = std::stod(chq_deck_get(deck, "output", "vtk_dt"));
   const double dt_hst = std::stod(chq_deck_get(deck, "output", "hst_dt"));
+  const double dt_rst = std::stod(chq_deck_get(deck, "output", "rst_dt"));
   const std::string ce = chq_deck_get(deck, "output", "compute_error");
   const bool compute_error = ce == "true" || ce == "1" || ce == "yes";
 
@@ -68,9 +70,18 @@ int main(int argc, char *argv[]) {
     return die("no HIP device");
   std::cout << "# execution space: HIP " << prop.gcnArchName << " (" << prop.name << ")\n";
 
+  // initial conditions or restart, main.cpp:191-201
+  double t = 0.0;
+  int cycle = 0;
   chq_ctx *ctx = nullptr;
   if (chq_create(&mesh, &cfg, &ctx)) return die(chq_last_error());
-  if (chq_deck_upload(deck, ctx) || chq_init(ctx, 0.0, tlim, nlim)) return die(chq_last_error());
+  if (rstfile.empty()) {
+    if (chq_deck_upload(deck, ctx)) return die(chq_last_error());
+  } else {
+    if (chq_restart_read(ctx, rstfile.c_str(), &t, &cycle)) return die(chq_last_error());
+    std::cout << "# restarted from " << rstfile << " at t = " << t << ", cycle = " << cycle << "\n";
+  }
+  if (chq_init_resume(ctx, t, cycle, tlim, nlim)) return die(chq_last_error());
   chq_deck_close(deck);
 
   std::vector<double> u_init, b_init;
@@ -82,15 +93,14 @@ int main(int argc, char *argv[]) {
   }
 
   // output scheduling, main.cpp:211-239
-  double t = 0.0;
-  int cycle = 0, nvtk = 0;
-  double next_vtk = t, next_hst = t;
+  int nvtk = 0, nrst = 0;
+  double next_vtk = t, next_hst = t, next_rst = t + dt_rst;
   std::ofstream hst;
   if (dt_hst > 0) {
-    hst.open(basename + ".hst");
-    hst << "# time dt mass mom1 mom2 mom3 etot ekin emag eint divB_max divB_L1\n";
+    hst.open(basename + ".hst", rstfile.empty() ? std::ios::out : std::ios::app);
+    if (rstfile.empty()) hst << "# time dt mass mom1 mom2 mom3 etot ekin emag eint divB_max divB_L1\n";
   }
-  auto do_outputs = [&](double dt, bool force) {
+  auto do_outputs = [&](double dt, bool force, bool final = false) {
     const double eps = 1e-10 * (tlim > 0 ? tlim : 1.0);
     if (dt_vtk > 0 && (t >= next_vtk - eps || force)) {
       char fn[512];
@@ -107,12 +117,19 @@ int main(int argc, char *argv[]) {
       hst.flush();
       next_hst += dt_hst;
     }
+    if (dt_rst > 0 && (t >= next_rst - eps || final)) {
+      char fn[512];
+      std::snprintf(fn, sizeof(fn), "%s.%04d.rst", basename.c_str(), nrst++);
+      if (chq_restart_write(ctx, fn, t, cycle)) std::cerr << "chasqui_amd: " << chq_last_error() << "\n";
+      next_rst += dt_rst;
+    }
   };
-  do_outputs(0.0, true);
+  if (rstfile.empty()) do_outputs(0.0, true);
 
   // time loop, main.cpp:241-305. Without per-step outputs or diagnostics the steps are
   // enqueued in chunks and the device decides when t reaches tlim.
-  const bool per_step = dt_vtk > 0 || dt_hst > 0 || ndiag > 0;
+  const int cycle0 = cycle;
+  const bool per_step = dt_vtk > 0 || dt_hst > 0 || dt_rst > 0 || ndiag > 0;
   chq_sync(ctx);
   const auto t0 = std::chrono::steady_clock::now();
   chq_status st{};
@@ -131,11 +148,11 @@ int main(int argc, char *argv[]) {
   }
   chq_sync(ctx);
   const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (dt_vtk > 0 || dt_hst > 0) do_outputs(0.0, true);
+  if (dt_vtk > 0 || dt_hst > 0 || dt_rst > 0) do_outputs(0.0, true, true);
 
   const double zones = double(mesh.nx1) * mesh.nx2 * mesh.nx3;
   std::cout << "# cycles = " << cycle << ", time = " << t << ", wall = " << wall << " s\n";
-  std::cout << "# zone-cycles/second = " << std::scientific << zones * cycle / wall << "\n";
+  std::cout << "# zone-cycles/second = " << std::scientific << zones * (cycle - cycle0) / wall << "\n";
   if (cfg.fofc) std::cout << "# first-order flux corrections applied: " << st.fofc_total << "\n";
 
   if (compute_error) {   // main.cpp:310-327

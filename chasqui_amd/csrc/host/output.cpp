@@ -9,6 +9,10 @@
 
 #include "../../../include/chasqui_amd.h"
 
+namespace chq {
+int fail(int code, const std::string &msg);   // chq_last_error() message (capi.cpp)
+}
+
 namespace {
 
 void put_be_floats(std::ofstream &f, const std::vector<float> &v) {
@@ -39,6 +43,18 @@ Geo geo_of(chq_ctx *ctx) {
   chq_geometry(ctx, iv, nullptr);
   return Geo{iv[0], iv[1], iv[2], iv[4], iv[5], iv[6], iv[7], iv[9], iv[11], iv[13], iv[14]};
 }
+
+// the restart header of restart.hpp:11-17, field for field (a file written by either code is
+// read by the other)
+struct RstHeader {
+  int32_t magic;
+  int32_t nx1, nx2, nx3, ng;
+  int32_t cycle;
+  double time;
+  int32_t real_size;
+};
+constexpr int32_t kRstMagic = 0x43485351;   // "CHSQ"
+constexpr int kRstArrays[4] = {CHQ_U0, CHQ_B1F0, CHQ_B2F0, CHQ_B3F0};
 
 }  // namespace
 
@@ -97,7 +113,7 @@ int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, dou
   dump("Bx1", b, 0);
   dump("Bx2", b, 1);
   dump("Bx3", b, 2);
-  return f ? CHQ_OK : CHQ_EINVAL;
+  return f ? CHQ_OK : chq::fail(CHQ_EINVAL, std::string("write failed: ") + path);
 }
 
 // History (output.hpp:91-119): mass, mom1..3, etot, ekin, emag, eint (volume integrals over
@@ -144,6 +160,55 @@ int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]
   for (int n = 0; n < 8; ++n) out[n] = s[n] * vol;
   out[8] = dbmax;
   out[9] = dbl1 / (double(g.nx1) * g.nx2 * g.nx3);
+  return CHQ_OK;
+}
+
+// restart::Write (restart.hpp:19-34): header, then u0 and b1f0..b3f0 in the reference layout
+int chq_restart_write(chq_ctx *ctx, const char *path, double time, int cycle) {
+  if (!ctx || !path) return CHQ_EINVAL;
+  int iv[16];
+  if (int e = chq_geometry(ctx, iv, nullptr)) return e;
+  RstHeader h;
+  std::memset(&h, 0, sizeof(h));
+  h.magic = kRstMagic;
+  h.nx1 = iv[0]; h.nx2 = iv[1]; h.nx3 = iv[2]; h.ng = iv[3];
+  h.cycle = cycle;
+  h.time = time;
+  h.real_size = int32_t(sizeof(double));
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return chq::fail(CHQ_EINVAL, std::string("cannot write restart file ") + path);
+  f.write(reinterpret_cast<const char *>(&h), sizeof(h));
+  std::vector<double> buf;
+  for (int a : kRstArrays) {
+    buf.resize(chq_array_count(ctx, a));
+    if (int e = chq_download(ctx, a, buf.data(), buf.size())) return e;
+    f.write(reinterpret_cast<const char *>(buf.data()), std::streamsize(buf.size() * sizeof(double)));
+  }
+  return f ? CHQ_OK : chq::fail(CHQ_EINVAL, std::string("write failed: ") + path);
+}
+
+// restart::Read (restart.hpp:37-58) into u0 and b1f0..b3f0; the caller then runs chq_init_resume
+// (boundaries + con2prim, main.cpp:196-201)
+int chq_restart_read(chq_ctx *ctx, const char *path, double *time, int *cycle) {
+  if (!ctx || !path) return CHQ_EINVAL;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return chq::fail(CHQ_EINVAL, std::string("cannot open restart file ") + path);
+  RstHeader h;
+  f.read(reinterpret_cast<char *>(&h), sizeof(h));
+  int iv[16];
+  if (int e = chq_geometry(ctx, iv, nullptr)) return e;
+  if (!f || h.magic != kRstMagic) return chq::fail(CHQ_EINVAL, std::string("not a Chasqui restart file: ") + path);
+  if (h.nx1 != iv[0] || h.nx2 != iv[1] || h.nx3 != iv[2] || h.ng != iv[3] || h.real_size != int32_t(sizeof(double)))
+    return chq::fail(CHQ_EINVAL, "restart file does not match the mesh (nx, nghost or precision differ)");
+  std::vector<double> buf;
+  for (int a : kRstArrays) {
+    buf.resize(chq_array_count(ctx, a));
+    f.read(reinterpret_cast<char *>(buf.data()), std::streamsize(buf.size() * sizeof(double)));
+    if (!f) return chq::fail(CHQ_EINVAL, std::string("restart file is truncated: ") + path);
+    if (int e = chq_upload(ctx, a, buf.data(), buf.size())) return e;
+  }
+  if (time) *time = h.time;
+  if (cycle) *cycle = h.cycle;
   return CHQ_OK;
 }
 

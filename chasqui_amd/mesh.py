@@ -138,8 +138,18 @@ class Mesh:
         return n.value
 
     # ---- device-resident time loop
-    def init(self, t=0.0, tlim=1.0, nlim=-1):
-        check(self.L.chq_init(self.h, t, tlim, nlim), "chq_init")
+    def init(self, t=0.0, tlim=1.0, nlim=-1, cycle=0):
+        check(self.L.chq_init_resume(self.h, t, int(cycle), tlim, nlim), "chq_init")
+
+    def restart_write(self, path, t, cycle):
+        """The reference's binary restart dump (restart.hpp) of u0 and the face fields."""
+        check(self.L.chq_restart_write(self.h, str(path).encode(), t, int(cycle)), "chq_restart_write")
+
+    def restart_read(self, path):
+        """Load a restart dump into u0 / b?f0; returns (t, cycle) for init(t, tlim, nlim, cycle)."""
+        t, cyc = C.c_double(), C.c_int()
+        check(self.L.chq_restart_read(self.h, str(path).encode(), C.byref(t), C.byref(cyc)), "chq_restart_read")
+        return t.value, cyc.value
 
     def step(self, n=1):
         check(self.L.chq_step(self.h, int(n)), "chq_step")

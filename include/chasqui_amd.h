@@ -118,6 +118,8 @@ int chq_stage(chq_ctx *ctx, int cur, int out, double g0, double g1, double beta_
 /* ---- the time loop with dt, t and the fofc count resident on the device ---- */
 /* main.cpp:192-201: boundaries(u0) + cons_to_prim(u0, dt) and sets t, tlim, nlim. */
 int chq_init(chq_ctx *ctx, double t, double tlim, int nlim);
+/* chq_init from a restart: the loop continues at (t, cycle) (main.cpp:196-201) */
+int chq_init_resume(chq_ctx *ctx, double t, int cycle, double tlim, int nlim);
 /* enqueue `nsteps` RK steps (main.cpp:246-292); asynchronous, no host round trip. Steps
  * after t reaches tlim (or cycle reaches nlim) are no-ops on the device. */
 int chq_step(chq_ctx *ctx, int nsteps);
@@ -170,6 +172,11 @@ int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, dou
 /* output::History (output.hpp:91-119) of set `buf`: out[10] = mass mom1 mom2 mom3 etot ekin
  * emag eint divB_max divB_L1 */
 int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]);
+/* restart::Write / restart::Read (restart.hpp:19-58): the reference's binary restart format
+ * (header + u0 + b1f0..b3f0, FP64), readable by either code. After chq_restart_read, call
+ * chq_init_resume with the returned time and cycle. */
+int chq_restart_write(chq_ctx *ctx, const char *path, double time, int cycle);
+int chq_restart_read(chq_ctx *ctx, const char *path, double *time, int *cycle);
 
 /* ---- batches: many independent runs advanced by shared launches (no counterpart in the
  * reference, which advances one mesh with a host-driven loop, main.cpp:246-297) ----

@@ -20,6 +20,7 @@
 #include <map>
 #include <sstream>
 #include <stdexcept>
+#include <fstream>
 #include <string>
 #include <vector>
 #include <algorithm>
@@ -944,6 +945,43 @@ struct Sim {
     ++cycle;
     last_dt = dt;
     return true;
+  }
+
+  // binary restart dump (restart.hpp:11-34): header, then u0 and the three face fields of
+  // set 0 in the reference layout
+  struct RstHeader {
+    int magic = 0x43485351;   // "CHSQ"
+    int nx1 = 0, nx2 = 0, nx3 = 0, ng = 0;
+    int cycle = 0;
+    double time = 0.0;
+    int real_size = 0;
+  };
+  void restart_write(const std::string &path) const {
+    RstHeader h;
+    h.nx1 = m.nx1; h.nx2 = m.nx2; h.nx3 = m.nx3; h.ng = m.ng; h.cycle = cycle; h.time = t;
+    h.real_size = int(sizeof(Real));
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + path);
+    f.write(reinterpret_cast<const char *>(&h), sizeof(h));
+    for (const auto *v : {&m.u0.v, &m.b1f0.v, &m.b2f0.v, &m.b3f0.v})
+      f.write(reinterpret_cast<const char *>(v->data()), std::streamsize(v->size() * sizeof(Real)));
+  }
+  // restart.hpp:37-58, then the driver's boundaries + con2prim (main.cpp:196-201)
+  void restart_read(const std::string &path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open restart file " + path);
+    RstHeader h;
+    f.read(reinterpret_cast<char *>(&h), sizeof(h));
+    if (h.magic != 0x43485351) throw std::runtime_error("not a Chasqui restart file: " + path);
+    if (h.nx1 != m.nx1 || h.nx2 != m.nx2 || h.nx3 != m.nx3 || h.ng != m.ng || h.real_size != int(sizeof(Real)))
+      throw std::runtime_error("restart file does not match the mesh");
+    for (auto *v : {&m.u0.v, &m.b1f0.v, &m.b2f0.v, &m.b3f0.v})
+      f.read(reinterpret_cast<char *>(v->data()), std::streamsize(v->size() * sizeof(Real)));
+    if (!f) throw std::runtime_error("restart file is truncated: " + path);
+    t = h.time;
+    cycle = h.cycle;
+    boundaries(m, cfg, 0);
+    min_dt = cons_to_prim(m, cfg, 0, true);
   }
 
   // L1 error vs the initial state over the 9 fields (main.cpp:310-327)

@@ -95,6 +95,15 @@ int orc_step(void *p, int n) {
 }
 
 void orc_l1_error(void *p, double *err10) { S(p)->l1_error(err10); }
+
+int orc_restart_write(void *p, const char *path) {
+  try { S(p)->restart_write(path); } catch (const std::exception &e) { g_err = e.what(); return -1; }
+  return 0;
+}
+int orc_restart_read(void *p, const char *path) {
+  try { S(p)->restart_read(path); } catch (const std::exception &e) { g_err = e.what(); return -1; }
+  return 0;
+}
 long orc_fofc_count(void *p) { return S(p)->fofc_count; }
 
 // ---- single kernels, acting on the mesh state exactly as the reference kernels do

@@ -46,6 +46,8 @@ def lib():
     L.orc_flag.argtypes = [vp, lp]
     L.orc_step.argtypes = [vp, C.c_int]
     L.orc_l1_error.argtypes = [vp, dp]
+    L.orc_restart_write.argtypes = [vp, C.c_char_p]
+    L.orc_restart_read.argtypes = [vp, C.c_char_p]
     L.orc_fofc_count.restype = C.c_long
     L.orc_fofc_count.argtypes = [vp]
     L.orc_fluxes.argtypes = [vp, C.c_int, C.c_int]
@@ -174,6 +176,15 @@ class OracleSim:
 
     def fofc_count(self):
         return self.L.orc_fofc_count(self.h)
+
+    def restart_write(self, path):
+        if self.L.orc_restart_write(self.h, str(path).encode()):
+            raise RuntimeError(self.L.orc_last_error().decode())
+
+    def restart_read(self, path):
+        """Restart dump into this (deck-configured) simulation, then boundaries + con2prim."""
+        if self.L.orc_restart_read(self.h, str(path).encode()):
+            raise RuntimeError(self.L.orc_last_error().decode())
 
     # single kernels
     def fluxes(self, d, cur):

@@ -1,0 +1,136 @@
+"""Restart dumps in the reference's binary format (restart.hpp: header + u0 + b1f0..b3f0).
+
+The reference's own check (tests/physics.sh:58-59, 93-95) runs Orszag-Tang to t = 0.2 writing
+a dump every 0.1, resumes from the first dump, and requires the final state to be bitwise the
+uninterrupted one. Here: the oracle's restatement on the CPU, the device pipeline, files
+crossing between the two in both directions, and the driver executable.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import DECKS, deck
+
+po = pytest.importorskip("pyoracle")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "chasqui_amd", "_lib", "chasqui_amd")
+
+OT = ("orszag_tang.in", ["mesh.nx1=32", "mesh.nx2=24"])
+BLAST = ("blast_3d.in", ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12", "mhd.pfloor=0.1"])   # fofc fires
+
+
+def _state(o):
+    return {n: o.get(n) for n in ("u0", "b1f0", "b2f0", "b3f0", "w", "bcc")}
+
+
+def _same(a, b, label):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), f"{label}: {k}"
+
+
+@pytest.mark.parametrize("name,ov", [OT, BLAST])
+def test_oracle_restart_is_bitwise(tmp_path, name, ov):
+    """Oracle: N steps, dump, resume in a fresh simulation, M steps == N + M steps."""
+    text = deck(name)
+    full = po.OracleSim(text, ov)
+    full.step(7)
+    ref = _state(full)
+    a = po.OracleSim(text, ov)
+    a.step(4)
+    path = tmp_path / "a.rst"
+    a.restart_write(path)
+    assert os.path.getsize(path) == 40 + 8 * (a.get("u0").size + 3 * a.get("b1f0").size)
+    b = po.OracleSim(text, ov)
+    b.restart_read(path)
+    assert b.cycle == 4 and b.time == a.time
+    b.step(3)
+    assert b.cycle == 7 and b.time == full.time
+    _same(_state(b), ref, "resumed")
+
+
+def test_oracle_restart_rejects_mismatch(tmp_path):
+    a = po.OracleSim(deck(OT[0]), OT[1])
+    path = tmp_path / "a.rst"
+    a.restart_write(path)
+    b = po.OracleSim(deck(OT[0]), ["mesh.nx1=16", "mesh.nx2=24"])
+    with pytest.raises(RuntimeError, match="does not match"):
+        b.restart_read(path)
+
+
+def _mesh(name, ov, prec="fp64"):
+    from chasqui_amd.deck import Deck
+    d = Deck(deck(name), ov + [f"mhd.precision={prec}"])
+    m = d.make_mesh()
+    tl, nl = d.tlim, d.nlim
+    d.close()
+    return m, tl, nl
+
+
+def _gpu_state(m):
+    return {n: m.download(n) for n in ("u0", "b1f0", "b2f0", "b3f0", "w", "bcc")}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,ov", [OT, BLAST])
+def test_gpu_restart_is_bitwise_and_interoperates(tmp_path, name, ov):
+    print(f"case restart {name} {ov}", flush=True)
+    text = deck(name)
+    # uninterrupted device run
+    m, tl, nl = _mesh(name, ov)
+    m.step(7)
+    ref = _gpu_state(m)
+    ref_st = m.status()
+    m.close()
+    # device run dumped at cycle 4, resumed on the device
+    m, tl, nl = _mesh(name, ov)
+    m.step(4)
+    st = m.status()
+    dump = tmp_path / "gpu.rst"
+    m.restart_write(dump, st["t"], st["cycle"])
+    m.close()
+    r, _, _ = _mesh(name, ov)
+    t, cyc = r.restart_read(dump)
+    assert (t, cyc) == (st["t"], 4)
+    r.init(t, tl, nl, cyc)
+    r.step(3)
+    rs = r.status()
+    assert rs["cycle"] == 7 and rs["t"] == ref_st["t"] and rs["min_dt"] == ref_st["min_dt"]
+    _same(_gpu_state(r), ref, "device resume")
+    r.close()
+    # the device's dump read by the oracle, and the oracle's dump read by the device
+    o = po.OracleSim(text, ov)
+    o.restart_read(dump)
+    o.step(3)
+    _same({k: o.get(k) for k in ("u0", "b1f0", "b2f0", "b3f0")},
+          {k: ref[k] for k in ("u0", "b1f0", "b2f0", "b3f0")}, "oracle resumed from a device dump")
+    o2 = po.OracleSim(text, ov)
+    o2.step(4)
+    odump = tmp_path / "oracle.rst"
+    o2.restart_write(odump)
+    g, _, _ = _mesh(name, ov)
+    t, cyc = g.restart_read(odump)
+    g.init(t, tl, nl, cyc)
+    g.step(3)
+    _same(_gpu_state(g), ref, "device resumed from an oracle dump")
+    g.close()
+
+
+@pytest.mark.gpu
+def test_driver_restart_like_the_reference(tmp_path):
+    """physics.sh:58-59: OT 64^2 to t = 0.2 with dumps every 0.1, resumed from the first dump;
+    the final VTK files are identical."""
+    ot = os.path.join(DECKS, "orszag_tang.in")
+    common = ["mesh.nx1=64", "mesh.nx2=64", "time.tlim=0.2", "time.ndiag=0", "output.vtk_dt=0.2", "output.hst_dt=-1"]
+    r1 = subprocess.run([EXE, "-i", ot] + common + ["output.basename=full", "output.rst_dt=0.1"], cwd=tmp_path,
+                        capture_output=True, text=True, timeout=300)
+    assert r1.returncode == 0, r1.stderr
+    assert (tmp_path / "full.0000.rst").exists()
+    r2 = subprocess.run([EXE, "-i", ot, "-r", "full.0000.rst"] + common + ["output.basename=resumed"], cwd=tmp_path,
+                        capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    assert "# restarted from full.0000.rst" in r2.stdout
+    a = sorted(p for p in os.listdir(tmp_path) if p.startswith("full.") and p.endswith(".vtk"))[-1]
+    b = sorted(p for p in os.listdir(tmp_path) if p.startswith("resumed.") and p.endswith(".vtk"))[-1]
+    assert (tmp_path / a).read_bytes() == (tmp_path / b).read_bytes()
