@@ -115,6 +115,12 @@ def test_reference_stage_entry_bitwise(name, ov):
 
 
 STEP_CASES = [
+    # ragged and minimal meshes: every multi-cell dimension at least nghost cells
+    ("blast_3d.in", ["mesh.nx1=17", "mesh.nx2=5", "mesh.nx3=9", "mesh.bc_x3=outflow"], 5),
+    ("blast_3d.in", ["mesh.nx1=3", "mesh.nx2=3", "mesh.nx3=3"], 4),
+    ("orszag_tang.in", ["mesh.nx1=3", "mesh.nx2=70"], 6),
+    ("cpaw_1d.in", ["mesh.nx1=3"], 6),
+    ("blast_3d.in", ["mesh.nx1=130", "mesh.nx2=3", "mesh.nx3=4", "mhd.reconstruct=wenoz", "mhd.rsolver=lhlld"], 3),
     ("blast_3d.in", ["mesh.nx1=24", "mesh.nx2=24", "mesh.nx3=24"], 6),
     ("blast_3d.in", ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=16", "mhd.pfloor=0.1"], 3),   # fofc fires
     ("orszag_tang.in", ["mesh.nx1=32", "mesh.nx2=32"], 10),
