@@ -203,8 +203,8 @@ __global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB
 // same thread in the previous iteration, so every input plane is read once per segment instead
 // of being fetched again by the blocks of the next plane. Same expressions, same order as
 // corner3_at / update_at.
-// planes per thread, measured at 256^3 (corner: 4 beats 8, 16, 32, 64; mixed update: 8)
-constexpr int KSEG_CORNER = 4, KSEG_UPDATE = 8;
+// planes per thread, measured at 256^3 (corner: 4 beats 8, 16, 32, 64; mixed update: 4 beats 8, 16)
+constexpr int KSEG_CORNER = 4, KSEG_UPDATE = 4;
 
 template <class T>
 __global__ void __launch_bounds__(256) k_corner_march(const CornerArgsT<T> a, const VB v, const int nk, const int kseg) {
@@ -456,14 +456,17 @@ __device__ __forceinline__ int wave_sum(int v) {
 // flag_mode 1 writes the flag array (reference semantics); 2 appends flagged cells to a list.
 // DT: reduce the CFL minimum. ENT: resynchronise u[IENT] (and w[IENT] if write_went); the
 // lean variant (no pow) serves RK stages whose u[IENT] nothing reads.
-template <class T, bool DT, bool ENT>
-__device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i, double &mdt, int &nbad_local) {
+// bz: the cell-centred x3 field (the caller averages the two x3 faces). DE: the dual
+// energy branches are compiled in (without them the kernel holds no pow() for the flag and
+// needs far fewer registers).
+template <class T, bool DT, bool ENT, bool DE>
+__device__ __forceinline__ void c2p_core(const C2pArgsT<T> &a, int k, int j, int i, const double bz, double &mdt,
+                                         int &nbad_local) {
   const Geom &g = a.g;
   const Eos e = a.eos;
-  const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
+  const long long c = g.at(k, j, i), sj = g.pitch, cs = g.cstride;
   const double bx = 0.5 * (a.b1[c] + a.b1[c + 1]);
   const double by = g.two_d ? 0.5 * (a.b2[c] + a.b2[c + sj]) : a.b2[c];
-  const double bz = g.three_d ? 0.5 * (a.b3[c] + a.b3[c + sk]) : a.b3[c];
   double *u = a.u + c;
   const double u0 = u[IDN * cs], u1 = u[IM1 * cs], u2 = u[IM2 * cs], u3 = u[IM3 * cs], u4 = u[IEN * cs];
   const bool active = i >= g.is && i <= g.ie && j >= g.js && j <= g.je && k >= g.ks && k <= g.ke;
@@ -473,7 +476,7 @@ __device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i
       const double kin = 0.5 * (sq(u1) + sq(u2) + sq(u3)) / u0;
       const double ei = u4 - kin - 0.5 * (bx * bx + by * by + bz * bz);
       double p = e.gm1 * ei;
-      if (e.dual_energy && ei < e.de_eta * fabs(u4)) p = (u[IENT * cs] / u0) * pow(u0, e.gamma);
+      if (DE && e.dual_energy && ei < e.de_eta * fabs(u4)) p = (u[IENT * cs] / u0) * pow(u0, e.gamma);
       bad = p <= e.pfloor;
     }
     if (a.flag_mode == 1) {
@@ -492,7 +495,7 @@ __device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i
   const double ei = u4 - kin - 0.5 * bsq;
   double p = e.gm1 * ei;
   double se = 0.0;
-  if (ENT && e.dual_energy && ei < e.de_eta * fabs(u4)) {
+  if (ENT && DE && e.dual_energy && ei < e.de_eta * fabs(u4)) {
     se = u[IENT * cs] * id;
     p = se * pow(d, e.gamma);
     p = fmax(p, e.pfloor);
@@ -531,6 +534,14 @@ __device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i
   }
 }
 
+template <class T, bool DT, bool ENT, bool DE>
+__device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i, double &mdt, int &nbad_local) {
+  const Geom &g = a.g;
+  const long long c = g.at(k, j, i);
+  const double bz = g.three_d ? 0.5 * (a.b3[c] + a.b3[c + g.plane]) : a.b3[c];
+  c2p_core<T, DT, ENT, DE>(a, k, j, i, bz, mdt, nbad_local);
+}
+
 // Block merge of the CFL minimum and the flag count into the control block (the block's
 // minimum goes out only when it beats the current one). Every thread of the block calls it.
 template <bool DT, class T>
@@ -556,14 +567,21 @@ __device__ __forceinline__ void c2p_block_merge(const C2pArgsT<T> &a, double mdt
 // cons -> prim over all cells, physics.hpp:437-494. The CFL minimum is reduced with wave64
 // shuffles and one LDS pass per block, then merged with a 64-bit atomicMin on the bit
 // pattern, issued only when the block's minimum beats the current one.
-template <class T, bool DT, bool ENT>
+// LOOP: grid-stride over the tiles (the conditional launches); otherwise one tile per block,
+// which keeps the loop-invariant constants of pow() out of the registers of the whole kernel.
+template <class T, bool DT, bool ENT, bool DE, bool LOOP>
 __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   double mdt = 1e300;
   int nbad_local = 0;
-  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+  if (LOOP) {
+    for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+      int k, j, i;
+      if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
+    }
+  } else {
     int k, j, i;
-    if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT>(a, k, j, i, mdt, nbad_local);
+    if (v.at(int(blockIdx.x), k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
   }
   if (!DT && a.flag_mode != 1) return;
   __shared__ double smin[BX * BY / 64];
@@ -708,7 +726,7 @@ __device__ __forceinline__ void c2p_tile(const RunArgs<T> &run, const Ctrl *ctl,
   double mdt = 1e300;
   int nb = 0;
   int k, j, i;
-  if (c2p_vb(a.g).at(tile, k, j, i)) c2p_at<T, LAST, LAST>(a, k, j, i, mdt, nb);
+  if (c2p_vb(a.g).at(tile, k, j, i)) c2p_at<T, LAST, LAST, false>(a, k, j, i, mdt, nb);   // batches: no dual energy
   if (LAST) c2p_block_merge<LAST>(a, mdt, nb, smin, ssum);
 }
 
@@ -914,19 +932,43 @@ hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <class T>
-hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s) {
+namespace {
+template <class T, bool DE>
+void launch_c2p_de(const C2pArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
+  const bool ent = a.entropy_out || a.eos.dual_energy;
   const VB v = c2p_vb(g);
   const dim3 grid(grid_of(v, a.only_if_bad)), block(256);
-  const bool ent = a.entropy_out || a.eos.dual_energy;
-  if (a.compute_dt) {
-    if (ent) hipLaunchKernelGGL((k_c2p<T, true, true>), grid, block, 0, s, a, v);
-    else hipLaunchKernelGGL((k_c2p<T, true, false>), grid, block, 0, s, a, v);
+  if (a.only_if_bad) {
+    if (a.compute_dt) {
+      if (ent) hipLaunchKernelGGL((k_c2p<T, true, true, DE, true>), grid, block, 0, s, a, v);
+      else hipLaunchKernelGGL((k_c2p<T, true, false, DE, true>), grid, block, 0, s, a, v);
+    } else {
+      if (ent) hipLaunchKernelGGL((k_c2p<T, false, true, DE, true>), grid, block, 0, s, a, v);
+      else hipLaunchKernelGGL((k_c2p<T, false, false, DE, true>), grid, block, 0, s, a, v);
+    }
   } else {
-    if (ent) hipLaunchKernelGGL((k_c2p<T, false, true>), grid, block, 0, s, a, v);
-    else hipLaunchKernelGGL((k_c2p<T, false, false>), grid, block, 0, s, a, v);
+    if (a.compute_dt) {
+      if (ent) hipLaunchKernelGGL((k_c2p<T, true, true, DE, false>), grid, block, 0, s, a, v);
+      else hipLaunchKernelGGL((k_c2p<T, true, false, DE, false>), grid, block, 0, s, a, v);
+    } else {
+      if (ent) hipLaunchKernelGGL((k_c2p<T, false, true, DE, false>), grid, block, 0, s, a, v);
+      else hipLaunchKernelGGL((k_c2p<T, false, false, DE, false>), grid, block, 0, s, a, v);
+    }
   }
+}
+}  // namespace
+
+template <class T>
+hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s) {
+  // dual energy only on the FP64 path (mixed precision and batches reject it)
+  if constexpr (std::is_same<T, double>::value) {
+    if (a.eos.dual_energy) {
+      launch_c2p_de<T, true>(a, s);
+      return hipGetLastError();
+    }
+  }
+  launch_c2p_de<T, false>(a, s);
   return hipGetLastError();
 }
 

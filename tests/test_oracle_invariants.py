@@ -86,3 +86,36 @@ def test_bad_deck_key_raises():
         po.OracleSim(deck("blast_3d.in"), ["mhd.reconstruct=nonsense"])
     with pytest.raises(RuntimeError):
         po.OracleSim(deck("blast_3d.in"), ["mesh.bc_x1=bogus"])
+
+
+def test_dual_energy_without_switch_is_the_total_energy_path():
+    """With dual energy on but e_int >= de_eta * E everywhere, the advected entropy is never
+    used and con2prim resynchronises u[IENT] from rho and p in every cell (physics.hpp:463-472):
+    the whole state is bitwise the plain path's."""
+    ov = ["mesh.nx1=32", "mesh.nx2=32", "mhd.reconstruct=ppmx", "mhd.rsolver=lhlld"]
+    a = po.OracleSim(deck("orszag_tang.in"), ov + ["mhd.dual_energy=true"])
+    b = po.OracleSim(deck("orszag_tang.in"), ov)
+    a.step(10)
+    b.step(10)
+    for f in ("u0", "b1f0", "b2f0", "b3f0", "w", "bcc"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+
+
+def test_dual_energy_switch_uses_the_entropy_pressure():
+    """Where the switch fires (de_eta raised), con2prim's pressure is s * rho^gamma with
+    s = u[IENT] / rho, and the total energy is rebuilt from it (physics.hpp:463-469)."""
+    o = po.OracleSim(deck("brio_wu.in"), ["mesh.nx1=200", "mhd.dual_energy=true", "mhd.de_eta=0.5"])
+    o.step(30)
+    gamma = 2.0
+    ks, js, i = o.ks, o.js, slice(o.is_, o.ie + 1)
+    u, w, bc = o.get("u0")[:, ks, js, i], o.get("w")[:, ks, js, i], o.get("bcc")[:, ks, js, i]
+    d = u[0]
+    kin = 0.5 * (u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / d
+    ei = u[4] - kin - 0.5 * (bc ** 2).sum(0)
+    sw = ei < 0.5 * np.abs(u[4])
+    assert np.count_nonzero(sw) > 0.3 * d.size
+    p_ent = (u[5][sw] / d[sw]) * d[sw] ** gamma
+    assert np.max(np.abs(w[4][sw] - p_ent) / p_ent) < 1e-13
+    # E = kin + B^2/2 + p/(gamma-1) in switched cells
+    e_re = kin[sw] + 0.5 * (bc[:, sw] ** 2).sum(0) + w[4][sw] / (gamma - 1.0)
+    assert np.max(np.abs(u[4][sw] - e_re) / np.abs(u[4][sw])) < 1e-13
