@@ -38,7 +38,8 @@ class Config(C.Structure):
                 ("gamma", C.c_double), ("dfloor", C.c_double), ("pfloor", C.c_double),
                 ("cfl", C.c_double), ("flatten", C.c_int), ("low_mach", C.c_int),
                 ("fofc", C.c_int), ("dual_energy", C.c_int), ("de_eta", C.c_double),
-                ("lhlld_phi", C.c_double), ("bc", C.c_int * 3), ("precision", C.c_int)]
+                ("lhlld_phi", C.c_double), ("bc", C.c_int * 3), ("precision", C.c_int),
+                ("characteristic", C.c_int)]
 
 
 class Status(C.Structure):

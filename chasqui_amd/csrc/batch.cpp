@@ -396,6 +396,8 @@ int chq_batch_add(chq_batch *b, chq_ctx *c) {
   if (c->batch) return fail(CHQ_EINVAL, "context already belongs to a batch");
   if (int(b->runs.size()) >= b->capacity) return fail(CHQ_EINVAL, "batch is full");
   if (c->eos.dual_energy) return fail(CHQ_EUNSUP, "batched runs need dual energy off");
+  if (c->cfg.characteristic && c->cfg.recon != 0)
+    return fail(CHQ_EUNSUP, "batched runs need primitive-variable reconstruction");
   if (c->device != b->device) return fail(CHQ_EINVAL, "context and batch are on different devices");
   const int prec = c->mixed ? 1 : 0;
   if (!b->keyed) {

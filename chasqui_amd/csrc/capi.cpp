@@ -146,6 +146,8 @@ int check_cfg(const chq_mesh_desc *m, const chq_config *c) {
   if (c->recon < 0 || c->recon > 4) return fail(CHQ_EINVAL, "unknown reconstruction");
   if (c->precision < 0 || c->precision > 1) return fail(CHQ_EINVAL, "precision must be 0 (fp64) or 1 (mixed)");
   if (c->precision == 1 && c->dual_energy) return fail(CHQ_EUNSUP, "mixed precision with dual energy is not supported");
+  if (c->precision == 1 && c->characteristic && c->recon != 0)
+    return fail(CHQ_EUNSUP, "mixed precision with characteristic reconstruction is not supported");
   if (c->rsolver < 0 || c->rsolver > 2) return fail(CHQ_EINVAL, "unknown Riemann solver");
   if (c->nstage < 1 || c->nstage > 4) return fail(CHQ_EINVAL, "nstage must be 1..4");
   const int ngmin = (c->recon == 2 || c->recon == 3 || c->recon == 4) ? 3 : 2;   // main.cpp:169-171

@@ -93,6 +93,7 @@ struct chq_ctx {
     a.flatten = cfg.flatten;
     a.low_mach = cfg.low_mach;
     a.entropy = 1;
+    a.charac = cfg.characteristic && cfg.recon != 0;   // physics.hpp:71
     a.lhlld_phi = cfg.lhlld_phi;
     // physics.hpp:85-88
     int il = g.is, iu = g.ie, jl = g.js, ju = g.je, kl = g.ks, ku = g.ke;
@@ -173,6 +174,7 @@ struct chq_ctx {
     FluxArgs a0 = flux_args(d, cur, wset, c);
     FluxArgsT<T> a;
     a.g = a0.g; a.eos = a0.eos; a.flatten = a0.flatten; a.low_mach = a0.low_mach; a.entropy = a0.entropy;
+    a.charac = a0.charac;
     a.lhlld_phi = a0.lhlld_phi;
     a.il = a0.il; a.iu = a0.iu; a.jl = a0.jl; a.ju = a0.ju; a.kl = a0.kl; a.ku = a0.ku;
     a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.bn = a0.bn; a.flx = f.flx[d];

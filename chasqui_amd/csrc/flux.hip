@@ -32,6 +32,7 @@ __global__ void __launch_bounds__(BX * BY) k_repair(const FluxArgs a, const int 
 }  // namespace
 
 hipError_t launch_flux(const FluxArgsT<double> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s) {
+  if (a.charac) return launch_flux_charac(a, dir, rc, rs, entropy, s);
   return fluxk::launch_flux_t<double>(a, dir, rc, rs, entropy, s);
 }
 

@@ -11,7 +11,7 @@ namespace chq {
 // The mixed path never transports the entropy variable (it requires dual energy off, where
 // con2prim rebuilds u[IENT] from rho and p), so only the ENT = false kernels are built.
 hipError_t launch_flux(const FluxArgsT<float> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s) {
-  if (entropy) return hipErrorInvalidValue;
+  if (entropy || a.charac) return hipErrorInvalidValue;
   if (dir == 0) return fluxk::go_flux_rc<float, 0, false>(a, rc, rs, s);
   if (dir == 1) return fluxk::go_flux_rc<float, 1, false>(a, rc, rs, s);
   return fluxk::go_flux_rc<float, 2, false>(a, rc, rs, s);

@@ -259,7 +259,7 @@ void Deck::configure() {
   const std::string s = p.get_string("mhd", "rsolver", "hlld");
   cfg.rsolver = s == "hlle" ? 0 : s == "hlld" ? 1 : s == "lhlld" ? 2 : -1;
   if (cfg.rsolver < 0) throw std::runtime_error("unknown rsolver: " + s);
-  if (p.get_bool("mhd", "characteristic", false)) throw std::runtime_error("mhd.characteristic = true is not supported");
+  cfg.characteristic = p.get_bool("mhd", "characteristic", false) ? 1 : 0;   // main.cpp:123
   cfg.lhlld_phi = p.get_real("mhd", "lhlld_phi", -1.0);
   cfg.dual_energy = p.get_bool("mhd", "dual_energy", false);
   cfg.de_eta = p.get_real("mhd", "de_eta", 1.0e-3);

@@ -16,6 +16,7 @@ struct FluxArgsT {
   Eos eos;
   int flatten, low_mach;
   int entropy;        // reconstruct and flux the entropy variable (IENT)
+  int charac;         // reconstruct in characteristic variables (FP64 single-mesh launches only)
   double lhlld_phi;
   int il, iu, jl, ju, kl, ku;
   const T *w;         // primitives, 6 components (IENT used only when entropy is transported)
@@ -201,6 +202,8 @@ struct CondList {
 // launches (defined in stage.hip / flux.hip / flux32.hip)
 hipError_t launch_flux(const FluxArgsT<double> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s);
 hipError_t launch_flux(const FluxArgsT<float> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s);
+// characteristic-variable reconstruction (FP64; launch_flux forwards when a.charac is set)
+hipError_t launch_flux_charac(const FluxArgsT<double> &a, int dir, Recon rc, RSolver rs, bool entropy, hipStream_t s);
 hipError_t launch_repair(const FluxArgs &a, int dir, bool only_if_bad, hipStream_t s);
 hipError_t launch_repair_list(const FluxArgsT<double> a[3], int ndir, const int *list, hipStream_t s);
 hipError_t launch_repair_list(const FluxArgsT<float> a[3], int ndir, const int *list, hipStream_t s);

@@ -246,6 +246,10 @@ int chq_server_submit(chq_server *s, const char *deck_text, const char *override
   chq_deck_run(j.deck, &j.tlim, &j.nlim, &ndiag);
   if (precision >= 0) j.cfg.precision = (precision == 1 && !j.cfg.dual_energy) ? 1 : 0;
   if (j.cfg.dual_energy) { chq_deck_close(j.deck); return fail(CHQ_EUNSUP, "served runs need dual energy off"); }
+  if (j.cfg.characteristic && j.cfg.recon != 0) {
+    chq_deck_close(j.deck);
+    return fail(CHQ_EUNSUP, "served runs need primitive-variable reconstruction");
+  }
   j.bytes = chq_bytes_for(&j.mesh, &j.cfg);
   if (j.bytes == 0) { chq_deck_close(j.deck); return fail(CHQ_EINVAL, "invalid mesh or configuration"); }
   if (j.bytes > s->budget) { chq_deck_close(j.deck); return fail(CHQ_ENOMEM, "run larger than the HBM budget"); }

@@ -15,7 +15,8 @@ from ._capi import ARRAY_IDS, CELL_ARRAYS, FACE_MULTI, check, lib
 
 def make_config(recon="ppm", rsolver="hlld", integrator="rk3", gamma=5.0 / 3.0, dfloor=1e-10,
                 pfloor=1e-12, cfl=0.3, flatten=True, low_mach=False, fofc=True, dual_energy=False,
-                de_eta=1e-3, lhlld_phi=-1.0, bc=("periodic", "periodic", "periodic"), precision="fp64"):
+                de_eta=1e-3, lhlld_phi=-1.0, bc=("periodic", "periodic", "periodic"), precision="fp64",
+                characteristic=False):
     c = _capi.Config()
     c.recon = _capi.RECON[recon]
     c.rsolver = _capi.RSOLVER[rsolver]
@@ -26,6 +27,7 @@ def make_config(recon="ppm", rsolver="hlld", integrator="rk3", gamma=5.0 / 3.0, 
     for d in range(3):
         c.bc[d] = _capi.BC[bc[d]]
     c.precision = _capi.PRECISION[precision]
+    c.characteristic = int(characteristic)
     return c
 
 

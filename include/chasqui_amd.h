@@ -45,7 +45,10 @@ typedef struct {
  * precision: 0 = FP64 throughout (the reference's arithmetic, bitwise); 1 = mixed: primitives,
  * reconstruction, Riemann solve, fluxes and EMFs in FP32, con2prim, RK combine, CT and all sums
  * in FP64 (conserved state and face B stay FP64). Mixed needs dual_energy = 0. No counterpart
- * in the reference, which is FP64 only (defs.hpp:11 Real = double). */
+ * in the reference, which is FP64 only (defs.hpp:11 Real = double).
+ * characteristic: reconstruct in the characteristic variables of each side's cell
+ * (Config::characteristic, physics.hpp:30, 71, 110-150); ignored with dc; FP64 only, and not
+ * in batches or served runs. */
 typedef struct {
   int recon, rsolver, nstage;
   double gamma, dfloor, pfloor, cfl;
@@ -53,6 +56,7 @@ typedef struct {
   double de_eta, lhlld_phi;
   int bc[3];
   int precision;
+  int characteristic;
 } chq_config;
 
 /* Array identifiers for chq_upload / chq_download / chq_array_count. */

@@ -382,4 +382,97 @@ inline void hlld(const Prim &L, const Prim &R, Real bn, const Eos &eos, Real f[N
   else { for (int n = 0; n < NF; ++n) f[n] = 0.5 * (flss[n] + frss[n]); }
 }
 
+// ---------------------------------------------------------------- characteristic variables
+// Eigensystem of the 1D primitive ideal-MHD system in W = (rho, u, v, w, p, By, Bz) with the
+// normal field bx (Roe & Balsara 1996; Stone et al. 2008 App. A), eigen_mhd.hpp:15-82.
+// Waves ordered u-cf, u-ca, u-cs, u, u+cs, u+ca, u+cf; R holds the right eigenvectors as
+// columns, L the left ones as rows (L R = I). The velocity does not enter. Every entry is the
+// reference's expression with its association.
+inline void eigensystem(Real d, Real p, Real by, Real bz, Real bx, Real gamma, Real L[7][7], Real R[7][7]) {
+  const Real sqrtd = std::sqrt(d);
+  const Real isqrtd = 1.0 / sqrtd;
+  const Real asq = gamma * p / d;
+  const Real a = std::sqrt(asq);
+  const Real casq = bx * bx / d;
+  const Real bperpsq = by * by + bz * bz;
+  const Real ctsq = bperpsq / d;
+  const Real tsum = asq + casq + ctsq;
+  const Real tdif = asq + ctsq - casq;
+  const Real disc = std::sqrt(std::fmax(tdif * tdif + 4.0 * ctsq * casq, 0.0));
+  const Real cfsq = 0.5 * (tsum + disc);
+  const Real cssq = std::fmax(0.5 * (tsum - disc), 0.0);
+  const Real cf = std::sqrt(cfsq), cs = std::sqrt(cssq);
+  Real alf, als;                                                  // eigen_mhd.hpp:33-46
+  if (cfsq - cssq <= 1e-12 * cfsq) { alf = 1.0; als = 0.0; }
+  else if (ctsq <= 1e-12 * (asq + casq)) { alf = asq >= casq ? 1.0 : 0.0; als = asq >= casq ? 0.0 : 1.0; }
+  else if (asq - cssq <= 0.0) { alf = 0.0; als = 1.0; }
+  else if (cfsq - asq <= 0.0) { alf = 1.0; als = 0.0; }
+  else {
+    alf = std::sqrt((asq - cssq) / (cfsq - cssq));
+    als = std::sqrt((cfsq - asq) / (cfsq - cssq));
+  }
+  Real bety, betz;                                                // eigen_mhd.hpp:48-55
+  if (bperpsq > 0.0) {
+    const Real ib = 1.0 / std::sqrt(bperpsq);
+    bety = by * ib;
+    betz = bz * ib;
+  } else {
+    bety = 1.0 / std::sqrt(2.0);
+    betz = bety;
+  }
+  const Real s = bx >= 0.0 ? 1.0 : -1.0;
+  const Real cff = cf * alf, css = cs * als;
+  const Real qf = cff * s, qs = css * s;
+  const Real af = a * alf * sqrtd, as = a * als * sqrtd;
+  for (int r = 0; r < 7; ++r)
+    for (int c = 0; c < 7; ++c) { L[r][c] = 0.0; R[r][c] = 0.0; }
+  // right eigenvectors, eigen_mhd.hpp:63-70 (row = variable, column = wave)
+  R[0][0] = d * alf; R[0][2] = d * als; R[0][3] = 1.0; R[0][4] = d * als; R[0][6] = d * alf;
+  R[1][0] = -cff; R[1][2] = -css; R[1][4] = css; R[1][6] = cff;
+  R[2][0] = qs * bety; R[2][1] = -betz; R[2][2] = -qf * bety; R[2][4] = qf * bety; R[2][5] = betz; R[2][6] = -qs * bety;
+  R[3][0] = qs * betz; R[3][1] = bety; R[3][2] = -qf * betz; R[3][4] = qf * betz; R[3][5] = -bety; R[3][6] = -qs * betz;
+  R[4][0] = d * alf * asq; R[4][2] = d * als * asq; R[4][4] = d * als * asq; R[4][6] = d * alf * asq;
+  R[5][0] = as * bety; R[5][1] = -betz * s * sqrtd; R[5][2] = -af * bety; R[5][4] = -af * bety;
+  R[5][5] = -betz * s * sqrtd; R[5][6] = as * bety;
+  R[6][0] = as * betz; R[6][1] = bety * s * sqrtd; R[6][2] = -af * betz; R[6][4] = -af * betz;
+  R[6][5] = bety * s * sqrtd; R[6][6] = as * betz;
+  // left eigenvectors, eigen_mhd.hpp:72-81 (row = wave, column = variable)
+  const Real norm = 0.5 / asq;
+  const Real ncff = norm * cff, ncss = norm * css, nqf = norm * qf, nqs = norm * qs;
+  const Real naf = norm * af / d, nas = norm * as / d;
+  L[0][1] = -ncff; L[0][2] = nqs * bety; L[0][3] = nqs * betz; L[0][4] = norm * alf / d; L[0][5] = nas * bety; L[0][6] = nas * betz;
+  L[1][2] = -0.5 * betz; L[1][3] = 0.5 * bety; L[1][5] = -0.5 * betz * s * isqrtd; L[1][6] = 0.5 * bety * s * isqrtd;
+  L[2][1] = -ncss; L[2][2] = -nqf * bety; L[2][3] = -nqf * betz; L[2][4] = norm * als / d; L[2][5] = -naf * bety; L[2][6] = -naf * betz;
+  L[3][0] = 1.0; L[3][4] = -1.0 / asq;
+  L[4][1] = ncss; L[4][2] = nqf * bety; L[4][3] = nqf * betz; L[4][4] = norm * als / d; L[4][5] = -naf * bety; L[4][6] = -naf * betz;
+  L[5][2] = 0.5 * betz; L[5][3] = -0.5 * bety; L[5][5] = -0.5 * betz * s * isqrtd; L[5][6] = 0.5 * bety * s * isqrtd;
+  L[6][1] = ncff; L[6][2] = -nqs * bety; L[6][3] = -nqs * betz; L[6][4] = norm * alf / d; L[6][5] = nas * bety; L[6][6] = nas * betz;
+}
+
+// Reconstruction of the 7 primitive variables in characteristic variables for one side of a
+// face, physics.hpp:110-150: project the 6-point stencils with the left eigenvectors of that
+// side's cell (stencil point `cell`: 2 for the left side, 3 for the right), reconstruct each
+// characteristic variable, take the requested side (left: 0, right: 1) and map back with R.
+inline void charac_side(Recon rc, const Real st[7][6], int cell, Real bx, Real gamma, int side, Real out[7]) {
+  Real Lm[7][7], Rm[7][7];
+  eigensystem(st[0][cell], st[4][cell], st[5][cell], st[6][cell], bx, gamma, Lm, Rm);
+  Real ch[7];
+  for (int kk = 0; kk < 7; ++kk) {
+    Real c[6];
+    for (int p = 0; p < 6; ++p) {
+      Real acc = 0.0;
+      for (int n = 0; n < 7; ++n) acc += Lm[kk][n] * st[n][p];
+      c[p] = acc;
+    }
+    Real cl, cr;
+    reconstruct(rc, c, cl, cr);
+    ch[kk] = side == 0 ? cl : cr;
+  }
+  for (int n = 0; n < 7; ++n) {
+    Real acc = 0.0;
+    for (int kk = 0; kk < 7; ++kk) acc += Rm[n][kk] * ch[kk];
+    out[n] = acc;
+  }
+}
+
 }  // namespace orc

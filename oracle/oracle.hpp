@@ -10,8 +10,8 @@
 // cites the reference file:line it follows. Min-reductions and integer counts are exact,
 // so results do not depend on the OpenMP thread count.
 //
-// Not restated (out of scope of the stage pipeline): characteristic reconstruction,
-// diffusion, gravity, cooling, super-time-stepping, the positivity flux limiter, restart.
+// Not restated (out of scope of the stage pipeline): diffusion, gravity, cooling,
+// super-time-stepping, the positivity flux limiter.
 // Requesting any of them raises an error instead of silently running something else.
 #pragma once
 #include <cstdint>
@@ -116,6 +116,7 @@ struct Cfg {   // physics.hpp:22-33 + the driver switches of main.cpp:112-163 th
   Real lhlld_phi = -1.0;
   Bc bc[3] = {Bc::periodic, Bc::periodic, Bc::periodic};
   bool fofc = true;
+  bool characteristic = false;   // physics.hpp:30
 };
 
 // ---------------------------------------------------------------- mesh (mesh.hpp:13-71)
@@ -158,12 +159,13 @@ struct Grid {
 };
 
 // ---------------------------------------------------------------- fused face flux
-// One direction of the reconstruction + Riemann kernel, physics.hpp:65-226
-// (primitive-variable reconstruction only; the characteristic branch 110-150 is out of scope).
+// One direction of the reconstruction + Riemann kernel, physics.hpp:65-226, with the
+// characteristic-variable branch of physics.hpp:110-150 (charac_side).
 template <int DIR, Recon RC, RSolver RS>
 void fluxes(Grid &m, const Cfg &cfg, const A3 &bn) {
   const Eos eos = cfg.eos;
   const bool flat = cfg.flatten && RC != Recon::dc && RC != Recon::plm;
+  const bool charac = cfg.characteristic && RC != Recon::dc;      // physics.hpp:71
   constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
   constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
   constexpr int B1 = (DIR + 1) % 3, B2 = (DIR + 2) % 3;
@@ -203,7 +205,12 @@ void fluxes(Grid &m, const Cfg &cfg, const A3 &bn) {
         load(w, IDN, st[0]); load(w, IU, st[1]); load(w, IV, st[2]); load(w, IW, st[3]);
         load(w, IPR, st[4]); load(bcc, B1, st[5]); load(bcc, B2, st[6]);
         Real wl[7], wr[7];
-        for (int n = 0; n < 7; ++n) reconstruct(RC, st[n], wl[n], wr[n]);   // physics.hpp:109
+        if (!charac) {
+          for (int n = 0; n < 7; ++n) reconstruct(RC, st[n], wl[n], wr[n]);   // physics.hpp:109
+        } else {   // each side in the characteristic variables of its own cell (normal field bcc(DIR))
+          charac_side(RC, st, 2, bcc(DIR, k - dk, j - dj, i - di), eos.gamma, 0, wl);
+          charac_side(RC, st, 3, bcc(DIR, k, j, i), eos.gamma, 1, wr);
+        }
         Real qe[6], sl, sr;                                                    // physics.hpp:152-154
         load(w, IENT, qe); reconstruct(RC, qe, sl, sr);
         const Real sl0 = qe[2], sr0 = qe[3];
@@ -883,7 +890,7 @@ struct Sim {
     const std::string s = p.str("mhd", "rsolver", "hlld");
     cfg.rsolver = s == "hlld" ? RSolver::hlld : s == "lhlld" ? RSolver::lhlld : s == "hlle" ? RSolver::hlle
                 : throw std::runtime_error("unknown rsolver: " + s);
-    if (p.boolean("mhd", "characteristic", false)) throw std::runtime_error("characteristic reconstruction is not restated");
+    cfg.characteristic = p.boolean("mhd", "characteristic", false);
     cfg.lhlld_phi = p.real("mhd", "lhlld_phi", -1.0);
     cfg.eos.dual_energy = p.boolean("mhd", "dual_energy", false);
     cfg.eos.de_eta = p.real("mhd", "de_eta", 1.0e-3);

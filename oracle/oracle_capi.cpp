@@ -162,5 +162,20 @@ double orc_fast_speed(const double *S7, double bn, double gamma) {
   return fast_speed(P, bn, gamma);
 }
 double orc_noise(unsigned long long seed, unsigned long long idx) { return noise_pm1(seed, idx); }
+// eigensystem of n states st7 = (d, p, by, bz, bx) x n: L and R as n row-major 7x7 blocks
+void orc_eigensystem(const double *st5, long n, double gamma, double *L, double *R) {
+  for (long s = 0; s < n; ++s) {
+    const double *a = st5 + 5 * s;
+    eigensystem(a[0], a[1], a[2], a[3], a[4], gamma, reinterpret_cast<double(*)[7]>(L + 49 * s),
+                reinterpret_cast<double(*)[7]>(R + 49 * s));
+  }
+}
+// one side of a face in characteristic variables: st = 7x6 stencils per sample, bx per sample
+void orc_charac_side(int recon, const double *st, const double *bx, long n, double gamma, int cell, int side,
+                     double *out7) {
+  for (long s = 0; s < n; ++s)
+    charac_side(Recon(recon), reinterpret_cast<const double(*)[6]>(st + 42 * s), cell, bx[s], gamma, side,
+                out7 + 7 * s);
+}
 
 }  // extern "C"

@@ -72,6 +72,8 @@ def lib():
     L.orc_fast_speed.argtypes = [dp, C.c_double, C.c_double]
     L.orc_noise.restype = C.c_double
     L.orc_noise.argtypes = [C.c_ulonglong, C.c_ulonglong]
+    L.orc_eigensystem.argtypes = [dp, C.c_long, C.c_double, dp, dp]
+    L.orc_charac_side.argtypes = [C.c_int, dp, dp, C.c_long, C.c_double, C.c_int, C.c_int, dp]
     _lib = L
     return L
 
@@ -259,3 +261,24 @@ def phys_flux(S, bn, gamma=5.0 / 3.0):
 def fast_speed(S, bn, gamma=5.0 / 3.0):
     S = np.ascontiguousarray(S, dtype=np.float64)
     return lib().orc_fast_speed(_dp(S), bn, gamma)
+
+
+def eigensystem(d, p, by, bz, bx, gamma=5.0 / 3.0):
+    """Left (rows) and right (columns) eigenvectors of the primitive MHD system (n, 7, 7)."""
+    st = np.ascontiguousarray(np.stack(np.broadcast_arrays(d, p, by, bz, bx), axis=-1), dtype=np.float64).reshape(-1, 5)
+    n = st.shape[0]
+    L = np.empty((n, 7, 7))
+    R = np.empty((n, 7, 7))
+    lib().orc_eigensystem(_dp(st), n, gamma, _dp(L), _dp(R))
+    return L, R
+
+
+def charac_side(method, st, bx, side, gamma=5.0 / 3.0):
+    """One face side in characteristic variables: st (n, 7, 6) stencils, bx (n,) normal field of
+    the side's cell; side 0 = left state (cell = stencil point 2), 1 = right (point 3)."""
+    st = np.ascontiguousarray(st, dtype=np.float64).reshape(-1, 7, 6)
+    n = st.shape[0]
+    bx = np.ascontiguousarray(np.broadcast_to(bx, (n,)), dtype=np.float64)
+    out = np.empty((n, 7))
+    lib().orc_charac_side(RECON[method], _dp(st), _dp(bx), n, gamma, 2 + side, side, _dp(out))
+    return out

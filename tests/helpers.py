@@ -53,7 +53,8 @@ def gpu_mesh_for(text, overrides=()):
                       fofc=_b(g("mhd.fofc", "true")), dual_energy=_b(g("mhd.dual_energy", "false")),
                       de_eta=float(g("mhd.de_eta", 1e-3)), lhlld_phi=float(g("mhd.lhlld_phi", -1.0)),
                       bc=(g("mesh.bc_x1", "periodic"), g("mesh.bc_x2", "periodic"), g("mesh.bc_x3", "periodic")),
-                      precision=g("mhd.precision", "fp64"))
+                      precision=g("mhd.precision", "fp64"),
+                      characteristic=_b(g("mhd.characteristic", "false")))
     return Mesh(md, cfg), kv
 
 

@@ -47,6 +47,15 @@ KERNEL_CASES = [
     ("cpaw_1d.in", ["mesh.nx1=32"]),
     ("brio_wu.in", ["mesh.nx1=64", "mhd.reconstruct=ppmx", "mhd.low_mach=true"]),
     ("blast_3d.in", ["mesh.nx1=16", "mesh.nx2=12", "mesh.nx3=8", "mesh.bc_x1=reflect", "mesh.bc_x2=outflow"]),
+] + [   # reconstruction in characteristic variables (physics.hpp:110-150)
+    ("blast_3d.in", ["mesh.nx1=16", "mesh.nx2=12", "mesh.nx3=10", "mhd.characteristic=true", f"mhd.reconstruct={r}",
+                     f"mhd.rsolver={s}"])
+    for r, s in (("plm", "hlld"), ("ppm", "hlld"), ("ppmx", "lhlld"), ("wenoz", "hlle"), ("dc", "hlld"))
+] + [
+    ("orszag_tang.in", ["mesh.nx1=24", "mesh.nx2=20", "mhd.characteristic=true", "mhd.reconstruct=ppmx",
+                        "mhd.rsolver=lhlld", "mhd.dual_energy=true"]),
+    ("brio_wu.in", ["mesh.nx1=64", "mhd.characteristic=true", "mhd.low_mach=true"]),
+    ("cpaw_1d.in", ["mesh.nx1=32", "mhd.characteristic=true", "mhd.reconstruct=plm"]),
 ]
 
 
@@ -127,6 +136,12 @@ STEP_CASES = [
     ("brio_wu.in", ["mesh.nx1=200"], 30),
     ("cpaw_3d_oblique.in", ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=16"], 4),
     ("cpaw_1d.in", ["mesh.nx1=64", "mhd.reconstruct=plm", "time.integrator=rk2"], 10),
+    # characteristic variables: the reference's recommended preset (inputs/recommended_mhd.in)
+    # on Orszag-Tang, and the blast (fused stage order, fofc) with characteristic PPM
+    ("orszag_tang.in", ["mesh.nx1=32", "mesh.nx2=32", "mhd.characteristic=true", "mhd.reconstruct=ppmx",
+                        "mhd.rsolver=lhlld", "mhd.dual_energy=true", "time.cfl=0.4"], 10),
+    ("blast_3d.in", ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=16", "mhd.characteristic=true"], 4),
+    ("blast_3d.in", ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12", "mhd.characteristic=true", "mhd.pfloor=0.1"], 3),
 ]
 
 

@@ -1,5 +1,7 @@
 """The C++ host layer on a CPU-only host: deck parsing and problem setups (bitwise equal to
 the oracle's restatement of pgen.hpp), error handling of bad decks, parameter echo."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -73,7 +75,6 @@ def test_overrides_win_over_file():
     (["time.integrator=rk7"], "unknown integrator"),
     (["mesh.nghost=2"], "nghost too small"),
     (["problem.name=nothere"], "unknown problem"),
-    (["mhd.characteristic=true"], "not supported"),
     (["diffusion.eta=0.1"], "not supported"),
 ])
 def test_bad_decks_raise(ov, msg):
@@ -84,3 +85,20 @@ def test_bad_decks_raise(ov, msg):
 def test_missing_required_key():
     with pytest.raises(DeckError, match="missing parameter mesh.nx1"):
         Deck("[problem]\nname = cpaw\n")
+
+
+def test_characteristic_key_and_its_limits():
+    """mhd.characteristic (main.cpp:123) reaches the configuration; the device accepts it in FP64
+    and refuses it in mixed precision (checked without a GPU through chq_bytes_for)."""
+    from chasqui_amd import _capi
+    d = Deck(deck("orszag_tang.in"), ["mhd.characteristic=true", "mhd.reconstruct=ppmx"])
+    assert d.config.characteristic == 1
+    L = _capi.lib()
+    md, cfg = d.mesh_desc, d.config
+    assert L.chq_bytes_for(C.byref(md), C.byref(cfg)) > 0
+    cfg.precision = 1
+    assert L.chq_bytes_for(C.byref(md), C.byref(cfg)) == 0
+    assert "characteristic" in L.chq_last_error().decode()
+    cfg.recon = 0   # donor cell ignores the switch (physics.hpp:71)
+    assert L.chq_bytes_for(C.byref(md), C.byref(cfg)) > 0
+

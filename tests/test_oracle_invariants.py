@@ -32,6 +32,11 @@ def totals(o):
     ("blast_3d.in", ["mesh.nx1=24", "mesh.nx2=24", "mesh.nx3=24"], 15),
     ("orszag_tang.in", ["mesh.nx1=48", "mesh.nx2=48"], 60),
     ("cpaw_3d_oblique.in", ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12"], 10),
+    # the reference's recommended preset (inputs/recommended_mhd.in): characteristic PPMX,
+    # LHLLD, flattening, dual energy, fofc
+    ("orszag_tang.in", ["mesh.nx1=48", "mesh.nx2=48", "mhd.characteristic=true", "mhd.reconstruct=ppmx",
+                        "mhd.rsolver=lhlld", "mhd.dual_energy=true", "time.cfl=0.4"], 40),
+    ("blast_3d.in", ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=16", "mhd.characteristic=true"], 8),
 ])
 def test_divb_and_conservation(name, ov, steps):
     o = po.OracleSim(deck(name), ov)

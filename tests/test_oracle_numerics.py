@@ -190,3 +190,105 @@ def test_fast_speed_closed_form():
     bx2 = bn ** 2 / 2.0
     cf = np.sqrt(0.5 * ((a2 + b2) + np.sqrt((a2 + b2) ** 2 - 4 * a2 * bx2)))
     assert abs(po.fast_speed(s, bn, gamma) - cf) < 1e-14
+
+
+# ---------------------------------------------------------------- characteristic variables
+def _eigen_samples(n=20000, seed=7, gamma=5.0 / 3.0):
+    """Random primitive states with the degenerate families of tests/eigen_test.cpp:16-22:
+    no transverse field, no normal field, the hydro limit, and cf = cs = a."""
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(0.05, 3.0, n)
+    p = rng.uniform(0.05, 3.0, n)
+    u, v, w = (rng.uniform(-1.5, 1.5, n) for _ in range(3))
+    by, bz, bx = (rng.uniform(-1.5, 1.5, n) for _ in range(3))
+    case = np.arange(n) % 5
+    by[case == 1] = 0.0
+    bz[case == 1] = 0.0
+    bx[case == 2] = 0.0
+    bx[case == 3] = 0.0
+    by[case == 3] = 0.0
+    bz[case == 3] = 0.0
+    by[case == 4] = 0.0
+    bz[case == 4] = 0.0
+    bx[case == 4] = np.sqrt(gamma * p[case == 4])
+    return d, u, v, w, p, by, bz, bx
+
+
+def test_eigensystem_inverse_and_jacobian():
+    """The reference's own check (tests/eigen_test.cpp:24-51): L R = I and A R = R Lambda for
+    the primitive Jacobian A, to 1e-9, over 20000 states including the degenerate ones."""
+    g = 5.0 / 3.0
+    d, u, v, w, p, by, bz, bx = _eigen_samples(gamma=g)
+    L, R = po.eigensystem(d, p, by, bz, bx, g)
+    n = d.size
+    A = np.zeros((n, 7, 7))
+    A[:, 0, 0] = u; A[:, 0, 1] = d
+    A[:, 1, 1] = u; A[:, 1, 4] = 1.0 / d; A[:, 1, 5] = by / d; A[:, 1, 6] = bz / d
+    A[:, 2, 2] = u; A[:, 2, 5] = -bx / d
+    A[:, 3, 3] = u; A[:, 3, 6] = -bx / d
+    A[:, 4, 1] = g * p; A[:, 4, 4] = u
+    A[:, 5, 1] = by; A[:, 5, 2] = -bx; A[:, 5, 5] = u
+    A[:, 6, 1] = bz; A[:, 6, 3] = -bx; A[:, 6, 6] = u
+    asq, casq, ctsq = g * p / d, bx * bx / d, (by * by + bz * bz) / d
+    tsum, tdif = asq + casq + ctsq, asq + ctsq - casq
+    disc = np.sqrt(np.maximum(tdif * tdif + 4 * ctsq * casq, 0.0))
+    cf = np.sqrt(0.5 * (tsum + disc))
+    cs = np.sqrt(np.maximum(0.5 * (tsum - disc), 0.0))
+    ca = np.sqrt(casq)
+    lam = np.stack([u - cf, u - ca, u - cs, u, u + cs, u + ca, u + cf], axis=1)
+    elr = np.abs(L @ R - np.eye(7)).max(axis=(1, 2))
+    ear = np.abs(A @ R - R * lam[:, None, :]).max(axis=(1, 2))
+    assert elr.max() < 1e-9, elr.max()
+    assert ear.max() < 1e-9, ear.max()
+
+
+def test_eigensystem_closed_forms():
+    """Hydro limit (B = 0): the acoustic eigenvectors of Euler in primitive variables, and the
+    entropy wave row L[3] = (1, 0, 0, 0, -1/a^2, 0, 0)."""
+    g = 1.4
+    d, p = 1.3, 0.7
+    L, R = po.eigensystem(d, p, 0.0, 0.0, 0.0, g)
+    L, R = L[0], R[0]
+    a2 = g * p / d
+    a = np.sqrt(a2)
+    assert np.allclose(R[:, 6], [d, a, 0, 0, d * a2, 0, 0], rtol=1e-15, atol=0)
+    assert np.allclose(R[:, 0], [d, -a, 0, 0, d * a2, 0, 0], rtol=1e-15, atol=0)
+    assert np.array_equal(L[3], [1.0, 0, 0, 0, -1.0 / a2, 0, 0])
+    assert np.allclose(L @ R, np.eye(7), atol=1e-15)
+
+
+def test_charac_reconstruction_of_constant_and_linear_data():
+    """R L = I, so a constant stencil reconstructs to the state itself and, for PLM and PPM, a
+    linear profile (a single slope, not limited) to the primitive reconstruction's values, up
+    to round-off; a transposed L or R fails this."""
+    rng = np.random.default_rng(3)
+    n = 200
+    base = np.stack([rng.uniform(0.5, 2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n),
+                     rng.uniform(0.3, 2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)], axis=1)
+    bx = rng.uniform(-1, 1, n)
+    const = np.repeat(base[:, :, None], 6, axis=2)
+    for m in ("plm", "ppm", "ppmx", "wenoz"):
+        for side in (0, 1):
+            out = po.charac_side(m, const, bx, side)
+            assert np.max(np.abs(out - base)) < 1e-13, m
+    slope = rng.uniform(-1e-3, 1e-3, (n, 7))
+    lin = base[:, :, None] + slope[:, :, None] * (np.arange(6) - 2.5)[None, None, :]
+    for m in ("plm", "ppm"):
+        for side in (0, 1):
+            out = po.charac_side(m, lin, bx, side)
+            prim = np.stack([np.array(po.reconstruct(m, lin[:, v, :])[side]) for v in range(7)], axis=1)
+            assert np.max(np.abs(out - prim)) < 1e-12, (m, side, np.max(np.abs(out - prim)))
+
+
+def test_charac_limits_a_single_wave_like_the_primitive_scheme():
+    """A jump carried by one characteristic field (a right eigenvector times a step) is, in
+    characteristic variables, a scalar step in that field alone: PPM then gives the cell
+    values exactly (no overshoot in any primitive), where the primitive scheme mixes."""
+    d, p, by, bz, bx, g = 1.0, 0.6, 0.8, -0.3, 0.5, 5.0 / 3.0
+    L, R = po.eigensystem(d, p, by, bz, bx, g)
+    q0 = np.array([d, 0.1, -0.2, 0.05, p, by, bz])
+    step = np.array([0, 0, 0, 1.0, 1.0, 1.0]) * 1e-6    # a tiny jump: the eigenvectors barely change
+    st = q0[:, None] + R[0][:, 6][:, None] * step[None, :]
+    for side, cell in ((0, 2), (1, 3)):
+        out = po.charac_side("ppm", st[None], np.array([bx]), side)[0]
+        assert np.max(np.abs(out - st[:, cell])) < 1e-13
