@@ -185,13 +185,14 @@ __global__ void __launch_bounds__(256) k_fluxc_x(const FluxArgsT<double> a, cons
   constexpr int H = W ? 3 : 2;
   constexpr int FPW = fpw_v<RC>;
   const int lane = int(threadIdx.x) & 63;
-  const int seg = wave % nseg, row = wave / nseg;
-  const int nrow_j = a.ju - a.jl + 1;
-  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
   const Geom &g = a.g;
-  const int F0 = a.il + seg * FPW;
-  int ci = F0 - H + lane;
-  ci = ci < 0 ? 0 : (ci > g.ni - 1 ? g.ni - 1 : ci);
+  // the same lane -> cell map as the primitive x kernel (rows laid end to end)
+  const int nrow_j = a.ju - a.jl + 1;
+  const long long ncell = (long long)nseg * nrow_j * (a.ku - a.kl + 1);
+  const long long v = (long long)wave * FPW - H + lane;
+  const long long vc = v < 0 ? 0 : (v >= ncell ? ncell - 1 : v);
+  const int row = int(vc / nseg), ci = int(vc - (long long)row * nseg);
+  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
   const long long row0 = g.at(k, j, 0);
   const double *src[NV];
   var_ptrs<double, 0, NV>(a, src);
@@ -223,9 +224,8 @@ __global__ void __launch_bounds__(256) k_fluxc_x(const FluxArgsT<double> a, cons
   double Lface[NV];
 #pragma unroll
   for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
-  const int i = F0 - H + lane;
-  if (lane < H || lane >= H + FPW || i > a.iu) return;
-  solve_store<double, 0, RS, ENT, NV>(a, row0 + i, 1, Lface, Rf);
+  if (lane < H || lane >= H + FPW || v != vc || ci < a.il || ci > a.iu) return;
+  solve_store<double, 0, RS, ENT, NV>(a, row0 + ci, 1, Lface, Rf);
 }
 
 // ---------------------------------------------------------------- y/z faces: register march
@@ -235,17 +235,15 @@ __global__ void __launch_bounds__(BX * BY) k_fluxc_m(const FluxArgsT<double> a) 
   constexpr int NV = ENT ? 8 : 7;
   constexpr bool W = wide_v<RC>;
   const Geom &g = a.g;
-  const int i = a.il + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
-  const int t = nt == 1 ? 0 : int(blockIdx.y) * BY + int(threadIdx.y);
-  const int sg = nt == 1 ? int(blockIdx.z) * BY + int(threadIdx.y) : int(blockIdx.z);
+  int i, t, sg;
+  if (!m_item<double, DIR>(a, (long long)blockIdx.x * (BX * BY) + int(threadIdx.y) * BX + int(threadIdx.x), i, t, sg))
+    return;
   const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
   const int F0 = lo_d + sg * SEG;
   const int F1 = min(F0 + SEG - 1, hi_d);
-  if (i > a.iu || F0 > hi_d) return;
   int j, k;
-  if constexpr (DIR == 1) { k = a.kl + t; j = 0; if (k > a.ku) return; }
-  else { j = a.jl + t; k = 0; if (j > a.ju) return; }
+  if constexpr (DIR == 1) { k = a.kl + t; j = 0; }
+  else { j = a.jl + t; k = 0; }
   const long long sd = DIR == 1 ? g.pitch : g.plane;
   const long long base = DIR == 1 ? g.at(k, 0, i) : g.at(0, j, i);
   const double *src[NV];

@@ -6,11 +6,13 @@
 // Reference: kernels::Fluxes<DIR,R,S> (physics.hpp:65-226) and fofc::RepairFluxes<DIR>
 // (fofc.hpp:46-96). L/R states never leave registers. Each cell's two half-states are
 // computed once (recon_cell.hpp) and shared by its two faces:
-//   * x faces (DIR 0): one wave64 per row segment, lane = cell; neighbour values move
-//     between lanes with cross-lane shuffles; a wave yields 64 - 5 faces (halo 3 + 2).
+//   * x faces (DIR 0): lane = cell over the rows' cells laid end to end; neighbour values
+//     move between lanes with cross-lane shuffles; a wave yields 64 - 5 faces (halo 3 + 2)
+//     and may span the end of one row and the start of the next.
 //   * y and z faces (DIR 1, 2): lane = i (coalesced loads), each thread marches along j or
 //     k over a segment of SEG faces with a register window: every primitive is loaded
-//     once per segment and each cell is reconstructed once.
+//     once per segment and each cell is reconstructed once. Threads are numbered with i
+//     fastest over the exact face range, so no lane idles at a row end.
 // For T = double the arithmetic and its order are the reference's: fluxes are bitwise equal.
 #pragma once
 #include "kernels.hpp"
@@ -110,13 +112,15 @@ __device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nse
   constexpr int H = W ? 3 : 2;            // lanes before the first face: cell halo + 1
   constexpr int FPW = fpw_v<RC>;
   const int lane = int(threadIdx.x) & 63;
-  const int seg = wave % nseg, row = wave / nseg;
-  const int nrow_j = a.ju - a.jl + 1;
-  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
   const Geom &g = a.g;
-  const int F0 = a.il + seg * FPW;
-  int ci = F0 - H + lane;                 // this lane's cell
-  ci = ci < 0 ? 0 : (ci > g.ni - 1 ? g.ni - 1 : ci);   // halo lanes past the row end: any valid cell
+  // this lane's cell in the rows laid end to end (nseg = cells per row); lanes before the
+  // first or past the last cell read any valid cell and compute no face
+  const int nrow_j = a.ju - a.jl + 1;
+  const long long ncell = (long long)nseg * nrow_j * (a.ku - a.kl + 1);
+  const long long v = (long long)wave * FPW - H + lane;
+  const long long vc = v < 0 ? 0 : (v >= ncell ? ncell - 1 : v);
+  const int row = int(vc / nseg), ci = int(vc - (long long)row * nseg);
+  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
   const long long row0 = g.at(k, j, 0);
   const T *src[NV];
   var_ptrs<T, 0, NV>(a, src);
@@ -171,9 +175,10 @@ __device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nse
   T Lface[NV];
 #pragma unroll
   for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
-  const int i = F0 - H + lane;
-  if (lane < H || lane >= H + FPW || i > a.iu) return;
-  solve_store<T, 0, RS, ENT, NV>(a, row0 + i, 1, Lface, Rf);
+  // face ci (between cells ci - 1 and ci): its stencil ci - H .. ci + 2 lies in this row and in
+  // this wave's lanes
+  if (lane < H || lane >= H + FPW || v != vc || ci < a.il || ci > a.iu) return;
+  solve_store<T, 0, RS, ENT, NV>(a, row0 + ci, 1, Lface, Rf);
 }
 
 template <class T, Recon RC, RSolver RS, bool ENT>
@@ -186,22 +191,19 @@ __global__ void __launch_bounds__(256) k_flux_x(const FluxArgsT<T> a, const int 
 
 // ---------------------------------------------------------------- y/z faces: register march
 template <class T, int DIR, Recon RC, RSolver RS, bool ENT>
-__device__ __forceinline__ void flux_m_tile(const FluxArgsT<T> &a, const int bx, const int by, const int bz) {
+__device__ __forceinline__ void flux_m_tile(const FluxArgsT<T> &a, const int bx) {
   constexpr int NV = ENT ? 8 : 7;
   constexpr bool W = wide_v<RC>;
   const Geom &g = a.g;
-  const int i = a.il + bx * BX + int(threadIdx.x);
-  const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
-  // transverse index (k for DIR 1, j for DIR 2) and segment along DIR
-  const int t = nt == 1 ? 0 : by * BY + int(threadIdx.y);
-  const int sg = nt == 1 ? bz * BY + int(threadIdx.y) : bz;
+  // i, transverse index t (k for DIR 1, j for DIR 2) and segment along DIR
+  int i, t, sg;
+  if (!m_item<T, DIR>(a, (long long)bx * (BX * BY) + int(threadIdx.y) * BX + int(threadIdx.x), i, t, sg)) return;
   const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
   const int F0 = lo_d + sg * SEG;
   const int F1 = min(F0 + SEG - 1, hi_d);
-  if (i > a.iu || F0 > hi_d) return;
   int j, k;
-  if constexpr (DIR == 1) { k = a.kl + t; j = 0; if (k > a.ku) return; }
-  else { j = a.jl + t; k = 0; if (j > a.ju) return; }
+  if constexpr (DIR == 1) { k = a.kl + t; j = 0; }
+  else { j = a.jl + t; k = 0; }
   const long long sd = DIR == 1 ? g.pitch : g.plane;
   const long long base = DIR == 1 ? g.at(k, 0, i) : g.at(0, j, i);   // index of (d = 0)
   const T *src[NV];
@@ -309,7 +311,7 @@ __device__ __forceinline__ void flux_m_tile(const FluxArgsT<T> &a, const int bx,
 template <class T, int DIR, Recon RC, RSolver RS, bool ENT>
 __global__ void __launch_bounds__(BX * BY) k_flux_m(const FluxArgsT<T> a) {
   if (!a.ctrl->active) return;
-  flux_m_tile<T, DIR, RC, RS, ENT>(a, int(blockIdx.x), int(blockIdx.y), int(blockIdx.z));
+  flux_m_tile<T, DIR, RC, RS, ENT>(a, int(blockIdx.x));
 }
 
 // Donor-cell HLLE flux and face EMFs of one face from the stage-input primitives
@@ -438,10 +440,7 @@ __global__ void __launch_bounds__(BX * BY) kb_flux_m(const RunArgs<T> *R, const 
     const Ctrl *ctl = R[m.run].ctrl;
     if (!ctl->active) continue;
     const FluxArgsT<T> a = R[m.run].flux[cur][ctl->wcur][DIR];
-    int gx, gy, gz;
-    m_shape<T, DIR>(a, gx, gy, gz);
-    const int bx = m.tile % gx, rest = m.tile / gx;
-    flux_m_tile<T, DIR, RC, RS, false>(a, bx, rest % gy, rest / gy);
+    flux_m_tile<T, DIR, RC, RS, false>(a, m.tile);
   }
 }
 

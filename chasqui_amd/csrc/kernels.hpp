@@ -128,11 +128,13 @@ inline __host__ __device__ long long bc_points(const Geom &g, int d) {
   return (long long)g.ng * (g.nj + 1) * (g.ni + 1);
 }
 
-// face-flux decomposition (flux_impl.hpp): x faces by wave64 row segments, y/z faces by
-// BX x BY thread tiles marching SEG faces along the direction
+// face-flux decomposition (flux_impl.hpp): x faces by wave64 windows over the rows' cells laid
+// end to end, y/z faces by threads marching SEG faces along the direction; both linearised so
+// that no lane idles at the end of a row (a 258-wide row in 64-wide tiles would leave a fifth
+// of the lanes idle)
 namespace fluxk {
 constexpr int BX = 64, BY = 4;
-constexpr int SEG = 16;   // faces per thread in the y/z march
+constexpr int SEG = 24;   // faces per thread in the y/z march (measured at 256^3: 24 beats 8, 16, 32)
 
 template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
 inline __host__ __device__ int fpw_of(Recon rc) {
@@ -141,24 +143,38 @@ inline __host__ __device__ int fpw_of(Recon rc) {
 }
 template <Recon RC> constexpr int fpw_v = 64 - (wide_v<RC> ? 3 : 2) - (wide_v<RC> ? 2 : 1);   // faces per wave
 
-// x-face work: nseg row segments of FPW faces per row, one wave each; 4 waves per block
+// x-face work: the cells of all rows (ni per row, rows (j, k) of the face range) laid end to
+// end; wave w holds cells w * fpw - H ... w * fpw - H + 63 and computes the faces of its middle
+// fpw lanes; 4 waves per block. nseg = ni (cells per row).
 template <class T>
 __host__ __device__ inline void x_shape(const FluxArgsT<T> &a, int fpw, int &nseg, int &nwaves) {
-  nseg = div_up(a.iu - a.il + 1, fpw);
-  nwaves = nseg * (a.ju - a.jl + 1) * (a.ku - a.kl + 1);
+  nseg = a.g.ni;
+  const long long cells = (long long)nseg * (a.ju - a.jl + 1) * (a.ku - a.kl + 1);
+  nwaves = int((cells + fpw - 1) / fpw);
 }
 
-// y/z-face work: tiles of BX faces in i x BY rows x SEG faces along DIR. When there is one
-// transverse row (2D y faces, DIR 1 with one k plane) the BY thread rows take consecutive
-// segments instead, so no thread row idles.
+// y/z-face work: one thread per (i, transverse row, segment of SEG faces along DIR), i fastest,
+// in blocks of BX x BY threads (gx blocks; gy = gz = 1).
 template <class T, int DIR>
 __host__ __device__ inline void m_shape(const FluxArgsT<T> &a, int &gx, int &gy, int &gz) {
   const int nx = a.iu - a.il + 1;
   const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
   const int nd = DIR == 1 ? a.ju - a.jl + 1 : a.ku - a.kl + 1;
-  gx = div_up(nx, BX);
-  if (nt == 1) { gy = 1; gz = div_up(div_up(nd, SEG), BY); }
-  else { gy = div_up(nt, BY); gz = div_up(nd, SEG); }
+  gx = int(((long long)nx * nt * div_up(nd, SEG) + BX * BY - 1) / (BX * BY));
+  gy = 1;
+  gz = 1;
+}
+// thread `lin` of the y/z decomposition: i, transverse index t and segment sg (false: no work)
+template <class T, int DIR>
+__host__ __device__ inline bool m_item(const FluxArgsT<T> &a, long long lin, int &i, int &t, int &sg) {
+  const int nx = a.iu - a.il + 1;
+  const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
+  const int nd = DIR == 1 ? a.ju - a.jl + 1 : a.ku - a.kl + 1;
+  const long long r = lin / nx;
+  i = a.il + int(lin - r * nx);
+  t = int(r % nt);
+  sg = int(r / nt);
+  return sg < div_up(nd, SEG);
 }
 
 }  // namespace fluxk
