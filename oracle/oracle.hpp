@@ -10,8 +10,8 @@
 // cites the reference file:line it follows. Min-reductions and integer counts are exact,
 // so results do not depend on the OpenMP thread count.
 //
-// Not restated (out of scope of the stage pipeline): diffusion, gravity, cooling,
-// super-time-stepping, the positivity flux limiter.
+// Not restated (out of scope of the stage pipeline): gravity, cooling, the positivity flux
+// limiter.
 // Requesting any of them raises an error instead of silently running something else.
 #pragma once
 #include <cstdint>
@@ -117,6 +117,9 @@ struct Cfg {   // physics.hpp:22-33 + the driver switches of main.cpp:112-163 th
   Bc bc[3] = {Bc::periodic, Bc::periodic, Bc::periodic};
   bool fofc = true;
   bool characteristic = false;   // physics.hpp:30
+  Real nu = 0.0, kappa = 0.0, eta = 0.0;   // diffusion::Coeffs (diffusion.hpp:18-21)
+  bool sts = false;                        // RKL2 super-time-stepping of the diffusive terms
+  bool diffusive() const { return nu > 0.0 || kappa > 0.0 || eta > 0.0; }
 };
 
 // ---------------------------------------------------------------- mesh (mesh.hpp:13-71)
@@ -402,11 +405,9 @@ inline void update(Grid &m, int out, Real g0, Real g1, Real bdt) {
 
 // ---------------------------------------------------------------- boundaries
 // physics.hpp:354-432: x1, then x2, then x3 (so edges and corners are filled).
-inline void boundaries(Grid &m, const Cfg &cfg, int s) {
+inline void boundaries_on(Grid &m, const Cfg &cfg, A4 &u, A3 &b1, A3 &b2, A3 &b3) {
   const int ng = m.ng, nx1 = m.nx1, nx2 = m.nx2, nx3 = m.nx3;
   const int is = m.is, ie = m.ie, js = m.js, je = m.je, ks = m.ks, ke = m.ke, ni = m.ni, nj = m.nj, nk = m.nk;
-  A4 &u = m.u(s);
-  A3 &b1 = m.bf(s, 0), &b2 = m.bf(s, 1), &b3 = m.bf(s, 2);
   {
     const int bc = int(cfg.bc[0]);
 #pragma omp parallel for schedule(static)
@@ -483,16 +484,17 @@ inline void boundaries(Grid &m, const Cfg &cfg, int s) {
         }
   }
 }
+inline void boundaries(Grid &m, const Cfg &cfg, int s) { boundaries_on(m, cfg, m.u(s), m.bf(s, 0), m.bf(s, 1), m.bf(s, 2)); }
 
 // ---------------------------------------------------------------- con2prim + dt
 // physics.hpp:437-494. Rewrites u[IENT] (and u[IEN] with dual energy) in place.
-inline Real cons_to_prim(Grid &m, const Cfg &cfg, int s, bool compute_dt) {
+inline Real cons_to_prim_on(Grid &m, const Cfg &cfg, A4 &u, const A3 &b1, const A3 &b2, const A3 &b3,
+                            bool compute_dt) {
   const Eos eos = cfg.eos;
   const int is = m.is, ie = m.ie, js = m.js, je = m.je, ks = m.ks, ke = m.ke;
   const bool two_d = m.two_d, three_d = m.three_d;
   const Real dx1 = m.dx1, dx2 = m.dx2, dx3 = m.dx3;
-  A4 &u = m.u(s), &w = m.w, &bcc = m.bcc, &ecc = m.ecc;
-  const A3 &b1 = m.bf(s, 0), &b2 = m.bf(s, 1), &b3 = m.bf(s, 2);
+  A4 &w = m.w, &bcc = m.bcc, &ecc = m.ecc;
   Real mdt = 1e300;
 #pragma omp parallel for collapse(2) schedule(static) reduction(min : mdt)
   for (int k = 0; k < m.nk; ++k)
@@ -540,6 +542,168 @@ inline Real cons_to_prim(Grid &m, const Cfg &cfg, int s, bool compute_dt) {
       }
   return mdt;
 }
+inline Real cons_to_prim(Grid &m, const Cfg &cfg, int s, bool compute_dt) {
+  return cons_to_prim_on(m, cfg, m.u(s), m.bf(s, 0), m.bf(s, 1), m.bf(s, 2), compute_dt);
+}
+
+// ---------------------------------------------------------------- explicit diffusion
+// Viscosity, conduction and Ohmic resistivity added to the fluxes / edge EMFs of the same
+// stage (diffusion.hpp:1-176). Coefficients: nu, kappa, eta (a zero coefficient skips its term).
+struct Diff {
+  Real nu = 0.0, kappa = 0.0, eta = 0.0;
+  bool any() const { return nu > 0.0 || kappa > 0.0 || eta > 0.0; }
+};
+inline Diff diff_of(const Cfg &c) { return Diff{c.nu, c.kappa, c.eta}; }
+
+// current density on edges from the face fields, diffusion.hpp:25-47
+struct Curl {
+  const A3 &b1, &b2, &b3;
+  Real idx1, idx2, idx3;
+  bool two_d, three_d;
+  Real J1(int k, int j, int i) const {
+    Real j1 = 0.0;
+    if (two_d) j1 += (b3(k, j, i) - b3(k, j - 1, i)) * idx2;
+    if (three_d) j1 -= (b2(k, j, i) - b2(k - 1, j, i)) * idx3;
+    return j1;
+  }
+  Real J2(int k, int j, int i) const {
+    Real j2 = -(b3(k, j, i) - b3(k, j, i - 1)) * idx1;
+    if (three_d) j2 += (b1(k, j, i) - b1(k - 1, j, i)) * idx3;
+    return j2;
+  }
+  Real J3(int k, int j, int i) const {
+    Real j3 = (b2(k, j, i) - b2(k, j, i - 1)) * idx1;
+    if (two_d) j3 -= (b1(k, j, i) - b1(k, j - 1, i)) * idx2;
+    return j3;
+  }
+};
+inline Curl curl_of(const Grid &m, const A3 &b1, const A3 &b2, const A3 &b3) {
+  return Curl{b1, b2, b3, 1.0 / m.dx1, 1.0 / m.dx2, 1.0 / m.dx3, m.two_d, m.three_d};
+}
+
+// diffusive fluxes of one face (k, j, i) of direction DIR added to f{DIR+1},
+// FaceCtx::Add (diffusion.hpp:73-128)
+template <int DIR>
+void diff_face(Grid &m, const Diff &c, const Curl &J, int k, int j, int i) {
+  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
+  constexpr int t1 = (DIR + 1) % 3, t2 = (DIR + 2) % 3;
+  constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  constexpr int di1 = t1 == 0, dj1 = t1 == 1, dk1 = t1 == 2, di2 = t2 == 0, dj2 = t2 == 1, dk2 = t2 == 2;
+  const Real idx[3] = {1.0 / m.dx1, 1.0 / m.dx2, 1.0 / m.dx3};
+  const Real idn = idx[DIR], idt1 = idx[t1], idt2 = idx[t2];
+  const bool has[3] = {true, m.two_d, m.three_d};
+  const bool has_t1 = has[t1], has_t2 = has[t2];
+  const A4 &w = m.w, &bcc = m.bcc;
+  A4 &flx = DIR == 0 ? m.f1 : DIR == 1 ? m.f2 : m.f3;
+  const int kL = k - dk, jL = j - dj, iL = i - di;
+  if (c.nu > 0.0) {
+    const Real mu = c.nu * 0.5 * (w(IDN, kL, jL, iL) + w(IDN, k, j, i));
+    const Real dun = (w(IU, k, j, i) - w(IU, kL, jL, iL)) * idn;
+    const Real dvn = (w(IV, k, j, i) - w(IV, kL, jL, iL)) * idn;
+    const Real dwn = (w(IW, k, j, i) - w(IW, kL, jL, iL)) * idn;
+    Real dut1 = 0.0, dvt1 = 0.0, dut2 = 0.0, dwt2 = 0.0;
+    if (has_t1) {
+      dut1 = 0.25 * idt1 * ((w(IU, k + dk1, j + dj1, i + di1) - w(IU, k - dk1, j - dj1, i - di1))
+                            + (w(IU, kL + dk1, jL + dj1, iL + di1) - w(IU, kL - dk1, jL - dj1, iL - di1)));
+      dvt1 = 0.25 * idt1 * ((w(IV, k + dk1, j + dj1, i + di1) - w(IV, k - dk1, j - dj1, i - di1))
+                            + (w(IV, kL + dk1, jL + dj1, iL + di1) - w(IV, kL - dk1, jL - dj1, iL - di1)));
+    }
+    if (has_t2) {
+      dut2 = 0.25 * idt2 * ((w(IU, k + dk2, j + dj2, i + di2) - w(IU, k - dk2, j - dj2, i - di2))
+                            + (w(IU, kL + dk2, jL + dj2, iL + di2) - w(IU, kL - dk2, jL - dj2, iL - di2)));
+      dwt2 = 0.25 * idt2 * ((w(IW, k + dk2, j + dj2, i + di2) - w(IW, k - dk2, j - dj2, i - di2))
+                            + (w(IW, kL + dk2, jL + dj2, iL + di2) - w(IW, kL - dk2, jL - dj2, iL - di2)));
+    }
+    const Real divv = dun + dvt1 + dwt2;
+    const Real tnn = mu * (2.0 * dun - (2.0 / 3.0) * divv);
+    const Real tnt1 = mu * (dvn + dut1);
+    const Real tnt2 = mu * (dwn + dut2);
+    const Real uf = 0.5 * (w(IU, kL, jL, iL) + w(IU, k, j, i));
+    const Real vf = 0.5 * (w(IV, kL, jL, iL) + w(IV, k, j, i));
+    const Real wf = 0.5 * (w(IW, kL, jL, iL) + w(IW, k, j, i));
+    flx(IU, k, j, i) -= tnn;
+    flx(IV, k, j, i) -= tnt1;
+    flx(IW, k, j, i) -= tnt2;
+    flx(IEN, k, j, i) -= tnn * uf + tnt1 * vf + tnt2 * wf;
+  }
+  if (c.kappa > 0.0) {
+    const Real tL = w(IPR, kL, jL, iL) / w(IDN, kL, jL, iL), tR = w(IPR, k, j, i) / w(IDN, k, j, i);
+    flx(IEN, k, j, i) -= c.kappa * (tR - tL) * idn;
+  }
+  if (c.eta > 0.0) {
+    Real et1, et2;
+    if (DIR == 0) {
+      et1 = c.eta * 0.5 * (J.J2(k, j, i) + (has_t2 ? J.J2(k + 1, j, i) : J.J2(k, j, i)));
+      et2 = c.eta * 0.5 * (J.J3(k, j, i) + (has_t1 ? J.J3(k, j + 1, i) : J.J3(k, j, i)));
+    } else if (DIR == 1) {
+      et1 = c.eta * 0.5 * (J.J3(k, j, i) + J.J3(k, j, i + 1));
+      et2 = c.eta * 0.5 * (J.J1(k, j, i) + (has_t1 ? J.J1(k + 1, j, i) : J.J1(k, j, i)));
+    } else {
+      et1 = c.eta * 0.5 * (J.J1(k, j, i) + J.J1(k, j + 1, i));
+      et2 = c.eta * 0.5 * (J.J2(k, j, i) + J.J2(k, j, i + 1));
+    }
+    const Real bt1 = 0.5 * (bcc(t1, kL, jL, iL) + bcc(t1, k, j, i));
+    const Real bt2 = 0.5 * (bcc(t2, kL, jL, iL) + bcc(t2, k, j, i));
+    flx(IEN, k, j, i) += et1 * bt2 - et2 * bt1;
+  }
+}
+
+// the face range of the diffusive fluxes: active faces only (diffusion.hpp:131-137)
+template <int DIR>
+bool diff_face_active(const Grid &m, int k, int j, int i) {
+  if (DIR == 0) return i >= m.is && i <= m.ie + 1 && j >= m.js && j <= m.je && k >= m.ks && k <= m.ke;
+  if (DIR == 1) return j >= m.js && j <= m.je + 1 && i >= m.is && i <= m.ie && k >= m.ks && k <= m.ke;
+  return k >= m.ks && k <= m.ke + 1 && i >= m.is && i <= m.ie && j >= m.js && j <= m.je;
+}
+
+template <int DIR>
+void diffusion_fluxes(Grid &m, const Diff &c, const A3 &b1, const A3 &b2, const A3 &b3) {   // diffusion.hpp:131-138
+  if (!c.any()) return;
+  const Curl J = curl_of(m, b1, b2, b3);
+  const int iu = DIR == 0 ? m.ie + 1 : m.ie, ju = DIR == 1 ? m.je + 1 : m.je, ku = DIR == 2 ? m.ke + 1 : m.ke;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = m.ks; k <= ku; ++k)
+    for (int j = m.js; j <= ju; ++j)
+      for (int i = m.is; i <= iu; ++i) diff_face<DIR>(m, c, J, k, j, i);
+}
+inline void diffusion_fluxes_all(Grid &m, const Diff &c, const A3 &b1, const A3 &b2, const A3 &b3) {
+  diffusion_fluxes<0>(m, c, b1, b2, b3);
+  if (m.two_d) diffusion_fluxes<1>(m, c, b1, b2, b3);
+  if (m.three_d) diffusion_fluxes<2>(m, c, b1, b2, b3);
+}
+
+// eta J added to the edge EMFs, diffusion.hpp:141-156
+inline void resistive_emf(Grid &m, const Diff &c, const A3 &b1, const A3 &b2, const A3 &b3) {
+  if (c.eta <= 0.0) return;
+  const Curl J = curl_of(m, b1, b2, b3);
+  const int ku = m.three_d ? m.ke + 1 : m.ks + 1, ju = m.two_d ? m.je + 1 : m.js + 1;
+  const int ie = m.ie, je = m.je, ke = m.ke;
+  const bool two_d = m.two_d, three_d = m.three_d;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = m.ks; k <= ku; ++k)
+    for (int j = m.js; j <= ju; ++j)
+      for (int i = m.is; i <= ie + 1; ++i) {
+        if (i <= ie) m.e1(k, j, i) += c.eta * J.J1(k, j, i);
+        if (j <= je || !two_d) m.e2(k, j, i) += c.eta * J.J2(k, j, i);
+        if (k <= ke || !three_d) m.e3(k, j, i) += c.eta * J.J3(k, j, i);
+      }
+}
+
+// parabolic time-step limit over the active cells, diffusion.hpp:159-174
+inline Real diffusion_dt(const Grid &m, const Cfg &cfg, const Diff &c) {
+  if (!c.any()) return 1e300;
+  const int ndim = 1 + (m.two_d ? 1 : 0) + (m.three_d ? 1 : 0);
+  Real dxmin = m.dx1;
+  if (m.two_d) dxmin = std::fmin(dxmin, m.dx2);
+  if (m.three_d) dxmin = std::fmin(dxmin, m.dx3);
+  Real rho_min = 1e300;
+  for (int k = m.ks; k <= m.ke; ++k)
+    for (int j = m.js; j <= m.je; ++j)
+      for (int i = m.is; i <= m.ie; ++i) rho_min = std::fmin(rho_min, m.w(IDN, k, j, i));
+  const Real chi = (c.kappa > 0) ? c.kappa * cfg.eos.gm1 / rho_min : 0.0;
+  const Real dmax = std::fmax(std::fmax(c.nu, c.eta), chi);
+  return dxmin * dxmin / (2.0 * ndim * dmax);
+}
 
 // ---------------------------------------------------------------- fofc
 // flag cells whose tentative rho or p is at/below the floors, fofc.hpp:17-44
@@ -574,8 +738,10 @@ inline int fofc_flag(Grid &m, const Cfg &cfg, int s) {
 
 // donor-cell HLLE fluxes on faces touching a flagged active cell, fofc.hpp:46-96
 // (diffusion is out of scope, so the diffusive re-add of fofc.hpp:89-93 never applies)
+// fofc.hpp:46-96; with diffusion coefficients dc the repaired active faces get their
+// diffusive fluxes again (dc is empty under super-time-stepping, main.cpp:82)
 template <int DIR>
-void fofc_repair(Grid &m, const Cfg &cfg, const A3 &bn) {
+void fofc_repair(Grid &m, const Cfg &cfg, const A3 &bn, const Diff &dc, const A3 &b1c, const A3 &b2c, const A3 &b3c) {
   const Eos eos = cfg.eos;
   constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
   constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
@@ -589,6 +755,8 @@ void fofc_repair(Grid &m, const Cfg &cfg, const A3 &bn) {
   A4 &fx = DIR == 0 ? m.f1 : DIR == 1 ? m.f2 : m.f3;
   A3 &et2 = DIR == 0 ? m.e3x1 : DIR == 1 ? m.e1x2 : m.e2x3;
   A3 &et1 = DIR == 0 ? m.e2x1 : DIR == 1 ? m.e3x2 : m.e1x3;
+  const bool diff = dc.any();
+  const Curl J = curl_of(m, b1c, b2c, b3c);
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = kl; k <= ku; ++k)
     for (int j = jl; j <= ju; ++j)
@@ -610,10 +778,14 @@ void fofc_repair(Grid &m, const Cfg &cfg, const A3 &bn) {
                                            : 0.5 * (f[F_D] * (w(IENT, kL, jL, iL) + w(IENT, k, j, i)));
         et2(k, j, i) = -f[F_BY];
         et1(k, j, i) = f[F_BZ];
+        if (diff && diff_face_active<DIR>(m, k, j, i)) diff_face<DIR>(m, dc, J, k, j, i);
       }
 }
-inline void fofc_repair_any(Grid &m, const Cfg &c, int dir, const A3 &bn) {
-  if (dir == 0) fofc_repair<0>(m, c, bn); else if (dir == 1) fofc_repair<1>(m, c, bn); else fofc_repair<2>(m, c, bn);
+inline void fofc_repair_any(Grid &m, const Cfg &c, int dir, int cur, const Diff &dc) {
+  const A3 &bn = m.bf(cur, dir), &b1 = m.bf(cur, 0), &b2 = m.bf(cur, 1), &b3 = m.bf(cur, 2);
+  if (dir == 0) fofc_repair<0>(m, c, bn, dc, b1, b2, b3);
+  else if (dir == 1) fofc_repair<1>(m, c, bn, dc, b1, b2, b3);
+  else fofc_repair<2>(m, c, bn, dc, b1, b2, b3);
 }
 
 // ---------------------------------------------------------------- RK stage and step
@@ -633,26 +805,167 @@ inline StageCoef stage_coef(int nstage, int s) {
   return {0, 1, 0.5, 1, 0};
 }
 
-// Stage(), main.cpp:46-93 (ideal MHD only)
+// Stage(), main.cpp:46-93 (explicit diffusion in the stage unless super-time-stepping)
 inline long stage(Grid &m, const Cfg &cfg, int cur, int out, Real g0, Real g1, Real bdt) {
-  fluxes_any(m, cfg, 0, m.bf(cur, 0));
-  if (m.two_d) fluxes_any(m, cfg, 1, m.bf(cur, 1));
-  if (m.three_d) fluxes_any(m, cfg, 2, m.bf(cur, 2));
+  const A3 &b1c = m.bf(cur, 0), &b2c = m.bf(cur, 1), &b3c = m.bf(cur, 2);
+  const Diff dc = cfg.sts ? Diff{} : diff_of(cfg);
+  fluxes_any(m, cfg, 0, b1c);
+  if (m.two_d) fluxes_any(m, cfg, 1, b2c);
+  if (m.three_d) fluxes_any(m, cfg, 2, b3c);
+  diffusion_fluxes_all(m, dc, b1c, b2c, b3c);                      // main.cpp:55-59
   corner_emf(m);
+  resistive_emf(m, dc, b1c, b2c, b3c);                             // main.cpp:66
   update(m, out, g0, g1, bdt);
   long nbad = 0;
   if (cfg.fofc) {
     nbad = fofc_flag(m, cfg, out);
     if (nbad > 0) {
-      fofc_repair_any(m, cfg, 0, m.bf(cur, 0));
-      if (m.two_d) fofc_repair_any(m, cfg, 1, m.bf(cur, 1));
-      if (m.three_d) fofc_repair_any(m, cfg, 2, m.bf(cur, 2));
+      fofc_repair_any(m, cfg, 0, cur, dc);
+      if (m.two_d) fofc_repair_any(m, cfg, 1, cur, dc);
+      if (m.three_d) fofc_repair_any(m, cfg, 2, cur, dc);
       corner_emf(m);
+      resistive_emf(m, dc, b1c, b2c, b3c);
       update(m, out, g0, g1, bdt);
     }
   }
   boundaries(m, cfg, out);
   return nbad;
+}
+
+// ---------------------------------------------------------------- RKL2 super-time-stepping
+// sts.hpp:1-157 (Meyer, Balsara & Aslam 2014): the parabolic operator M(U) = -div F_diff,
+// M(B) = -curl(eta J) advanced over the whole hyperbolic step tau with s Runge-Kutta-Legendre
+// stages after the RK step (operator split).
+struct StsWork {   // sts.hpp:16-32
+  bool allocated = false;
+  A4 y0, y1, y2, m0, mj;
+  A3 b0[3], b1[3], b2[3], mb0[3], mbj[3];
+  void allocate(const Grid &m) {
+    if (allocated) return;
+    for (A4 *p : {&y0, &y1, &y2, &m0, &mj}) p->alloc(NVAR, m.nk, m.nj, m.ni);
+    for (int c = 0; c < 3; ++c)
+      for (A3 *p : {&b0[c], &b1[c], &b2[c], &mb0[c], &mbj[c]}) p->alloc(m.nk + 1, m.nj + 1, m.ni + 1);
+    allocated = true;
+  }
+};
+
+inline int sts_stages(Real tau, Real dt_par) {   // sts.hpp:35-39
+  const double r = tau / dt_par;
+  const int s = static_cast<int>(std::ceil(0.5 * (-1.0 + std::sqrt(9.0 + 16.0 * r))));
+  return s < 2 ? 2 : s;
+}
+inline double sts_b(int j) { return j <= 2 ? 1.0 / 3.0 : (double(j) * j + j - 2.0) / (2.0 * j * (j + 1.0)); }
+
+// M(U, B) with the primitives already in m.w / m.bcc: -div F_diff into mu, -curl(eta J) into
+// mb (sts.hpp:43-90)
+inline void sts_operator(Grid &m, const Diff &dc, const A3 &b1c, const A3 &b2c, const A3 &b3c, A4 &mu, A3 &mb1,
+                         A3 &mb2, A3 &mb3) {
+  std::fill(m.f1.v.begin(), m.f1.v.end(), 0.0);
+  std::fill(m.f2.v.begin(), m.f2.v.end(), 0.0);
+  std::fill(m.f3.v.begin(), m.f3.v.end(), 0.0);
+  std::fill(m.e1.v.begin(), m.e1.v.end(), 0.0);
+  std::fill(m.e2.v.begin(), m.e2.v.end(), 0.0);
+  std::fill(m.e3.v.begin(), m.e3.v.end(), 0.0);
+  diffusion_fluxes_all(m, dc, b1c, b2c, b3c);
+  resistive_emf(m, dc, b1c, b2c, b3c);
+  const int is = m.is, ie = m.ie, js = m.js, je = m.je, ks = m.ks, ke = m.ke;
+  const bool two_d = m.two_d, three_d = m.three_d;
+  const Real idx1 = 1.0 / m.dx1, idx2 = 1.0 / m.dx2, idx3 = 1.0 / m.dx3;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = ks; k <= ke + 1; ++k)
+    for (int j = js; j <= je + 1; ++j)
+      for (int i = is; i <= ie + 1; ++i) {
+        if (i <= ie && j <= je && k <= ke) {
+          for (int n = 0; n < NVAR; ++n) {
+            Real d = (m.f1(n, k, j, i + 1) - m.f1(n, k, j, i)) * idx1;
+            if (two_d) d += (m.f2(n, k, j + 1, i) - m.f2(n, k, j, i)) * idx2;
+            if (three_d) d += (m.f3(n, k + 1, j, i) - m.f3(n, k, j, i)) * idx3;
+            mu(n, k, j, i) = -d;
+          }
+        }
+        if (j <= je && k <= ke) {
+          Real c = 0.0;
+          if (two_d) c += (m.e3(k, j + 1, i) - m.e3(k, j, i)) * idx2;
+          if (three_d) c -= (m.e2(k + 1, j, i) - m.e2(k, j, i)) * idx3;
+          mb1(k, j, i) = -c;
+        }
+        if (i <= ie && k <= ke && (two_d || j == js)) {
+          Real c = -(m.e3(k, j, i + 1) - m.e3(k, j, i)) * idx1;
+          if (three_d) c += (m.e1(k + 1, j, i) - m.e1(k, j, i)) * idx3;
+          mb2(k, j, i) = -c;
+          if (!two_d) mb2(k, j + 1, i) = -c;
+        }
+        if (i <= ie && j <= je && (three_d || k == ks)) {
+          Real c = (m.e2(k, j, i + 1) - m.e2(k, j, i)) * idx1;
+          if (two_d) c -= (m.e1(k, j + 1, i) - m.e1(k, j, i)) * idx2;
+          mb3(k, j, i) = -c;
+          if (!three_d) mb3(k + 1, j, i) = -c;
+        }
+      }
+}
+
+// out = ((a x + b y) + c z) + (d p + e q) on the active cells (sts.hpp:93-100) and on the face
+// box [ks..ke+1] x [js..je+1] x [is..ie+1] (sts.hpp:101-107)
+inline void sts_combine(const Grid &m, A4 &out, Real a, const A4 &x, Real b, const A4 &y, Real c, const A4 &z, Real d,
+                        const A4 &p, Real e, const A4 &q) {
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = m.ks; k <= m.ke; ++k)
+    for (int j = m.js; j <= m.je; ++j)
+      for (int i = m.is; i <= m.ie; ++i)
+        for (int n = 0; n < NVAR; ++n)
+          out(n, k, j, i) = ((a * x(n, k, j, i) + b * y(n, k, j, i)) + c * z(n, k, j, i)) +
+                            (d * p(n, k, j, i) + e * q(n, k, j, i));
+}
+inline void sts_combine_b(const Grid &m, A3 &out, Real a, const A3 &x, Real b, const A3 &y, Real c, const A3 &z, Real d,
+                          const A3 &p, Real e, const A3 &q) {
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = m.ks; k <= m.ke + 1; ++k)
+    for (int j = m.js; j <= m.je + 1; ++j)
+      for (int i = m.is; i <= m.ie + 1; ++i)
+        out(k, j, i) = ((a * x(k, j, i) + b * y(k, j, i)) + c * z(k, j, i)) + (d * p(k, j, i) + e * q(k, j, i));
+}
+
+// sts::Advance (sts.hpp:111-157): the diffusive terms over tau from (u0, b?f0), result left
+// there; u1 / b?f1 serve as the fourth rotating array. Returns the number of stages.
+inline int sts_advance(Grid &m, const Cfg &cfg, const Diff &dc, StsWork &wk, Real tau, Real dt_par) {
+  wk.allocate(m);
+  const int s = sts_stages(tau, dt_par);
+  const double w1 = 4.0 / (double(s) * s + s - 2.0);
+  wk.y0 = m.u0;
+  wk.b0[0] = m.b1f0; wk.b0[1] = m.b2f0; wk.b0[2] = m.b3f0;
+  cons_to_prim_on(m, cfg, wk.y0, wk.b0[0], wk.b0[1], wk.b0[2], false);
+  sts_operator(m, dc, wk.b0[0], wk.b0[1], wk.b0[2], wk.m0, wk.mb0[0], wk.mb0[1], wk.mb0[2]);
+  const Real mt1 = Real(sts_b(1) * w1) * tau;
+  sts_combine(m, wk.y1, 1, wk.y0, 0, wk.y0, 0, wk.y0, mt1, wk.m0, 0, wk.m0);
+  for (int c = 0; c < 3; ++c) sts_combine_b(m, wk.b1[c], 1, wk.b0[c], 0, wk.b0[c], 0, wk.b0[c], mt1, wk.mb0[c], 0, wk.mb0[c]);
+  boundaries_on(m, cfg, wk.y1, wk.b1[0], wk.b1[1], wk.b1[2]);
+  A4 *yjm2 = &wk.y0, *yjm1 = &wk.y1, *yj = &wk.y2;
+  A3 *bjm2[3] = {&wk.b0[0], &wk.b0[1], &wk.b0[2]}, *bjm1[3] = {&wk.b1[0], &wk.b1[1], &wk.b1[2]};
+  A3 *bjn[3] = {&wk.b2[0], &wk.b2[1], &wk.b2[2]};
+  A4 *spare = &m.u1;
+  A3 *spareb[3] = {&m.b1f1, &m.b2f1, &m.b3f1};
+  for (int j = 2; j <= s; ++j) {
+    const double b_j = sts_b(j), b_jm1 = sts_b(j - 1), b_jm2 = sts_b(j - 2);
+    const double mu = (2.0 * j - 1.0) / j * b_j / b_jm1;
+    const double nu = -(j - 1.0) / j * b_j / b_jm2;
+    const double mut = mu * w1;
+    const double gt = -(1.0 - b_jm1) * mut;
+    cons_to_prim_on(m, cfg, *yjm1, *bjm1[0], *bjm1[1], *bjm1[2], false);
+    sts_operator(m, dc, *bjm1[0], *bjm1[1], *bjm1[2], wk.mj, wk.mbj[0], wk.mbj[1], wk.mbj[2]);
+    sts_combine(m, *yj, Real(mu), *yjm1, Real(nu), *yjm2, Real(1.0 - mu - nu), wk.y0, Real(mut) * tau, wk.mj,
+                Real(gt) * tau, wk.m0);
+    for (int c = 0; c < 3; ++c)
+      sts_combine_b(m, *bjn[c], Real(mu), *bjm1[c], Real(nu), *bjm2[c], Real(1.0 - mu - nu), wk.b0[c],
+                    Real(mut) * tau, wk.mbj[c], Real(gt) * tau, wk.mb0[c]);
+    boundaries_on(m, cfg, *yj, *bjn[0], *bjn[1], *bjn[2]);
+    A4 *freed = j == 2 ? spare : yjm2;
+    A3 *freedb[3] = {j == 2 ? spareb[0] : bjm2[0], j == 2 ? spareb[1] : bjm2[1], j == 2 ? spareb[2] : bjm2[2]};
+    yjm2 = yjm1; yjm1 = yj; yj = freed;
+    for (int c = 0; c < 3; ++c) { bjm2[c] = bjm1[c]; bjm1[c] = bjn[c]; bjn[c] = freedb[c]; }
+  }
+  m.u0 = *yjm1;
+  m.b1f0 = *bjm1[0]; m.b2f0 = *bjm1[1]; m.b3f0 = *bjm1[2];
+  return s;
 }
 
 // ---------------------------------------------------------------- problem setups
@@ -834,6 +1147,39 @@ inline void pg_shock_tube(Deck &p, const Grid &m, Init &h) {
       }
 }
 
+// sinusoidal decay test of one diffusive term along one axis, pgen.hpp:235-266: var = b
+// (B_{axis+1} on faces, uniform total pressure), v (v_{axis+1}) or temp (T at uniform p)
+inline void pg_diffusion_test(Deck &p, const Grid &m, Init &h) {
+  const std::string var = p.str("problem", "var", "b");
+  const int axis = p.integer("problem", "axis", 1);
+  const Real amp = p.real("problem", "amp", 1.0e-3);
+  const Real d0 = p.real("problem", "dens", 1.0), p0 = p.real("problem", "pres", 1.0);
+  const Real L = axis == 1 ? (m.x1max - m.x1min) : axis == 2 ? (m.x2max - m.x2min) : (m.x3max - m.x3min);
+  const Real kw = 2.0 * kPi / L;
+  const int comp = axis % 3;
+  auto coord = [&](int kk, int j, int i) { return axis == 1 ? m.x1v(i) : axis == 2 ? m.x2v(j) : m.x3v(kk); };
+  for (int kk = 0; kk < m.nk + 1; ++kk)
+    for (int j = 0; j < m.nj + 1; ++j)
+      for (int i = 0; i < m.ni + 1; ++i) {
+        h.b1(kk, j, i) = 0.0; h.b2(kk, j, i) = 0.0; h.b3(kk, j, i) = 0.0;
+        if (var == "b") {
+          const Real val = amp * std::sin(kw * coord(kk, j, i));
+          if (comp == 0) h.b1(kk, j, i) = val; else if (comp == 1) h.b2(kk, j, i) = val; else h.b3(kk, j, i) = val;
+        }
+      }
+  for (int kk = 0; kk < m.nk; ++kk)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        const Real x = coord(kk, j, i);
+        Real d = d0, pr = p0, v[3] = {0.0, 0.0, 0.0};
+        if (var == "v") v[comp] = amp * std::sin(kw * x);
+        if (var == "temp") { const Real T = p0 / d0 + amp * std::sin(kw * x); d = p0 / T; }
+        if (var == "b") pr = p0 - 0.5 * sqr(amp * std::sin(kw * x));
+        h.w(IDN, kk, j, i) = d; h.w(IVX, kk, j, i) = v[0]; h.w(IVY, kk, j, i) = v[1]; h.w(IVZ, kk, j, i) = v[2];
+        h.w(IPR, kk, j, i) = pr;
+      }
+}
+
 // fill primitives and faces, convert to conserved (pgen.hpp:269-307)
 inline void initialize(const std::string &name, Deck &p, Grid &m, const Cfg &cfg) {
   Init h;
@@ -843,6 +1189,7 @@ inline void initialize(const std::string &name, Deck &p, Grid &m, const Cfg &cfg
   else if (name == "orszag_tang") pg_orszag_tang(p, m, h);
   else if (name == "blast") pg_blast(p, m, h);
   else if (name == "shock_tube") pg_shock_tube(p, m, h);
+  else if (name == "diffusion_test") pg_diffusion_test(p, m, h);
   else throw std::runtime_error("unknown problem: " + name);
   for (int k = 0; k < m.nk; ++k)
     for (int j = 0; j < m.nj; ++j)
@@ -876,7 +1223,8 @@ struct Sim {
   Grid m;
   Real t = 0.0, tlim = 1.0, min_dt = 0.0, last_dt = 0.0;
   int cycle = 0, nlim = -1;
-  long fofc_count = 0;
+  long fofc_count = 0, sts_count = 0;
+  StsWork sts_work;
   std::string problem;
   A4 u_init, bcc_init;
 
@@ -909,9 +1257,10 @@ struct Sim {
     cfg.bc[2] = parse_bc(p.str("mesh", "bc_x3", "periodic"));
     cfg.fofc = p.boolean("mhd", "fofc", true);
     if (p.boolean("mhd", "flux_limiter", false)) throw std::runtime_error("flux_limiter is not restated");
-    if (p.boolean("diffusion", "sts", false) || p.real("diffusion", "nu", 0.0) != 0.0 ||
-        p.real("diffusion", "kappa", 0.0) != 0.0 || p.real("diffusion", "eta", 0.0) != 0.0)
-      throw std::runtime_error("diffusion is not restated");
+    cfg.sts = p.boolean("diffusion", "sts", false);                  // main.cpp:142-146
+    cfg.nu = p.real("diffusion", "nu", 0.0);
+    cfg.kappa = p.real("diffusion", "kappa", 0.0);
+    cfg.eta = p.real("diffusion", "eta", 0.0);
     if (p.boolean("gravity", "uniform", false) || p.boolean("gravity", "point_mass", false) ||
         p.boolean("cooling", "enabled", false))
       throw std::runtime_error("source terms are not restated");
@@ -932,6 +1281,7 @@ struct Sim {
     if (pgen_only) return;
     boundaries(m, cfg, 0);
     min_dt = cons_to_prim(m, cfg, 0, true);
+    if (!cfg.sts) min_dt = std::fmin(min_dt, diffusion_dt(m, cfg, diff_of(cfg)));   // main.cpp:202
     u_init = m.u0;
     bcc_init = m.bcc;
   }
@@ -945,8 +1295,14 @@ struct Sim {
       const StageCoef c = stage_coef(cfg.nstage, s);
       const bool last = s == cfg.nstage;
       fofc_count += stage(m, cfg, c.cur, c.out, c.g0, c.g1, c.beta * dt);
-      const Real mdt = cons_to_prim(m, cfg, c.out, last);
-      if (last) min_dt = std::fmin(mdt, 1e300);
+      const bool split = last && cfg.sts;                              // main.cpp:273-275
+      const Real mdt = cons_to_prim(m, cfg, c.out, last && !split);
+      if (last && !split) min_dt = std::fmin(mdt, diffusion_dt(m, cfg, diff_of(cfg)));
+    }
+    if (cfg.sts) {                                                     // main.cpp:278-290
+      sts_count += sts_advance(m, cfg, diff_of(cfg), sts_work, dt, diffusion_dt(m, cfg, diff_of(cfg)));
+      boundaries(m, cfg, 0);
+      min_dt = cons_to_prim(m, cfg, 0, true);
     }
     t += dt;
     ++cycle;

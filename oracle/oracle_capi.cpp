@@ -119,7 +119,7 @@ double orc_cons_to_prim(void *p, int s, int compute_dt) { return cons_to_prim(S(
 int orc_fofc_flag(void *p, int s) { return fofc_flag(S(p)->m, S(p)->cfg, s); }
 int orc_fofc_repair(void *p, int dir, int cur) {
   Sim &s = *S(p);
-  fofc_repair_any(s.m, s.cfg, dir, s.m.bf(cur, dir));
+  fofc_repair_any(s.m, s.cfg, dir, cur, s.cfg.sts ? Diff{} : diff_of(s.cfg));
   return 0;
 }
 long orc_stage(void *p, int cur, int out, double g0, double g1, double bdt) {

@@ -62,3 +62,51 @@ def test_cpaw_2d_oblique_order(method):
         errs.append(e[9] - e[5])
     order = math.log2(errs[0] / errs[1])
     assert abs(order - 2.0) < 0.1, order
+
+
+# ---------------------------------------------------------------- explicit diffusion, RKL2 STS
+from diffusion_cases import DECAY_CASES, DIFF_DECK, decay_amplitude, exact_amplitude  # noqa: E402
+
+
+def _active(o):
+    return (slice(o.ks, o.ke + 1), slice(o.js, o.je + 1), slice(o.is_, o.ie + 1))
+
+
+@pytest.mark.parametrize("sts", [False, True], ids=["explicit", "sts"])
+@pytest.mark.parametrize("case", DECAY_CASES, ids=[c[0] for c in DECAY_CASES])
+def test_diffusion_decay_exact_solution(case, sts):
+    """physics.sh:30-36 and 72-81: viscous, resistive and conductive decay of a sinusoid along
+    each axis against exp(-D k^2 t), in the stage (explicit) and operator-split (RKL2 STS)."""
+    label, ov, var, axis, D, t, tol = case
+    o = po.OracleSim(DIFF_DECK, ov + (["diffusion.sts=true"] if sts else []))
+    o.run()
+    assert abs(o.time - t) < 1e-12
+    A = decay_amplitude(o.get("w"), o.get("bcc"), _active(o), var, axis)
+    assert abs(A / exact_amplitude(D, t) - 1) < tol, (label, A, exact_amplitude(D, t))
+
+
+def test_sts_takes_hyperbolic_steps():
+    """RKL2 (sts.hpp:35-39): the loop keeps the hyperbolic step, the parabolic limit is covered by
+    s stages with tau <= dt_par (s^2 + s - 2) / 4, so far fewer steps are taken."""
+    ov = DECAY_CASES[0][1]
+    a = po.OracleSim(DIFF_DECK, ov)
+    a.run()
+    b = po.OracleSim(DIFF_DECK, ov + ["diffusion.sts=true"])
+    b.run()
+    assert b.cycle < a.cycle / 1.5
+
+
+def test_resistive_mhd_keeps_div_b_and_conserves():
+    """Ohmic resistivity and viscosity on the 3D blast: CT keeps div B at round-off (the resistive
+    EMF goes through the same curl, diffusion.hpp:1-7) and the periodic box conserves mass,
+    momentum and energy."""
+    from test_oracle_invariants import div_b, totals
+    ov = ["mesh.nx1=16", "mesh.nx2=16", "mesh.nx3=16", "diffusion.eta=0.01", "diffusion.nu=0.01",
+          "diffusion.kappa=0.01"]
+    for extra in ([], ["diffusion.sts=true"]):
+        o = po.OracleSim(deck("blast_3d.in"), ov + extra)
+        t0 = totals(o)
+        o.step(6)
+        assert div_b(o) < 5e-13
+        t1 = totals(o)
+        assert np.all(np.abs(t1[:5] - t0[:5]) <= 1e-12 * np.maximum(np.abs(t0[:5]), 1.0))
