@@ -39,7 +39,8 @@ class Config(C.Structure):
                 ("cfl", C.c_double), ("flatten", C.c_int), ("low_mach", C.c_int),
                 ("fofc", C.c_int), ("dual_energy", C.c_int), ("de_eta", C.c_double),
                 ("lhlld_phi", C.c_double), ("bc", C.c_int * 3), ("precision", C.c_int),
-                ("characteristic", C.c_int)]
+                ("characteristic", C.c_int), ("nu", C.c_double), ("kappa", C.c_double),
+                ("eta", C.c_double), ("sts", C.c_int)]
 
 
 class Status(C.Structure):
@@ -83,6 +84,9 @@ def lib():
         "chq_cons_to_prim": (C.c_int, [vp, C.c_int, C.c_int, dp]),
         "chq_fofc_flag": (C.c_int, [vp, C.c_int, ip]),
         "chq_fofc_repair": (C.c_int, [vp, C.c_int, C.c_int]),
+        "chq_diffusion_fluxes": (C.c_int, [vp, C.c_int, C.c_int]),
+        "chq_resistive_emf": (C.c_int, [vp, C.c_int]),
+        "chq_diffusion_dt": (C.c_int, [vp, C.POINTER(C.c_double)]),
         "chq_stage": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_long)]),
         "chq_init": (C.c_int, [vp, C.c_double, C.c_double, C.c_int]),
@@ -149,7 +153,7 @@ def exported_symbols():
         "chq_last_error", "chq_version", "chq_create", "chq_destroy", "chq_geometry",
         "chq_device_bytes", "chq_array_count", "chq_upload", "chq_download", "chq_fluxes",
         "chq_corner_emf", "chq_update", "chq_boundaries", "chq_cons_to_prim", "chq_fofc_flag",
-        "chq_fofc_repair", "chq_stage", "chq_init", "chq_step", "chq_step_timed",
+        "chq_fofc_repair", "chq_diffusion_fluxes", "chq_resistive_emf", "chq_diffusion_dt", "chq_stage", "chq_init", "chq_step", "chq_step_timed",
         "chq_get_status", "chq_sync", "chq_stream", "chq_kernel_timing", "chq_kernel_timing_read",
         "chq_deck_open", "chq_deck_open_lazy", "chq_deck_prepare", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
         "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",

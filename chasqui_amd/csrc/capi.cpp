@@ -148,6 +148,8 @@ int check_cfg(const chq_mesh_desc *m, const chq_config *c) {
   if (c->precision == 1 && c->dual_energy) return fail(CHQ_EUNSUP, "mixed precision with dual energy is not supported");
   if (c->precision == 1 && c->characteristic && c->recon != 0)
     return fail(CHQ_EUNSUP, "mixed precision with characteristic reconstruction is not supported");
+  if (c->precision == 1 && (c->nu > 0.0 || c->kappa > 0.0 || c->eta > 0.0 || c->sts))
+    return fail(CHQ_EUNSUP, "mixed precision with diffusion is not supported");
   if (c->rsolver < 0 || c->rsolver > 2) return fail(CHQ_EINVAL, "unknown Riemann solver");
   if (c->nstage < 1 || c->nstage > 4) return fail(CHQ_EINVAL, "nstage must be 1..4");
   const int ngmin = (c->recon == 2 || c->recon == 3 || c->recon == 4) ? 3 : 2;   // main.cpp:169-171
@@ -238,6 +240,7 @@ int chq_destroy(chq_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->arena) (void)hipFree(c->arena);
   if (c->staging) (void)hipFree(c->staging);
+  if (c->sts_buf) (void)hipFree(c->sts_buf);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
@@ -380,6 +383,187 @@ int chq_fofc_repair(chq_ctx *c, int dir, int cur) {
   return CHQ_OK;
 }
 
+// ---------------------------------------------------------------- diffusion
+namespace {
+// diffusion::TimeStep's limit (diffusion.hpp:159-174) from the minimum active density
+double parabolic_dt(const chq_ctx *c, double rho_min) {
+  if (!c->diffusive()) return 1e300;
+  const Geom &g = c->g;
+  const int ndim = 1 + (g.two_d ? 1 : 0) + (g.three_d ? 1 : 0);
+  double dxmin = g.dx1;
+  if (g.two_d) dxmin = std::fmin(dxmin, g.dx2);
+  if (g.three_d) dxmin = std::fmin(dxmin, g.dx3);
+  const double chi = (c->cfg.kappa > 0) ? c->cfg.kappa * c->eos.gm1 / rho_min : 0.0;
+  const double dmax = std::fmax(std::fmax(c->cfg.nu, c->cfg.eta), chi);
+  return dxmin * dxmin / (2.0 * ndim * dmax);
+}
+
+// minimum active density of primitive set `wset` (synchronises)
+int rho_min_now(chq_ctx *c, Ctrl *k, int wset, double *rho) {
+  HIPCHK(launch_rho_min(c->f64.w[wset], c->g, k, true, c->stream));
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, &k->rho_bits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::memcpy(rho, &bits, sizeof(bits));
+  return CHQ_OK;
+}
+
+// the diffusive fluxes of all present directions (diffusion::Fluxes, main.cpp:55-59)
+int enqueue_diff_fluxes(chq_ctx *c, const DiffArgs &a, bool flagged_only) {
+  if (!c->diff_any(a)) return CHQ_OK;
+  for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_diff_flux(a, d, flagged_only, c->stream));
+  return CHQ_OK;
+}
+
+int sts_alloc(chq_ctx *c) {
+  if (c->sts_buf) return CHQ_OK;
+  const size_t cs = size_t(c->g.cstride);
+  HIPCHK(hipMalloc(&c->sts_buf, (5 * NVAR + 15) * cs * sizeof(double)));
+  double *p = c->sts_buf;
+  for (int n = 0; n < 3; ++n, p += NVAR * cs) c->sy[n] = p;
+  c->sm0 = p; p += NVAR * cs;
+  c->smj = p; p += NVAR * cs;
+  for (int n = 0; n < 3; ++n)
+    for (int d = 0; d < 3; ++d, p += cs) c->syb[n][d] = p;
+  for (int d = 0; d < 3; ++d, p += cs) c->smb0[d] = p;
+  for (int d = 0; d < 3; ++d, p += cs) c->smbj[d] = p;
+  return CHQ_OK;
+}
+
+// sts::Operator (sts.hpp:43-90) on (u, b) whose primitives are in set wset: -div F_diff into
+// mu, -curl(eta J) into mb
+int sts_operator(chq_ctx *c, Ctrl *k, int wset, double *const b[3], double *mu, double *const mb[3]) {
+  const size_t cs = size_t(c->g.cstride);
+  for (int d = 0; d < 3; ++d) HIPCHK(launch_fill_zero(c->f64.flx[d], (long long)(NVAR * cs), c->stream));
+  for (int d = 0; d < 3; ++d) HIPCHK(launch_fill_zero(c->f64.ee[d], (long long)cs, c->stream));
+  const DiffArgs a = c->diff_args(b[0], b[1], b[2], wset, k, false);
+  if (int rc = enqueue_diff_fluxes(c, a, false)) return rc;
+  if (a.eta > 0.0) HIPCHK(launch_resistive_emf(a, c->stream));
+  HIPCHK(launch_sts_div(c->g, c->f64.flx[0], c->f64.flx[1], c->f64.flx[2], c->f64.ee[0], c->f64.ee[1], c->f64.ee[2],
+                        mu, mb[0], mb[1], mb[2], c->stream));
+  return CHQ_OK;
+}
+
+int sts_c2p(chq_ctx *c, Ctrl *k, int wset, double *u, double *const b[3]) {
+  C2pArgs a = c->c2p_args(0, wset, k);
+  a.u = u;
+  a.b1 = b[0]; a.b2 = b[1]; a.b3 = b[2];
+  a.compute_dt = 0;
+  a.flag_mode = 0;
+  HIPCHK(launch_c2p<double>(a, c->stream));
+  return CHQ_OK;
+}
+
+int sts_bc(chq_ctx *c, Ctrl *k, double *u, double *const b[3]) {
+  BcArgs a{};
+  a.g = c->g;
+  a.nvar = NVAR;
+  a.u = u;
+  a.b1 = b[0]; a.b2 = b[1]; a.b3 = b[2];
+  a.ctrl = k;
+  a.only_if_bad = 0;
+  a.bc = c->cfg.bc[0];
+  HIPCHK(launch_bc(a, 0, c->stream));
+  if (c->g.two_d) { a.bc = c->cfg.bc[1]; HIPCHK(launch_bc(a, 1, c->stream)); }
+  if (c->g.three_d) { a.bc = c->cfg.bc[2]; HIPCHK(launch_bc(a, 2, c->stream)); }
+  return CHQ_OK;
+}
+
+double sts_b(int j) { return j <= 2 ? 1.0 / 3.0 : (double(j) * j + j - 2.0) / (2.0 * j * (j + 1.0)); }
+
+// sts::Advance (sts.hpp:111-157) over tau from (u0, b?f0), result left there; u1 / b?f1 are
+// the fourth rotating set. Same stage count, coefficients and array rotation as the reference.
+int sts_advance(chq_ctx *c, Ctrl *k, int wset, double tau, double dt_par, int *stages) {
+  if (int rc = sts_alloc(c)) return rc;
+  const size_t cs = size_t(c->g.cstride);
+  const double r = tau / dt_par;
+  int s = static_cast<int>(std::ceil(0.5 * (-1.0 + std::sqrt(9.0 + 16.0 * r))));
+  if (s < 2) s = 2;
+  const double w1 = 4.0 / (double(s) * s + s - 2.0);
+  HIPCHK(hipMemcpyAsync(c->sy[0], c->u[0], NVAR * cs * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  for (int d = 0; d < 3; ++d)
+    HIPCHK(hipMemcpyAsync(c->syb[0][d], c->bf[0][d], cs * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (int rc = sts_c2p(c, k, wset, c->sy[0], c->syb[0])) return rc;
+  if (int rc = sts_operator(c, k, wset, c->syb[0], c->sm0, c->smb0)) return rc;
+  const double mt1 = double(sts_b(1) * w1) * tau;
+  {
+    const StsTerm t[5] = {{c->sy[0], 1}, {c->sy[0], 0}, {c->sy[0], 0}, {c->sm0, mt1}, {c->sm0, 0}};
+    HIPCHK(launch_sts_combine(c->g, c->sy[1], t, NVAR, false, c->stream));
+    for (int d = 0; d < 3; ++d) {
+      const StsTerm tb[5] = {{c->syb[0][d], 1}, {c->syb[0][d], 0}, {c->syb[0][d], 0}, {c->smb0[d], mt1}, {c->smb0[d], 0}};
+      HIPCHK(launch_sts_combine(c->g, c->syb[1][d], tb, 1, true, c->stream));
+    }
+  }
+  if (int rc = sts_bc(c, k, c->sy[1], c->syb[1])) return rc;
+  double *yjm2 = c->sy[0], *yjm1 = c->sy[1], *yj = c->sy[2];
+  double *bjm2[3], *bjm1[3], *bjn[3], *spareb[3];
+  for (int d = 0; d < 3; ++d) {
+    bjm2[d] = c->syb[0][d];
+    bjm1[d] = c->syb[1][d];
+    bjn[d] = c->syb[2][d];
+    spareb[d] = c->bf[1][d];
+  }
+  double *spare = c->u[1];
+  for (int j = 2; j <= s; ++j) {
+    const double b_j = sts_b(j), b_jm1 = sts_b(j - 1), b_jm2 = sts_b(j - 2);
+    const double mu = (2.0 * j - 1.0) / j * b_j / b_jm1;
+    const double nu = -(j - 1.0) / j * b_j / b_jm2;
+    const double mut = mu * w1;
+    const double gt = -(1.0 - b_jm1) * mut;
+    if (int rc = sts_c2p(c, k, wset, yjm1, bjm1)) return rc;
+    if (int rc = sts_operator(c, k, wset, bjm1, c->smj, c->smbj)) return rc;
+    const StsTerm t[5] = {{yjm1, mu}, {yjm2, nu}, {c->sy[0], 1.0 - mu - nu}, {c->smj, mut * tau}, {c->sm0, gt * tau}};
+    HIPCHK(launch_sts_combine(c->g, yj, t, NVAR, false, c->stream));
+    for (int d = 0; d < 3; ++d) {
+      const StsTerm tb[5] = {{bjm1[d], mu}, {bjm2[d], nu}, {c->syb[0][d], 1.0 - mu - nu}, {c->smbj[d], mut * tau},
+                             {c->smb0[d], gt * tau}};
+      HIPCHK(launch_sts_combine(c->g, bjn[d], tb, 1, true, c->stream));
+    }
+    if (int rc = sts_bc(c, k, yj, bjn)) return rc;
+    double *freed = j == 2 ? spare : yjm2;
+    double *freedb[3];
+    for (int d = 0; d < 3; ++d) freedb[d] = j == 2 ? spareb[d] : bjm2[d];
+    yjm2 = yjm1; yjm1 = yj; yj = freed;
+    for (int d = 0; d < 3; ++d) { bjm2[d] = bjm1[d]; bjm1[d] = bjn[d]; bjn[d] = freedb[d]; }
+  }
+  HIPCHK(hipMemcpyAsync(c->u[0], yjm1, NVAR * cs * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  for (int d = 0; d < 3; ++d)
+    HIPCHK(hipMemcpyAsync(c->bf[0][d], bjm1[d], cs * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  *stages = s;
+  return CHQ_OK;
+}
+}  // namespace
+
+int chq_diffusion_fluxes(chq_ctx *c, int dir, int cur) {
+  if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_diffusion_fluxes: FP64 contexts only");
+  if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
+  const DiffArgs a = c->diff_args(cur, c->wcur, c->ctrl_api, false);
+  if (c->diff_any(a)) HIPCHK(launch_diff_flux(a, dir, false, c->stream));
+  return CHQ_OK;
+}
+
+int chq_resistive_emf(chq_ctx *c, int cur) {
+  if (!c || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_resistive_emf: FP64 contexts only");
+  const DiffArgs a = c->diff_args(cur, c->wcur, c->ctrl_api, false);
+  if (a.eta > 0.0) HIPCHK(launch_resistive_emf(a, c->stream));
+  return CHQ_OK;
+}
+
+int chq_diffusion_dt(chq_ctx *c, double *dt) {
+  if (!c || !dt) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_diffusion_dt: FP64 contexts only");
+  double rho = 1e300;
+  if (c->diffusive())
+    if (int rc = rho_min_now(c, c->ctrl_api, c->wcur, &rho)) return rc;
+  *dt = parabolic_dt(c, rho);
+  return CHQ_OK;
+}
+
 // Stage(), main.cpp:46-93, reference order (flag before boundaries; fofc branch taken on the
 // device when the flag kernel counted cells)
 int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt, long *nbad) {
@@ -389,8 +573,11 @@ int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt
   const Ctrl *k = c->ctrl_api;
   const Recon rc = Recon(c->cfg.recon);
   const RSolver rs = RSolver(c->cfg.rsolver);
+  const DiffArgs da = c->diff_args(cur, c->wcur, k, true);
   for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_flux(c->flux_args(d, cur, c->wcur, k), d, rc, rs, true, c->stream));
+  if (int rc_ = enqueue_diff_fluxes(c, da, false)) return rc_;
   HIPCHK(launch_corner<double>(c->corner_args(c->wcur, k, 0), c->stream));
+  if (da.eta > 0.0) HIPCHK(launch_resistive_emf(da, c->stream));
   UpdateArgs ua = c->update_args(out, g0, g1, 6, k, 0);
   ua.beta_dt = beta_dt;
   HIPCHK(launch_update<double>(ua, c->stream));
@@ -398,7 +585,13 @@ int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt
   if (c->cfg.fofc) {
     HIPCHK(launch_flag(c->c2p_args(out, c->wcur, c->ctrl_api), c->stream));
     for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_repair(c->flux_args(d, cur, c->wcur, k), d, true, c->stream));
+    if (int rc_ = enqueue_diff_fluxes(c, da, true)) return rc_;
     HIPCHK(launch_corner<double>(c->corner_args(c->wcur, k, 1), c->stream));
+    if (da.eta > 0.0) {
+      DiffArgs db = da;
+      db.only_if_bad = 1;
+      HIPCHK(launch_resistive_emf(db, c->stream));
+    }
     UpdateArgs ub = c->update_args(out, g0, g1, 6, k, 1);
     ub.beta_dt = beta_dt;
     HIPCHK(launch_update<double>(ub, c->stream));
@@ -429,6 +622,11 @@ int chq_init_resume(chq_ctx *c, double t, int cycle, double tlim, int nlim) {
   h.nlim = nlim;
   h.cfl = c->cfg.cfl;
   h.min_dt = std::fmin(mdt, 1e300);   // main.cpp:201-202
+  if (!c->cfg.sts && c->diffusive()) {
+    double rho = 1e300;
+    if (int rc2 = rho_min_now(c, c->ctrl_api, c->wcur, &rho)) return rc2;
+    h.min_dt = std::fmin(h.min_dt, parabolic_dt(c, rho));
+  }
   h.active = 0;
   h.cycle = cycle;
   h.wcur = c->wcur;
@@ -511,7 +709,8 @@ static int enqueue_step(chq_ctx *c) {
   const RSolver rs = RSolver(c->cfg.rsolver);
   Ctrl *k = c->ctrl;
   const int nst = c->cfg.nstage;
-  const bool fused = !c->eos.dual_energy;
+  // dual energy, diffusion and super-time-stepping run in the reference kernel order
+  const bool fused = !c->eos.dual_energy && !c->diffusive() && !c->cfg.sts;
   const int nvar = c->entropy ? 6 : 5;
   HIPCHK(launch_ctrl_begin_step(k, c->stream));
   for (int s = 1; s <= nst; ++s) {
@@ -525,12 +724,15 @@ static int enqueue_step(chq_ctx *c) {
       c->wcur ^= 1;
     } else {
       const int wc = c->wcur;
+      const DiffArgs da = c->diff_args(cf.cur, wc, k, true);
       for (int d = 0; d < c->ndir(); ++d) {
         FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
         fa.entropy = c->entropy;
         HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
       }
+      if (int rc_ = enqueue_diff_fluxes(c, da, false)) return rc_;        // main.cpp:55-59
       HIPCHK(launch_corner<double>(c->corner_args(wc, k, 0), c->stream));
+      if (da.eta > 0.0) HIPCHK(launch_resistive_emf(da, c->stream));     // main.cpp:66
       UpdateArgs ua = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 0);
       ua.use_ctrl_dt = 1;
       ua.beta = cf.beta;
@@ -538,7 +740,13 @@ static int enqueue_step(chq_ctx *c) {
       if (c->cfg.fofc) {
         HIPCHK(launch_flag(c->c2p_args(cf.out, wc, k), c->stream));
         for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_repair(c->flux_args(d, cf.cur, wc, k), d, true, c->stream));
+        if (int rc_ = enqueue_diff_fluxes(c, da, true)) return rc_;       // fofc.hpp:90-93
         HIPCHK(launch_corner<double>(c->corner_args(wc, k, 1), c->stream));
+        if (da.eta > 0.0) {
+          DiffArgs db = da;
+          db.only_if_bad = 1;
+          HIPCHK(launch_resistive_emf(db, c->stream));
+        }
         UpdateArgs ub = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 1);
         ub.use_ctrl_dt = 1;
         ub.beta = cf.beta;
@@ -546,11 +754,35 @@ static int enqueue_step(chq_ctx *c) {
       }
       HIPCHK(c->boundaries(cf.out, nvar, k, 0));
       C2pArgs ca = c->c2p_args(cf.out, wc, k);
-      ca.compute_dt = last;
+      ca.compute_dt = last && !c->cfg.sts;   // main.cpp:273-274: split steps take dt after the STS
       ca.slot = 0;
       HIPCHK(launch_c2p<double>(ca, c->stream));
     }
     HIPCHK(launch_ctrl_end_stage(k, last, fused ? 1 : 0, c->stream));
+    if (last && !fused && !c->cfg.sts && c->diffusive()) {   // main.cpp:275
+      HIPCHK(launch_rho_min(c->f64.w[c->wcur], c->g, k, true, c->stream));
+      HIPCHK(launch_diff_dt_fold(k, c->diff_args(0, c->wcur, k, false), c->stream));
+    }
+  }
+  if (c->cfg.sts) {   // operator-split diffusion on the end-of-step state, main.cpp:278-290
+    double rho = 1e300;
+    if (c->diffusive())
+      if (int rc_ = rho_min_now(c, k, c->wcur, &rho)) return rc_;   // synchronises
+    Ctrl h{};
+    HIPCHK(hipMemcpyAsync(&h, k, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h.active) {
+      int s = 0;
+      if (int rc_ = sts_advance(c, k, c->wcur, h.dt, parabolic_dt(c, rho), &s)) return rc_;
+      c->sts_stages += s;
+      HIPCHK(c->boundaries(0, NVAR, k, 0));
+      HIPCHK(launch_ctrl_begin_stage(k, 1, c->stream));
+      C2pArgs ca = c->c2p_args(0, c->wcur, k);
+      ca.compute_dt = 1;
+      ca.slot = 0;
+      HIPCHK(launch_c2p<double>(ca, c->stream));
+      HIPCHK(launch_ctrl_take_min(k, 0, c->stream));
+    }
   }
   HIPCHK(launch_ctrl_end_step(k, c->stream));
   c->wcur_stale = fused;

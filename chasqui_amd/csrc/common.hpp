@@ -50,6 +50,7 @@ struct Ctrl {
   int nbad_last;                 // flags of the last stage (selects the repaired minimum)
   int wcur;                      // current primitive set (batched runs; solo contexts track it on the host)
   long long fofc_total;
+  unsigned long long rho_bits;   // minimum active density of the last con2prim (parabolic dt)
 };
 
 inline __host__ __device__ int div_up(int a, int b) { return (a + b - 1) / b; }

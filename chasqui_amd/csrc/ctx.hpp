@@ -69,6 +69,11 @@ struct chq_ctx {
   void *batch = nullptr;     // the batch advancing this context, if any
   hipStream_t batch_stream = nullptr;   // its stream (host reads wait for it)
   bool entropy = false;      // transport IENT in the time loop (only needed with dual energy)
+  // super-time-stepping work arrays (sts.hpp:16-32), allocated at the first operator-split step:
+  // y[0..2], m0, mj (NVAR components each) and the face sets yb[0..2], mb0, mbj (3 each)
+  double *sts_buf = nullptr;
+  double *sy[3]{}, *sm0 = nullptr, *smj = nullptr, *syb[3][3]{}, *smb0[3]{}, *smbj[3]{};
+  long long sts_stages = 0;  // RKL2 stages taken so far
   // per-kernel-class timing: event pairs recorded around launches of one class
   int timing_class = -1;
   std::vector<hipEvent_t> tev;   // begin/end pairs
@@ -169,6 +174,30 @@ struct chq_ctx {
     return e;
   }
   int ndir() const { return 1 + (g.two_d ? 1 : 0) + (g.three_d ? 1 : 0); }
+  bool diffusive() const { return cfg.nu > 0.0 || cfg.kappa > 0.0 || cfg.eta > 0.0; }
+  // diffusion launch description: primitives of set `wset`, current density from face set `cur`
+  // (or the given face arrays); in_stage = the coefficients the RK stage uses (none under STS)
+  DiffArgs diff_args(const double *b1, const double *b2, const double *b3, int wset, const Ctrl *c, bool in_stage) const {
+    DiffArgs a{};
+    a.g = g;
+    const bool on = !(in_stage && cfg.sts);
+    a.nu = on ? cfg.nu : 0.0;
+    a.kappa = on ? cfg.kappa : 0.0;
+    a.eta = on ? cfg.eta : 0.0;
+    a.gm1 = eos.gm1;
+    a.w = f64.w[wset];
+    a.bcc = f64.bcc[wset];
+    a.b1 = b1; a.b2 = b2; a.b3 = b3;
+    for (int d = 0; d < 3; ++d) a.f[d] = f64.flx[d];
+    a.e1 = f64.ee[0]; a.e2 = f64.ee[1]; a.e3 = f64.ee[2];
+    a.flag = flag;
+    a.ctrl = c;
+    return a;
+  }
+  DiffArgs diff_args(int cur, int wset, const Ctrl *c, bool in_stage) const {
+    return diff_args(bf[cur][0], bf[cur][1], bf[cur][2], wset, c, in_stage);
+  }
+  bool diff_any(const DiffArgs &a) const { return a.nu > 0.0 || a.kappa > 0.0 || a.eta > 0.0; }
 
   template <class T> FluxArgsT<T> flux_args_t(const FieldSet<T> &f, int d, int cur, int wset, const Ctrl *c) const {
     FluxArgs a0 = flux_args(d, cur, wset, c);

@@ -250,6 +250,51 @@ void shock_tube(Params &p, Fields &h) {
   });
 }
 
+// decay of a sinusoid in B, v or T along one axis under one diffusive term, pgen.hpp:235-266
+void diffusion_test(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const std::string var = p.get_string("problem", "var", "b");
+  const int axis = p.get_int("problem", "axis", 1);
+  const double amp = p.get_real("problem", "amp", 1.0e-3);
+  const double d0 = p.get_real("problem", "dens", 1.0), p0 = p.get_real("problem", "pres", 1.0);
+  const double L = axis == 1 ? (m.x1max - m.x1min) : axis == 2 ? (m.x2max - m.x2min) : (m.x3max - m.x3min);
+  const double kw = 2.0 * kPi / L;
+  const int comp = axis % 3;
+  auto coord = [&](int k, int j, int i) { return axis == 1 ? m.x1v(i) : axis == 2 ? m.x2v(j) : m.x3v(k); };
+  for_rows(m.nk + 1, m.nj + 1, m.ni + 1, [&](int k, int j) {
+      for (int i = 0; i < m.ni + 1; ++i) {
+        const size_t f = m.f(k, j, i);
+        h.b1[f] = 0.0;
+        h.b2[f] = 0.0;
+        h.b3[f] = 0.0;
+        if (var == "b") {
+          const double val = amp * std::sin(kw * coord(k, j, i));
+          (comp == 0 ? h.b1 : comp == 1 ? h.b2 : h.b3)[f] = val;
+        }
+      }
+  });
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
+      for (int i = 0; i < m.ni; ++i) {
+        const double x = coord(k, j, i);
+        double d = d0, pr = p0, v[3] = {0.0, 0.0, 0.0};
+        if (var == "v") v[comp] = amp * std::sin(kw * x);
+        if (var == "temp") {
+          const double T = p0 / d0 + amp * std::sin(kw * x);
+          d = p0 / T;
+        }
+        if (var == "b") {
+          const double s = amp * std::sin(kw * x);
+          pr = p0 - 0.5 * (s * s);
+        }
+        h.W(0, k, j, i) = d;
+        h.W(1, k, j, i) = v[0];
+        h.W(2, k, j, i) = v[1];
+        h.W(3, k, j, i) = v[2];
+        h.W(4, k, j, i) = pr;
+      }
+  });
+}
+
 }  // namespace
 
 void Deck::configure() {
@@ -280,9 +325,10 @@ void Deck::configure() {
   cfg.precision = pr == "fp64" ? 0 : pr == "mixed" ? 1 : -1;
   if (cfg.precision < 0) throw std::runtime_error("unknown precision: " + pr + " (fp64 | mixed)");
   if (p.get_bool("mhd", "flux_limiter", false)) throw std::runtime_error("mhd.flux_limiter is not supported");
-  if (p.get_bool("diffusion", "sts", false) || p.get_real("diffusion", "nu", 0.0) != 0.0 ||
-      p.get_real("diffusion", "kappa", 0.0) != 0.0 || p.get_real("diffusion", "eta", 0.0) != 0.0)
-    throw std::runtime_error("diffusive terms are not supported");
+  cfg.sts = p.get_bool("diffusion", "sts", false) ? 1 : 0;   // main.cpp:142-146
+  cfg.nu = p.get_real("diffusion", "nu", 0.0);
+  cfg.kappa = p.get_real("diffusion", "kappa", 0.0);
+  cfg.eta = p.get_real("diffusion", "eta", 0.0);
   if (p.get_bool("gravity", "uniform", false) || p.get_bool("gravity", "point_mass", false) ||
       p.get_bool("cooling", "enabled", false))
     throw std::runtime_error("gravity and cooling source terms are not supported");
@@ -323,6 +369,7 @@ void Deck::initialize() {
   else if (problem == "orszag_tang") orszag_tang(p, h);
   else if (problem == "blast") blast(p, h);
   else if (problem == "shock_tube") shock_tube(p, h);
+  else if (problem == "diffusion_test") diffusion_test(p, h);
   else throw std::runtime_error("unknown problem: " + problem);
   // conserved variables from primitives and face fields, pgen.hpp:286-301
   const double gm1 = cfg.gamma - 1.0;

@@ -227,6 +227,33 @@ template <class T> hipError_t launch_corner(const CornerArgsT<T> &a, hipStream_t
 template <class T> hipError_t launch_corner_list(const CornerArgsT<T> &a, const int *list, hipStream_t s);
 template <class T> hipError_t launch_update(const UpdateArgsT<T> &a, hipStream_t s);
 hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s);
+
+// explicit diffusion (FP64 single-mesh path), diffusion.hip
+struct DiffArgs {
+  Geom g;
+  double nu, kappa, eta, gm1;
+  const double *w, *bcc;                 // primitives, cell-centred B
+  const double *b1, *b2, *b3;            // face B the current density is taken from
+  double *f[3];                          // face fluxes (6 components each)
+  double *e1, *e2, *e3;                  // edge EMFs
+  const int *flag;                       // fofc flags (repair launches only)
+  const Ctrl *ctrl;
+  int only_if_bad;                       // skip unless the current stage flagged cells (fofc branch)
+};
+hipError_t launch_diff_flux(const DiffArgs &a, int dir, bool flagged_only, hipStream_t s);
+hipError_t launch_resistive_emf(const DiffArgs &a, hipStream_t s);
+// minimum of w[IDN] over the active cells into ctrl->rho_bits (reset first)
+hipError_t launch_rho_min(const double *w, const Geom &g, Ctrl *c, bool reset, hipStream_t s);
+// ctrl->min_dt = min(ctrl->min_dt, parabolic limit from ctrl->rho_bits) when the step ran
+hipError_t launch_diff_dt_fold(Ctrl *c, const DiffArgs &a, hipStream_t s);
+// super-time-stepping pieces
+hipError_t launch_fill_zero(double *p, long long n, hipStream_t s);
+hipError_t launch_sts_div(const Geom &g, const double *f1, const double *f2, const double *f3, const double *e1,
+                          const double *e2, const double *e3, double *mu, double *mb1, double *mb2, double *mb3,
+                          hipStream_t s);
+struct StsTerm { const double *x; double a; };
+hipError_t launch_sts_combine(const Geom &g, double *out, const StsTerm t[5], int ncomp, bool faces, hipStream_t s);
+hipError_t launch_ctrl_take_min(Ctrl *c, int slot, hipStream_t s);   // ctrl->min_dt = running minimum `slot`
 template <class T> hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s);
 hipError_t launch_flag(const C2pArgs &a, hipStream_t s);   // standalone FlagBadCells
 // control-block kernels

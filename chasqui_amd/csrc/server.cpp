@@ -250,6 +250,10 @@ int chq_server_submit(chq_server *s, const char *deck_text, const char *override
     chq_deck_close(j.deck);
     return fail(CHQ_EUNSUP, "served runs need primitive-variable reconstruction");
   }
+  if (j.cfg.nu > 0.0 || j.cfg.kappa > 0.0 || j.cfg.eta > 0.0 || j.cfg.sts) {
+    chq_deck_close(j.deck);
+    return fail(CHQ_EUNSUP, "served runs need diffusion off");
+  }
   j.bytes = chq_bytes_for(&j.mesh, &j.cfg);
   if (j.bytes == 0) { chq_deck_close(j.deck); return fail(CHQ_EINVAL, "invalid mesh or configuration"); }
   if (j.bytes > s->budget) { chq_deck_close(j.deck); return fail(CHQ_ENOMEM, "run larger than the HBM budget"); }

@@ -16,7 +16,7 @@ from ._capi import ARRAY_IDS, CELL_ARRAYS, FACE_MULTI, check, lib
 def make_config(recon="ppm", rsolver="hlld", integrator="rk3", gamma=5.0 / 3.0, dfloor=1e-10,
                 pfloor=1e-12, cfl=0.3, flatten=True, low_mach=False, fofc=True, dual_energy=False,
                 de_eta=1e-3, lhlld_phi=-1.0, bc=("periodic", "periodic", "periodic"), precision="fp64",
-                characteristic=False):
+                characteristic=False, nu=0.0, kappa=0.0, eta=0.0, sts=False):
     c = _capi.Config()
     c.recon = _capi.RECON[recon]
     c.rsolver = _capi.RSOLVER[rsolver]
@@ -28,6 +28,7 @@ def make_config(recon="ppm", rsolver="hlld", integrator="rk3", gamma=5.0 / 3.0, 
         c.bc[d] = _capi.BC[bc[d]]
     c.precision = _capi.PRECISION[precision]
     c.characteristic = int(characteristic)
+    c.nu, c.kappa, c.eta, c.sts = nu, kappa, eta, int(sts)
     return c
 
 
@@ -133,6 +134,17 @@ class Mesh:
 
     def repair_fluxes(self, d, cur):
         check(self.L.chq_fofc_repair(self.h, d, cur), "chq_fofc_repair")
+
+    def diffusion_fluxes(self, d, cur):
+        check(self.L.chq_diffusion_fluxes(self.h, d, cur), "chq_diffusion_fluxes")
+
+    def resistive_emf(self, cur):
+        check(self.L.chq_resistive_emf(self.h, cur), "chq_resistive_emf")
+
+    def diffusion_dt(self):
+        v = C.c_double(0.0)
+        check(self.L.chq_diffusion_dt(self.h, C.byref(v)), "chq_diffusion_dt")
+        return v.value
 
     def stage(self, cur, out, g0, g1, beta_dt):
         n = C.c_long(0)

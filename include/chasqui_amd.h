@@ -48,7 +48,11 @@ typedef struct {
  * in the reference, which is FP64 only (defs.hpp:11 Real = double).
  * characteristic: reconstruct in the characteristic variables of each side's cell
  * (Config::characteristic, physics.hpp:30, 71, 110-150); ignored with dc; FP64 only, and not
- * in batches or served runs. */
+ * in batches or served runs.
+ * nu, kappa, eta: viscosity, thermal conduction and Ohmic resistivity (diffusion::Coeffs,
+ * diffusion.hpp:18-21; main.cpp:144-146), added in every stage, or, with sts = 1, advanced by
+ * RKL2 super-time-stepping after each step (sts.hpp:111-157; main.cpp:142, 278-290). FP64
+ * single-mesh path only. */
 typedef struct {
   int recon, rsolver, nstage;
   double gamma, dfloor, pfloor, cfl;
@@ -57,6 +61,8 @@ typedef struct {
   int bc[3];
   int precision;
   int characteristic;
+  double nu, kappa, eta;
+  int sts;
 } chq_config;
 
 /* Array identifiers for chq_upload / chq_download / chq_array_count. */
@@ -115,6 +121,14 @@ int chq_cons_to_prim(chq_ctx *ctx, int buf, int compute_dt, double *min_dt);
 int chq_fofc_flag(chq_ctx *ctx, int buf, int *count);
 /* fofc::RepairFluxes<DIR> (fofc.hpp:46-96) from w/bcc and face B of set `cur`. */
 int chq_fofc_repair(chq_ctx *ctx, int dir, int cur);
+/* diffusion::Fluxes<DIR> (diffusion.hpp:131-138): adds the viscous, conductive and resistive
+ * fluxes of the context's coefficients to f{dir+1} on the active faces, from w/bcc and the
+ * face B of set `cur`. */
+int chq_diffusion_fluxes(chq_ctx *ctx, int dir, int cur);
+/* diffusion::ResistiveEMF (diffusion.hpp:141-156): adds eta J to e1, e2, e3. */
+int chq_resistive_emf(chq_ctx *ctx, int cur);
+/* diffusion::TimeStep (diffusion.hpp:159-174) from the current primitives (synchronises). */
+int chq_diffusion_dt(chq_ctx *ctx, double *dt);
 /* Stage() (main.cpp:46-93): fluxes, corner EMF, update, fofc, boundaries; *nbad (may be
  * NULL) receives the fofc count (synchronises when non-NULL). */
 int chq_stage(chq_ctx *ctx, int cur, int out, double g0, double g1, double beta_dt, long *nbad);

@@ -117,6 +117,23 @@ int orc_update(void *p, int out, double g0, double g1, double bdt) { update(S(p)
 int orc_boundaries(void *p, int s) { boundaries(S(p)->m, S(p)->cfg, s); return 0; }
 double orc_cons_to_prim(void *p, int s, int compute_dt) { return cons_to_prim(S(p)->m, S(p)->cfg, s, compute_dt != 0); }
 int orc_fofc_flag(void *p, int s) { return fofc_flag(S(p)->m, S(p)->cfg, s); }
+void orc_diffusion_fluxes(void *p, int dir, int cur) {
+  Sim &s = *S(p);
+  const Diff dc = diff_of(s.cfg);
+  const A3 &b1 = s.m.bf(cur, 0), &b2 = s.m.bf(cur, 1), &b3 = s.m.bf(cur, 2);
+  if (dir == 0) diffusion_fluxes<0>(s.m, dc, b1, b2, b3);
+  else if (dir == 1) diffusion_fluxes<1>(s.m, dc, b1, b2, b3);
+  else diffusion_fluxes<2>(s.m, dc, b1, b2, b3);
+}
+void orc_resistive_emf(void *p, int cur) {
+  Sim &s = *S(p);
+  resistive_emf(s.m, diff_of(s.cfg), s.m.bf(cur, 0), s.m.bf(cur, 1), s.m.bf(cur, 2));
+}
+double orc_diffusion_dt(void *p) {
+  Sim &s = *S(p);
+  return diffusion_dt(s.m, s.cfg, diff_of(s.cfg));
+}
+long orc_sts_count(void *p) { return S(p)->sts_count; }
 int orc_fofc_repair(void *p, int dir, int cur) {
   Sim &s = *S(p);
   fofc_repair_any(s.m, s.cfg, dir, cur, s.cfg.sts ? Diff{} : diff_of(s.cfg));

@@ -58,6 +58,12 @@ def lib():
     L.orc_cons_to_prim.argtypes = [vp, C.c_int, C.c_int]
     L.orc_fofc_flag.argtypes = [vp, C.c_int]
     L.orc_fofc_repair.argtypes = [vp, C.c_int, C.c_int]
+    L.orc_diffusion_fluxes.argtypes = [vp, C.c_int, C.c_int]
+    L.orc_resistive_emf.argtypes = [vp, C.c_int]
+    L.orc_diffusion_dt.restype = C.c_double
+    L.orc_diffusion_dt.argtypes = [vp]
+    L.orc_sts_count.restype = C.c_long
+    L.orc_sts_count.argtypes = [vp]
     L.orc_stage.restype = C.c_long
     L.orc_stage.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
     L.orc_set_floors.argtypes = [vp, C.c_double, C.c_double]
@@ -209,6 +215,19 @@ class OracleSim:
 
     def fofc_repair(self, d, cur):
         self.L.orc_fofc_repair(self.h, d, cur)
+
+    def diffusion_fluxes(self, d, cur):
+        self.L.orc_diffusion_fluxes(self.h, d, cur)
+
+    def resistive_emf(self, cur):
+        self.L.orc_resistive_emf(self.h, cur)
+
+    def diffusion_dt(self):
+        return self.L.orc_diffusion_dt(self.h)
+
+    @property
+    def sts_count(self):
+        return self.L.orc_sts_count(self.h)
 
     def stage(self, cur, out, g0, g1, bdt):
         return self.L.orc_stage(self.h, cur, out, g0, g1, bdt)

@@ -54,7 +54,9 @@ def gpu_mesh_for(text, overrides=()):
                       de_eta=float(g("mhd.de_eta", 1e-3)), lhlld_phi=float(g("mhd.lhlld_phi", -1.0)),
                       bc=(g("mesh.bc_x1", "periodic"), g("mesh.bc_x2", "periodic"), g("mesh.bc_x3", "periodic")),
                       precision=g("mhd.precision", "fp64"),
-                      characteristic=_b(g("mhd.characteristic", "false")))
+                      characteristic=_b(g("mhd.characteristic", "false")),
+                      nu=float(g("diffusion.nu", 0.0)), kappa=float(g("diffusion.kappa", 0.0)),
+                      eta=float(g("diffusion.eta", 0.0)), sts=_b(g("diffusion.sts", "false")))
     return Mesh(md, cfg), kv
 
 

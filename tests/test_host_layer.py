@@ -27,9 +27,12 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("name,ov", CASES)
+DIFF_CASES = [(None, c[1]) for c in __import__("diffusion_cases").DECAY_CASES]
+
+
+@pytest.mark.parametrize("name,ov", CASES + DIFF_CASES)
 def test_initial_state_bitwise_vs_oracle(name, ov):
-    text = deck(name)
+    text = deck(name) if name else __import__("diffusion_cases").DIFF_DECK
     d = Deck(text, ov)
     o = po.OracleSim(text, ov, pgen_only=True)
     for host, ora in (("u0", "u0"), ("w", "w"), ("b1f0", "b1f0"), ("b2f0", "b2f0"), ("b3f0", "b3f0")):
@@ -75,7 +78,8 @@ def test_overrides_win_over_file():
     (["time.integrator=rk7"], "unknown integrator"),
     (["mesh.nghost=2"], "nghost too small"),
     (["problem.name=nothere"], "unknown problem"),
-    (["diffusion.eta=0.1"], "not supported"),
+    (["gravity.uniform=true"], "not supported"),
+    (["mhd.flux_limiter=true"], "not supported"),
 ])
 def test_bad_decks_raise(ov, msg):
     with pytest.raises(DeckError, match=msg):
