@@ -46,7 +46,7 @@ class Config(C.Structure):
 class Status(C.Structure):
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("tlim", C.c_double), ("min_dt", C.c_double),
                 ("cycle", C.c_int), ("nlim", C.c_int), ("active", C.c_int), ("nbad_last", C.c_int),
-                ("fofc_total", C.c_longlong)]
+                ("fofc_total", C.c_longlong), ("sts_stages", C.c_longlong)]
 
 
 _lib = None

@@ -854,6 +854,7 @@ int chq_get_status(chq_ctx *c, chq_status *st) {
   st->active = h.active;
   st->nbad_last = h.nbad_last;
   st->fofc_total = h.fofc_total;
+  st->sts_stages = c->sts_stages;
   return CHQ_OK;
 }
 

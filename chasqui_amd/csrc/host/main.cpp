@@ -154,6 +154,7 @@ int main(int argc, char *argv[]) {
   std::cout << "# cycles = " << cycle << ", time = " << t << ", wall = " << wall << " s\n";
   std::cout << "# zone-cycles/second = " << std::scientific << zones * (cycle - cycle0) / wall << "\n";
   if (cfg.fofc) std::cout << "# first-order flux corrections applied: " << st.fofc_total << "\n";
+  if (cfg.sts) std::cout << "# super-time-stepping stages (total): " << st.sts_stages << "\n";
 
   if (compute_error) {   // main.cpp:310-327
     std::vector<double> u(u_init.size()), b(b_init.size());

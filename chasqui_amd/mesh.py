@@ -189,7 +189,7 @@ class Mesh:
         check(self.L.chq_get_status(self.h, C.byref(s)), "chq_get_status")
         return {"t": s.t, "dt": s.dt, "tlim": s.tlim, "min_dt": s.min_dt, "cycle": s.cycle,
                 "nlim": s.nlim, "active": s.active, "nbad_last": s.nbad_last,
-                "fofc_total": s.fofc_total}
+                "fofc_total": s.fofc_total, "sts_stages": s.sts_stages}
 
     def sync(self):
         check(self.L.chq_sync(self.h), "chq_sync")

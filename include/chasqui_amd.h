@@ -81,6 +81,7 @@ typedef struct {
   double t, dt, tlim, min_dt;
   int cycle, nlim, active, nbad_last;
   long long fofc_total;
+  long long sts_stages;   /* RKL2 stages taken (super-time-stepping, main.cpp:279, 308) */
 } chq_status;
 
 const char *chq_last_error(void);

@@ -82,3 +82,41 @@ def test_driver_rejects_bad_deck(tmp_path):
     r = subprocess.run([EXE, "-i", os.path.join(DECKS, "cpaw_1d.in"), "mhd.reconstruct=bogus"], cwd=tmp_path,
                        capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "unknown reconstruct" in r.stderr
+
+
+def _read_vtk(path):
+    """cell data of a legacy binary VTK file written by the driver (output.hpp layout)"""
+    data = path.read_bytes()
+    dims = [int(x) for x in re.search(rb"DIMENSIONS (\d+) (\d+) (\d+)", data).groups()]
+    n = int(re.search(rb"CELL_DATA (\d+)", data).group(1))
+    nx, ny, nz = dims[0] - 1, max(dims[1] - 1, 1), max(dims[2] - 1, 1)
+    out = {}
+    for name in ("rho", "prs", "vx1", "vx2", "vx3", "Bx1", "Bx2", "Bx3"):
+        m = re.search(rb"SCALARS " + name.encode() + rb" float\nLOOKUP_TABLE default\n", data)
+        out[name] = np.frombuffer(data[m.end():m.end() + 4 * n], dtype=">f4").astype(float).reshape(nz, ny, nx)
+    return out
+
+
+def test_driver_diffusion_decay_like_the_reference(tmp_path):
+    """tests/physics.sh:13-36, 60-81 as the reference runs them: the driver on a diffusion deck,
+    the final VTK read back, the sin(kx) amplitude against exp(-D k^2 t); then the same with
+    super-time-stepping, which must report its RKL2 stages."""
+    from diffusion_cases import DECAY_CASES, DIFF_DECK, exact_amplitude
+    (tmp_path / "diff.in").write_text(DIFF_DECK + "[output]\nbasename = diff\nvtk_dt = 0.5\n")
+    field = {"b": {1: "Bx2", 2: "Bx3", 3: "Bx1"}, "v": {1: "vx2", 2: "vx3", 3: "vx1"}}
+    k = 2 * np.pi
+    for label, ov, var, axis, D, t, tol in DECAY_CASES:
+        for sts in (False, True):
+            name = label + ("_sts" if sts else "")
+            extra = ["output.vtk_dt=2.0"] if label == "kap_x" else []
+            out = _run(["-i", "diff.in", f"output.basename={name}", "time.ndiag=0"] + ov + extra +
+                       (["diffusion.sts=true"] if sts else []), tmp_path)
+            if sts:
+                assert "# super-time-stepping stages (total): " in out
+            d = _read_vtk(tmp_path / f"{name}.0001.vtk")
+            f = d["prs"] / d["rho"] - 1.0 if var == "T" else d[field[var][axis]]
+            ax = 3 - axis   # (z, y, x) array axes
+            prof = f.mean(axis=tuple(a for a in range(3) if a != ax))
+            x = (np.arange(prof.size) + 0.5) / prof.size
+            A = np.sum(prof * np.sin(k * x)) / np.sum(np.sin(k * x) ** 2)
+            assert abs(A / exact_amplitude(D, t) - 1) < tol, (name, A)
