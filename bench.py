@@ -35,9 +35,9 @@ FLUX_DOUBLES_PER_CELL_STAGE = 52.0
 
 # HBM bytes per flux-kernel launch (mean of the x, y and z launches), measured with rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE and scaled by the reading of a plain 2 GiB device copy
-# (profiles/blast256_fp64_hbm_traffic.md): read 1241 / 1837 / 1837 MiB, write 935 / 992 / 992
+# (profiles/blast256_fp64_hbm_traffic.md): read 1252 / 1470 / 1471 MiB, write 940 / 962 / 962
 # MiB per x / y / z launch.
-FLUX_TRAFFIC_PER_LAUNCH = (1241 + 935 + 2 * (1837 + 992)) / 3 * 2 ** 20
+FLUX_TRAFFIC_PER_LAUNCH = (1252 + 940 + 1470 + 962 + 1471 + 962) / 3 * 2 ** 20
 
 DECK = """
 [mesh]

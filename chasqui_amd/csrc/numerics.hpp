@@ -253,13 +253,26 @@ template <class T> struct Rsv {
     vbs = (sm * bn + vs * bys) + ws * bzs;
     es = (((a * e - pt * S.u) + pts * sm) + bn * (vb - vbs)) / bs;
   }
-  // HLLD (Miyoshi & Kusano 2005) + LHLLD factors, riemann.hpp:96-234
+  // the energy and total pressure of pflux (same expressions)
+  static CHQ_FN void ept(const St<T> &s, T bn, T gm1, T &e, T &pt) {
+    const T v2 = s.u * s.u + s.v * s.v + s.w * s.w;
+    const T b2 = bn * bn + s.by * s.by + s.bz * s.bz;
+    e = s.p / gm1 + T(0.5) * (s.d * v2 + b2);
+    pt = s.p + T(0.5) * b2;
+  }
+  // fls = F(L) + S_L (U*_L - U_L) of one side (riemann.hpp:170-183), F(L) formed here
+  static CHQ_FN void fstar_hll(const St<T> &S, T bn, T gm1, T sw, const T us[NFLX], T f[NFLX]) {
+    T fs[NFLX], e, pt;
+    pflux(S, bn, gm1, e, pt, fs);
+    const T u[NFLX] = {S.d, S.d * S.u, S.d * S.v, S.d * S.w, e, S.by, S.bz};
+    for (int n = 0; n < NFLX; ++n) f[n] = fs[n] + sw * (us[n] - u[n]);
+  }
+  // HLLD (Miyoshi & Kusano 2005) + LHLLD factors, riemann.hpp:96-234. Each flux is formed
+  // only on the branch that returns it (same expressions, so the same values as forming all
+  // of them first), which keeps fewer values live across the star-state arithmetic.
   template <bool LH>
   static CHQ_FN void hlld(const St<T> &L, const St<T> &R, T bn, T gamma, T gm1, T f[NFLX],
                           T theta = T(1), T phi = T(1)) {
-    T fl[NFLX], fr[NFLX], el, er, ptl, ptr;
-    pflux(L, bn, gm1, el, ptl, fl);
-    pflux(R, bn, gm1, er, ptr, fr);
     const T cl = cfast(L, bn, gamma), cr = cfast(R, bn, gamma);
     T sl, sr;
     if constexpr (LH) {
@@ -270,8 +283,11 @@ template <class T> struct Rsv {
       sl = vmin(L.u - cl, R.u - cr);
       sr = vmax(L.u + cl, R.u + cr);
     }
-    if (sl >= T(0)) { for (int n = 0; n < NFLX; ++n) f[n] = fl[n]; return; }
-    if (sr <= T(0)) { for (int n = 0; n < NFLX; ++n) f[n] = fr[n]; return; }
+    if (sl >= T(0)) { T e, pt; pflux(L, bn, gm1, e, pt, f); return; }
+    if (sr <= T(0)) { T e, pt; pflux(R, bn, gm1, e, pt, f); return; }
+    T el, er, ptl, ptr;
+    ept(L, bn, gm1, el, ptl);
+    ept(R, bn, gm1, er, ptr);
     const T bn2 = bn * bn;
     const T al = sl - L.u, ar = sr - R.u;
     const T ml = L.d * al, mr = R.d * ar;
@@ -288,21 +304,25 @@ template <class T> struct Rsv {
     const T srs = sm + abn / qdr;
     const T uls[NFLX] = {dls, dls * sm, dls * vls, dls * wls, els, byls, bzls};
     const T urs[NFLX] = {drs, drs * sm, drs * vrs, drs * wrs, ers, byrs, bzrs};
-    T fls[NFLX], frs[NFLX];
-    if constexpr (LH) {
-      sflux(dls, sm, vls, wls, els, byls, bzls, bn, pts, fls);
-      sflux(drs, sm, vrs, wrs, ers, byrs, bzrs, bn, pts, frs);
-    } else {
-      const T ul[NFLX] = {L.d, L.d * L.u, L.d * L.v, L.d * L.w, el, L.by, L.bz};
-      const T ur[NFLX] = {R.d, R.d * R.u, R.d * R.v, R.d * R.w, er, R.by, R.bz};
-      for (int n = 0; n < NFLX; ++n) { fls[n] = fl[n] + sl * (uls[n] - ul[n]); frs[n] = fr[n] + sr * (urs[n] - ur[n]); }
-    }
-    if (sls > T(0)) { for (int n = 0; n < NFLX; ++n) f[n] = fls[n]; return; }
-    if (srs < T(0)) { for (int n = 0; n < NFLX; ++n) f[n] = frs[n]; return; }
+    auto flux_ls = [&](T o[NFLX]) {
+      if constexpr (LH) sflux(dls, sm, vls, wls, els, byls, bzls, bn, pts, o);
+      else fstar_hll(L, bn, gm1, sl, uls, o);
+    };
+    auto flux_rs = [&](T o[NFLX]) {
+      if constexpr (LH) sflux(drs, sm, vrs, wrs, ers, byrs, bzrs, bn, pts, o);
+      else fstar_hll(R, bn, gm1, sr, urs, o);
+    };
+    if (sls > T(0)) { flux_ls(f); return; }
+    if (srs < T(0)) { flux_rs(f); return; }
     if (bn2 <= Lit<T>::rel * (ptl + ptr)) {                      // riemann.hpp:199-207
-      if (sm > T(0)) { for (int n = 0; n < NFLX; ++n) f[n] = fls[n]; }
-      else if (sm < T(0)) { for (int n = 0; n < NFLX; ++n) f[n] = frs[n]; }
-      else { for (int n = 0; n < NFLX; ++n) f[n] = T(0.5) * (fls[n] + frs[n]); }
+      if (sm > T(0)) flux_ls(f);
+      else if (sm < T(0)) flux_rs(f);
+      else {
+        T fls[NFLX], frs[NFLX];
+        flux_ls(fls);
+        flux_rs(frs);
+        for (int n = 0; n < NFLX; ++n) f[n] = T(0.5) * (fls[n] + frs[n]);
+      }
       return;
     }
     const T s = sgn(bn);                                           // riemann.hpp:208-233
@@ -314,28 +334,30 @@ template <class T> struct Rsv {
     const T vbss = (sm * bn + vss * byss) + wss * bzss;
     const T elss = els - qdl * ((vbls - vbss) * s);
     const T erss = ers + qdr * ((vbrs - vbss) * s);
-    if (sm > T(0)) {
-      if constexpr (LH) sflux(dls, sm, vss, wss, elss, byss, bzss, bn, pts, f);
+    auto flux_lss = [&](T o[NFLX]) {
+      if constexpr (LH) sflux(dls, sm, vss, wss, elss, byss, bzss, bn, pts, o);
       else {
+        T fls[NFLX];
+        flux_ls(fls);
         const T ulss[NFLX] = {dls, dls * sm, dls * vss, dls * wss, elss, byss, bzss};
-        for (int n = 0; n < NFLX; ++n) f[n] = fls[n] + sls * (ulss[n] - uls[n]);
+        for (int n = 0; n < NFLX; ++n) o[n] = fls[n] + sls * (ulss[n] - uls[n]);
       }
-    } else if (sm < T(0)) {
-      if constexpr (LH) sflux(drs, sm, vss, wss, erss, byss, bzss, bn, pts, f);
+    };
+    auto flux_rss = [&](T o[NFLX]) {
+      if constexpr (LH) sflux(drs, sm, vss, wss, erss, byss, bzss, bn, pts, o);
       else {
+        T frs[NFLX];
+        flux_rs(frs);
         const T urss[NFLX] = {drs, drs * sm, drs * vss, drs * wss, erss, byss, bzss};
-        for (int n = 0; n < NFLX; ++n) f[n] = frs[n] + srs * (urss[n] - urs[n]);
+        for (int n = 0; n < NFLX; ++n) o[n] = frs[n] + srs * (urss[n] - urs[n]);
       }
-    } else {
+    };
+    if (sm > T(0)) flux_lss(f);
+    else if (sm < T(0)) flux_rss(f);
+    else {
       T a[NFLX], b[NFLX];
-      if constexpr (LH) {
-        sflux(dls, sm, vss, wss, elss, byss, bzss, bn, pts, a);
-        sflux(drs, sm, vss, wss, erss, byss, bzss, bn, pts, b);
-      } else {
-        const T ulss[NFLX] = {dls, dls * sm, dls * vss, dls * wss, elss, byss, bzss};
-        const T urss[NFLX] = {drs, drs * sm, drs * vss, drs * wss, erss, byss, bzss};
-        for (int n = 0; n < NFLX; ++n) { a[n] = fls[n] + sls * (ulss[n] - uls[n]); b[n] = frs[n] + srs * (urss[n] - urs[n]); }
-      }
+      flux_lss(a);
+      flux_rss(b);
       for (int n = 0; n < NFLX; ++n) f[n] = T(0.5) * (a[n] + b[n]);
     }
   }
