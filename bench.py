@@ -300,7 +300,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision side measurement")
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched Orszag-Tang ensemble measured beside the headline (0: skip)")
-    ap.add_argument("--serve-jobs", type=int, default=0, help="jobs of the served stream (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=100,
+                    help="jobs of the served stream measured beside the headline (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0,
                     help="Poisson arrival rate of the served stream, in multiples of the sequential FP64 rate")
     ap.add_argument("--serve-chunk", type=int, default=8, help="steps per serving-loop iteration")
