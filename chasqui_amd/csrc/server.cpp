@@ -281,6 +281,7 @@ int chq_server_pump(chq_server *s, int max_done, int *done_ids, int *n_done) {
     ++*n_done;
   };
   const double t0 = now_s();
+  const double poll0 = s->t_poll;   // the retire phase below excludes the time spent waiting on the device
   if (s->inflight) {
     std::vector<int> flags;
     for (auto &kv : s->batches) {
@@ -333,7 +334,7 @@ int chq_server_pump(chq_server *s, int max_done, int *done_ids, int *n_done) {
   s->to_free.clear();
   if (int e = join_started(s, false, report)) return e;   // finished setups join this chunk
   const double t1 = now_s();
-  s->t_retire += t1 - t0;
+  s->t_retire += (t1 - t0) - (s->t_poll - poll0);
   for (auto &kv : s->batches)
     if (chq_batch_size(kv.second) > 0) {
       if (int e = chq_batch_step(kv.second, s->chunk)) return e;
