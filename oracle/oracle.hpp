@@ -10,8 +10,7 @@
 // cites the reference file:line it follows. Min-reductions and integer counts are exact,
 // so results do not depend on the OpenMP thread count.
 //
-// Not restated (out of scope of the stage pipeline): gravity, cooling, the positivity flux
-// limiter.
+// Every physics option of the reference's Stage() and time loop is restated.
 // Requesting any of them raises an error instead of silently running something else.
 #pragma once
 #include <cstdint>
@@ -120,6 +119,12 @@ struct Cfg {   // physics.hpp:22-33 + the driver switches of main.cpp:112-163 th
   Real nu = 0.0, kappa = 0.0, eta = 0.0;   // diffusion::Coeffs (diffusion.hpp:18-21)
   bool sts = false;                        // RKL2 super-time-stepping of the diffusive terms
   bool diffusive() const { return nu > 0.0 || kappa > 0.0 || eta > 0.0; }
+  bool flux_limiter = false;               // Hu-Adams-Shu positivity flux limiter (main.cpp:141)
+  // gravity (srcterms.hpp:11-17) and cooling (45-50), main.cpp:148-162
+  bool grav_uniform = false, grav_point = false;
+  Real g1 = 0.0, g2 = 0.0, g3 = 0.0, gm = 0.0, gx0 = 0.0, gy0 = 0.0, gz0 = 0.0, rsoft = 0.0;
+  bool cooling = false;
+  Real lambda0 = 0.0, cool_alpha = 1.0, tfloor = 0.0;
 };
 
 // ---------------------------------------------------------------- mesh (mesh.hpp:13-71)
@@ -705,6 +710,130 @@ inline Real diffusion_dt(const Grid &m, const Cfg &cfg, const Diff &c) {
   return dxmin * dxmin / (2.0 * ndim * dmax);
 }
 
+// ---------------------------------------------------------------- positivity flux limiter
+// positivity::LimitFluxes<DIR> (positivity.hpp:24-89): each active face's high-order flux is
+// blended toward the Lax-Friedrichs flux with the largest theta (12-step bisection) for which
+// both one-sided half-updates keep rho >= dfloor and p >= pfloor (B held fixed).
+inline void cell_flux(const Prim &s, Real bn, Real sent, Real gm1, Real gamma, Real fc[6], Real uc[6], Real &cf) {
+  Real f[NF], e, pt;
+  phys_flux(s, bn, gm1, e, pt, f);
+  fc[0] = f[F_D]; fc[1] = f[F_MU]; fc[2] = f[F_MV]; fc[3] = f[F_MW]; fc[4] = f[F_E]; fc[5] = f[F_D] * sent;
+  uc[0] = s.d; uc[1] = s.d * s.u; uc[2] = s.d * s.v; uc[3] = s.d * s.w; uc[4] = e; uc[5] = s.d * sent;
+  cf = fast_speed(s, bn, gamma);
+}
+
+template <int DIR>
+long limit_fluxes(Grid &m, const Cfg &cfg, Real beta_dt) {
+  const Eos eos = cfg.eos;
+  constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
+  constexpr int B1 = (DIR + 1) % 3, B2 = (DIR + 2) % 3;
+  const int iu = DIR == 0 ? m.ie + 1 : m.ie, ju = DIR == 1 ? m.je + 1 : m.je, ku = DIR == 2 ? m.ke + 1 : m.ke;
+  const int ndim = 1 + (m.two_d ? 1 : 0) + (m.three_d ? 1 : 0);
+  const Real dx = DIR == 0 ? m.dx1 : DIR == 1 ? m.dx2 : m.dx3;
+  const Real lam2 = 2.0 * ndim * beta_dt / dx;
+  const A4 &w = m.w, &bcc = m.bcc;
+  A4 &flx = DIR == 0 ? m.f1 : DIR == 1 ? m.f2 : m.f3;
+  const int nv[6] = {IDN, IU, IV, IW, IEN, IENT};
+  long count = 0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : count)
+  for (int k = m.ks; k <= ku; ++k)
+    for (int j = m.js; j <= ju; ++j)
+      for (int i = m.is; i <= iu; ++i) {
+        const int kL = k - dk, jL = j - dj, iL = i - di;
+        const Prim L{w(IDN, kL, jL, iL), w(IU, kL, jL, iL), w(IV, kL, jL, iL), w(IW, kL, jL, iL), w(IPR, kL, jL, iL),
+                     bcc(B1, kL, jL, iL), bcc(B2, kL, jL, iL)};
+        const Prim R{w(IDN, k, j, i), w(IU, k, j, i), w(IV, k, j, i), w(IW, k, j, i), w(IPR, k, j, i),
+                     bcc(B1, k, j, i), bcc(B2, k, j, i)};
+        const Real bnL = bcc(DIR, kL, jL, iL), bnR = bcc(DIR, k, j, i);
+        Real fL[6], fR[6], uL[6], uR[6], cfL, cfR;
+        cell_flux(L, bnL, w(IENT, kL, jL, iL), eos.gm1, eos.gamma, fL, uL, cfL);
+        cell_flux(R, bnR, w(IENT, k, j, i), eos.gm1, eos.gamma, fR, uR, cfR);
+        const Real alpha = std::fmax(std::fabs(L.u) + cfL, std::fabs(R.u) + cfR);
+        Real fho[6], flf[6];
+        for (int n = 0; n < 6; ++n) {
+          fho[n] = flx(nv[n], k, j, i);
+          flf[n] = 0.5 * (fL[n] + fR[n]) - 0.5 * (alpha * (uR[n] - uL[n]));
+        }
+        const Real emL = 0.5 * (bnL * bnL + L.by * L.by + L.bz * L.bz);
+        const Real emR = 0.5 * (bnR * bnR + R.by * R.by + R.bz * R.bz);
+        auto ok = [&](Real th) {
+          Real f[6];
+          for (int n = 0; n < 6; ++n) f[n] = flf[n] + th * (fho[n] - flf[n]);
+          Real a[5], b[5];
+          for (int n = 0; n < 5; ++n) { a[n] = uL[n] - lam2 * (f[n] - fL[n]); b[n] = uR[n] + lam2 * (f[n] - fR[n]); }
+          if (a[0] < eos.dfloor || b[0] < eos.dfloor) return false;
+          const Real ea = a[4] - 0.5 * (a[1] * a[1] + a[2] * a[2] + a[3] * a[3]) / a[0] - emL;
+          const Real eb = b[4] - 0.5 * (b[1] * b[1] + b[2] * b[2] + b[3] * b[3]) / b[0] - emR;
+          Real pa = eos.gm1 * ea, pb = eos.gm1 * eb;
+          if (eos.dual_energy) {
+            const Real sa = (uL[5] - lam2 * (f[5] - fL[5])) / a[0], sb = (uR[5] + lam2 * (f[5] - fR[5])) / b[0];
+            if (ea < eos.de_eta * std::fabs(a[4])) pa = sa * std::pow(a[0], eos.gamma);
+            if (eb < eos.de_eta * std::fabs(b[4])) pb = sb * std::pow(b[0], eos.gamma);
+          }
+          return pa >= eos.pfloor && pb >= eos.pfloor;
+        };
+        if (ok(1.0)) continue;
+        Real lo = 0.0, hi = 1.0;
+        for (int it = 0; it < 12; ++it) {
+          const Real mid = 0.5 * (lo + hi);
+          if (ok(mid)) lo = mid; else hi = mid;
+        }
+        const Real th = lo;
+        for (int n = 0; n < 6; ++n) flx(nv[n], k, j, i) = flf[n] + th * (fho[n] - flf[n]);
+        ++count;
+      }
+  return count;
+}
+
+// ---------------------------------------------------------------- source terms
+// srcterms::ApplyGravity (srcterms.hpp:21-40): momentum and energy sources with beta*dt from
+// the stage primitives, uniform field and/or softened point mass
+inline void apply_gravity(Grid &m, const Cfg &c, A4 &u, Real beta_dt) {
+  if (!(c.grav_uniform || c.grav_point)) return;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = m.ks; k <= m.ke; ++k)
+    for (int j = m.js; j <= m.je; ++j)
+      for (int i = m.is; i <= m.ie; ++i) {
+        Real a1 = c.g1, a2 = c.g2, a3 = c.g3;
+        if (c.grav_point) {
+          const Real dx = m.x1v(i) - c.gx0, dy = m.two_d ? m.x2v(j) - c.gy0 : 0.0, dz = m.three_d ? m.x3v(k) - c.gz0 : 0.0;
+          const Real r2 = dx * dx + dy * dy + dz * dz + c.rsoft * c.rsoft;
+          const Real f = -c.gm / (r2 * std::sqrt(r2));
+          a1 += f * dx; a2 += f * dy; a3 += f * dz;
+        }
+        const Real d = m.w(IDN, k, j, i);
+        u(IM1, k, j, i) += beta_dt * d * a1;
+        u(IM2, k, j, i) += beta_dt * d * a2;
+        u(IM3, k, j, i) += beta_dt * d * a3;
+        u(IEN, k, j, i) += beta_dt * d * (m.w(IVX, k, j, i) * a1 + m.w(IVY, k, j, i) * a2 + m.w(IVZ, k, j, i) * a3);
+      }
+}
+
+// srcterms::ApplyCooling (srcterms.hpp:55-71): exact power-law cooling of T = p / rho over dt
+inline void apply_cooling(Grid &m, const Cfg &c, A4 &u, Real dt) {
+  if (!c.cooling) return;
+  const Real gm1 = c.eos.gm1;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = m.ks; k <= m.ke; ++k)
+    for (int j = m.js; j <= m.je; ++j)
+      for (int i = m.is; i <= m.ie; ++i) {
+        const Real d = m.w(IDN, k, j, i), p = m.w(IPR, k, j, i);
+        const Real t0 = p / d;
+        const Real cc = gm1 * d * c.lambda0;
+        Real t1;
+        if (std::fabs(c.cool_alpha - 1.0) < 1e-12) {
+          t1 = t0 * std::exp(-cc * dt);
+        } else {
+          const Real e = 1.0 - c.cool_alpha;
+          const Real base = std::pow(t0, e) + (c.cool_alpha - 1.0) * cc * dt;
+          t1 = base > 0.0 ? std::pow(base, 1.0 / e) : c.tfloor;
+        }
+        t1 = std::fmax(t1, c.tfloor);
+        u(IEN, k, j, i) += d * (t1 - t0) / gm1;
+      }
+}
+
 // ---------------------------------------------------------------- fofc
 // flag cells whose tentative rho or p is at/below the floors, fofc.hpp:17-44
 inline int fofc_flag(Grid &m, const Cfg &cfg, int s) {
@@ -805,17 +934,25 @@ inline StageCoef stage_coef(int nstage, int s) {
   return {0, 1, 0.5, 1, 0};
 }
 
-// Stage(), main.cpp:46-93 (explicit diffusion in the stage unless super-time-stepping)
-inline long stage(Grid &m, const Cfg &cfg, int cur, int out, Real g0, Real g1, Real bdt) {
+// Stage(), main.cpp:46-93 (explicit diffusion in the stage unless super-time-stepping);
+// *limited (may be null) accumulates the flux-limiter activations
+inline long stage(Grid &m, const Cfg &cfg, int cur, int out, Real g0, Real g1, Real bdt, long *limited = nullptr) {
   const A3 &b1c = m.bf(cur, 0), &b2c = m.bf(cur, 1), &b3c = m.bf(cur, 2);
   const Diff dc = cfg.sts ? Diff{} : diff_of(cfg);
   fluxes_any(m, cfg, 0, b1c);
   if (m.two_d) fluxes_any(m, cfg, 1, b2c);
   if (m.three_d) fluxes_any(m, cfg, 2, b3c);
   diffusion_fluxes_all(m, dc, b1c, b2c, b3c);                      // main.cpp:55-59
+  if (cfg.flux_limiter) {                                          // main.cpp:60-64
+    long n = limit_fluxes<0>(m, cfg, bdt);
+    if (m.two_d) n += limit_fluxes<1>(m, cfg, bdt);
+    if (m.three_d) n += limit_fluxes<2>(m, cfg, bdt);
+    if (limited) *limited += n;
+  }
   corner_emf(m);
   resistive_emf(m, dc, b1c, b2c, b3c);                             // main.cpp:66
   update(m, out, g0, g1, bdt);
+  apply_gravity(m, cfg, m.u(out), bdt);                            // main.cpp:76
   long nbad = 0;
   if (cfg.fofc) {
     nbad = fofc_flag(m, cfg, out);
@@ -826,6 +963,7 @@ inline long stage(Grid &m, const Cfg &cfg, int cur, int out, Real g0, Real g1, R
       corner_emf(m);
       resistive_emf(m, dc, b1c, b2c, b3c);
       update(m, out, g0, g1, bdt);
+      apply_gravity(m, cfg, m.u(out), bdt);                        // main.cpp:89
     }
   }
   boundaries(m, cfg, out);
@@ -1147,6 +1285,48 @@ inline void pg_shock_tube(Deck &p, const Grid &m, Init &h) {
       }
 }
 
+// uniform state (gravity and cooling tests), pgen.hpp:216-229
+inline void pg_uniform(Deck &p, const Grid &m, Init &h) {
+  const Real d = p.real("problem", "dens", 1.0), pr = p.real("problem", "pres", 1.0);
+  const Real vx = p.real("problem", "vx", 0.0), vy = p.real("problem", "vy", 0.0), vz = p.real("problem", "vz", 0.0);
+  const Real bx = p.real("problem", "bx", 0.0), by = p.real("problem", "by", 0.0), bz = p.real("problem", "bz", 0.0);
+  for (int k = 0; k < m.nk + 1; ++k)
+    for (int j = 0; j < m.nj + 1; ++j)
+      for (int i = 0; i < m.ni + 1; ++i) { h.b1(k, j, i) = bx; h.b2(k, j, i) = by; h.b3(k, j, i) = bz; }
+  for (int k = 0; k < m.nk; ++k)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        h.w(IDN, k, j, i) = d; h.w(IVX, k, j, i) = vx; h.w(IVY, k, j, i) = vy; h.w(IVZ, k, j, i) = vz;
+        h.w(IPR, k, j, i) = pr;
+      }
+}
+
+// Gresho vortex at a prescribed Mach number, pgen.hpp:188-214
+inline void pg_gresho(Deck &p, const Grid &m, Init &h) {
+  const Real mach = p.real("problem", "mach", 0.1);
+  const Real gamma = p.real("mhd", "gamma", 5.0 / 3.0);
+  const Real p0 = 1.0 / (gamma * mach * mach);
+  for (int k = 0; k < m.nk + 1; ++k)
+    for (int j = 0; j < m.nj + 1; ++j)
+      for (int i = 0; i < m.ni + 1; ++i) { h.b1(k, j, i) = 0.0; h.b2(k, j, i) = 0.0; h.b3(k, j, i) = 0.0; }
+  for (int k = 0; k < m.nk; ++k)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        const Real x = m.x1v(i), y = m.two_d ? m.x2v(j) : 0.0;
+        const Real r = std::sqrt(x * x + y * y);
+        Real vphi, pr;
+        if (r < 0.2) { vphi = 5.0 * r; pr = p0 + 12.5 * r * r; }
+        else if (r < 0.4) { vphi = 2.0 - 5.0 * r; pr = p0 + 12.5 * r * r - 20.0 * r + 4.0 * std::log(r / 0.2) + 4.0; }
+        else { vphi = 0.0; pr = p0 - 2.0 + 4.0 * std::log(2.0); }
+        const Real ir = r > 1e-14 ? 1.0 / r : 0.0;
+        h.w(IDN, k, j, i) = 1.0;
+        h.w(IVX, k, j, i) = -vphi * y * ir;
+        h.w(IVY, k, j, i) = vphi * x * ir;
+        h.w(IVZ, k, j, i) = 0.0;
+        h.w(IPR, k, j, i) = pr;
+      }
+}
+
 // sinusoidal decay test of one diffusive term along one axis, pgen.hpp:235-266: var = b
 // (B_{axis+1} on faces, uniform total pressure), v (v_{axis+1}) or temp (T at uniform p)
 inline void pg_diffusion_test(Deck &p, const Grid &m, Init &h) {
@@ -1190,6 +1370,8 @@ inline void initialize(const std::string &name, Deck &p, Grid &m, const Cfg &cfg
   else if (name == "blast") pg_blast(p, m, h);
   else if (name == "shock_tube") pg_shock_tube(p, m, h);
   else if (name == "diffusion_test") pg_diffusion_test(p, m, h);
+  else if (name == "uniform") pg_uniform(p, m, h);
+  else if (name == "gresho") pg_gresho(p, m, h);
   else throw std::runtime_error("unknown problem: " + name);
   for (int k = 0; k < m.nk; ++k)
     for (int j = 0; j < m.nj; ++j)
@@ -1223,7 +1405,7 @@ struct Sim {
   Grid m;
   Real t = 0.0, tlim = 1.0, min_dt = 0.0, last_dt = 0.0;
   int cycle = 0, nlim = -1;
-  long fofc_count = 0, sts_count = 0;
+  long fofc_count = 0, sts_count = 0, limiter_count = 0;
   StsWork sts_work;
   std::string problem;
   A4 u_init, bcc_init;
@@ -1256,14 +1438,25 @@ struct Sim {
     cfg.bc[1] = parse_bc(p.str("mesh", "bc_x2", "periodic"));
     cfg.bc[2] = parse_bc(p.str("mesh", "bc_x3", "periodic"));
     cfg.fofc = p.boolean("mhd", "fofc", true);
-    if (p.boolean("mhd", "flux_limiter", false)) throw std::runtime_error("flux_limiter is not restated");
+    cfg.flux_limiter = p.boolean("mhd", "flux_limiter", false);
     cfg.sts = p.boolean("diffusion", "sts", false);                  // main.cpp:142-146
     cfg.nu = p.real("diffusion", "nu", 0.0);
     cfg.kappa = p.real("diffusion", "kappa", 0.0);
     cfg.eta = p.real("diffusion", "eta", 0.0);
-    if (p.boolean("gravity", "uniform", false) || p.boolean("gravity", "point_mass", false) ||
-        p.boolean("cooling", "enabled", false))
-      throw std::runtime_error("source terms are not restated");
+    cfg.grav_uniform = p.boolean("gravity", "uniform", false);          // main.cpp:148-162
+    cfg.g1 = p.real("gravity", "g1", 0.0);
+    cfg.g2 = p.real("gravity", "g2", 0.0);
+    cfg.g3 = p.real("gravity", "g3", 0.0);
+    cfg.grav_point = p.boolean("gravity", "point_mass", false);
+    cfg.gm = p.real("gravity", "gm", 0.0);
+    cfg.gx0 = p.real("gravity", "x0", 0.0);
+    cfg.gy0 = p.real("gravity", "y0", 0.0);
+    cfg.gz0 = p.real("gravity", "z0", 0.0);
+    cfg.rsoft = p.real("gravity", "rsoft", 0.0);
+    cfg.cooling = p.boolean("cooling", "enabled", false);
+    cfg.lambda0 = p.real("cooling", "lambda0", 0.0);
+    cfg.cool_alpha = p.real("cooling", "alpha", 1.0);
+    cfg.tfloor = p.real("cooling", "tfloor", 0.0);
     m.nx1 = p.integer("mesh", "nx1");
     m.nx2 = p.integer("mesh", "nx2", 1);
     m.nx3 = p.integer("mesh", "nx3", 1);
@@ -1294,15 +1487,23 @@ struct Sim {
     for (int s = 1; s <= cfg.nstage; ++s) {
       const StageCoef c = stage_coef(cfg.nstage, s);
       const bool last = s == cfg.nstage;
-      fofc_count += stage(m, cfg, c.cur, c.out, c.g0, c.g1, c.beta * dt);
-      const bool split = last && cfg.sts;                              // main.cpp:273-275
+      fofc_count += stage(m, cfg, c.cur, c.out, c.g0, c.g1, c.beta * dt, &limiter_count);
+      const bool split = last && (cfg.sts || cfg.cooling);             // main.cpp:273-275
       const Real mdt = cons_to_prim(m, cfg, c.out, last && !split);
       if (last && !split) min_dt = std::fmin(mdt, diffusion_dt(m, cfg, diff_of(cfg)));
     }
-    if (cfg.sts) {                                                     // main.cpp:278-290
+    if (cfg.sts) {                                                     // main.cpp:278-281
       sts_count += sts_advance(m, cfg, diff_of(cfg), sts_work, dt, diffusion_dt(m, cfg, diff_of(cfg)));
       boundaries(m, cfg, 0);
+    }
+    if (cfg.cooling) {                                                 // main.cpp:282-286
+      cons_to_prim(m, cfg, 0, false);
+      apply_cooling(m, cfg, m.u0, dt);
+      boundaries(m, cfg, 0);
+    }
+    if (cfg.sts || cfg.cooling) {                                      // main.cpp:287-290
       min_dt = cons_to_prim(m, cfg, 0, true);
+      if (!cfg.sts) min_dt = std::fmin(min_dt, diffusion_dt(m, cfg, diff_of(cfg)));
     }
     t += dt;
     ++cycle;

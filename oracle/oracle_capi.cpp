@@ -139,6 +139,7 @@ int orc_fofc_repair(void *p, int dir, int cur) {
   fofc_repair_any(s.m, s.cfg, dir, cur, s.cfg.sts ? Diff{} : diff_of(s.cfg));
   return 0;
 }
+long orc_limiter_count(void *p) { return S(p)->limiter_count; }
 long orc_stage(void *p, int cur, int out, double g0, double g1, double bdt) {
   Sim &s = *S(p);
   return stage(s.m, s.cfg, cur, out, g0, g1, bdt);

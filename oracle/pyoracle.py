@@ -64,6 +64,8 @@ def lib():
     L.orc_diffusion_dt.argtypes = [vp]
     L.orc_sts_count.restype = C.c_long
     L.orc_sts_count.argtypes = [vp]
+    L.orc_limiter_count.restype = C.c_long
+    L.orc_limiter_count.argtypes = [vp]
     L.orc_stage.restype = C.c_long
     L.orc_stage.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
     L.orc_set_floors.argtypes = [vp, C.c_double, C.c_double]
@@ -228,6 +230,10 @@ class OracleSim:
     @property
     def sts_count(self):
         return self.L.orc_sts_count(self.h)
+
+    @property
+    def limiter_count(self):
+        return self.L.orc_limiter_count(self.h)
 
     def stage(self, cur, out, g0, g1, bdt):
         return self.L.orc_stage(self.h, cur, out, g0, g1, bdt)

@@ -110,3 +110,104 @@ def test_resistive_mhd_keeps_div_b_and_conserves():
         assert div_b(o) < 5e-13
         t1 = totals(o)
         assert np.all(np.abs(t1[:5] - t0[:5]) <= 1e-12 * np.maximum(np.abs(t0[:5]), 1.0))
+
+
+# ---------------------------------------------------------------- source terms, flux limiter
+UNI_DECK = """[mesh]
+nx1 = 16
+nx2 = 16
+nx3 = 16
+[time]
+tlim = 1.0
+[mhd]
+reconstruct = plm
+[problem]
+name = uniform
+dens = 2.0
+pres = 1.0
+"""
+
+
+def test_uniform_gravity_momentum():
+    """physics.sh:53, 84: a uniform box under uniform gravity gains momentum rho g t V exactly
+    (history mom1 = 1, mom3 = 2 to 1e-9)."""
+    o = po.OracleSim(UNI_DECK, ["gravity.uniform=true", "gravity.g1=0.5", "gravity.g2=-0.25", "gravity.g3=1.0"])
+    o.run()
+    g, r = o.geometry()
+    dv = r[0] * r[1] * r[2]
+    u = o.get("u0")[:, o.ks:o.ke + 1, o.js:o.je + 1, o.is_:o.ie + 1]
+    mom = u[1:4].reshape(3, -1).sum(axis=1) * dv
+    assert abs(mom[0] - 1.0) < 1e-9 and abs(mom[1] + 0.5) < 1e-9 and abs(mom[2] - 2.0) < 1e-9, mom
+
+
+def test_power_law_cooling_exact():
+    """physics.sh:54, 86: exact power-law cooling, T(1) = (sqrt(T0^(1/2)) - c/2)^2 for alpha = 1/2
+    with c = (gamma-1) rho lambda0, to 1e-6; and alpha = 1 decays like exp(-c t)."""
+    g = 5.0 / 3.0
+    o = po.OracleSim(UNI_DECK, ["cooling.enabled=true", "cooling.lambda0=0.1", "cooling.alpha=0.5"])
+    o.run()
+    w = o.get("w")[:, o.ks:o.ke + 1, o.js:o.je + 1, o.is_:o.ie + 1]
+    c = (g - 1) * 2.0 * 0.1
+    assert abs((w[4] / w[0]).mean() / (math.sqrt(0.5) - 0.5 * c) ** 2 - 1) < 1e-6
+    o = po.OracleSim(UNI_DECK, ["cooling.enabled=true", "cooling.lambda0=0.1", "cooling.alpha=1.0"])
+    o.run()
+    w = o.get("w")[:, o.ks:o.ke + 1, o.js:o.je + 1, o.is_:o.ie + 1]
+    assert abs((w[4] / w[0]).mean() / (0.5 * math.exp(-c)) - 1) < 1e-12
+
+
+def test_point_mass_gravity_is_radial():
+    """A softened point mass at the box centre (srcterms.hpp:25-31) pulls a uniform box
+    symmetrically: the momentum stays zero and the kinetic energy grows."""
+    ov = ["gravity.point_mass=true", "gravity.gm=0.05", "gravity.rsoft=0.05", "gravity.x0=0.5", "gravity.y0=0.5",
+          "gravity.z0=0.5", "time.nlim=10"]
+    o = po.OracleSim(UNI_DECK, ov)
+    o.step(10)
+    u = o.get("u0")[:, o.ks:o.ke + 1, o.js:o.je + 1, o.is_:o.ie + 1]
+    assert np.max(np.abs(u[1:4].reshape(3, -1).sum(axis=1))) < 1e-12
+    assert np.max(np.abs(u[1])) > 1e-4
+
+
+def test_flux_limiter_idle_on_smooth_flow():
+    """positivity.hpp:74: the high-order flux is kept wherever it is already safe; on a smooth
+    wave nothing is limited and the run is bitwise the unlimited one."""
+    a = po.OracleSim(deck("cpaw_1d.in"), ["mesh.nx1=64", "mhd.flux_limiter=true"])
+    b = po.OracleSim(deck("cpaw_1d.in"), ["mesh.nx1=64"])
+    a.step(20)
+    b.step(20)
+    assert a.limiter_count == 0
+    assert np.array_equal(a.get("u0"), b.get("u0"))
+
+
+def test_flux_limiter_keeps_a_strong_blast_positive():
+    """A strong blast without fofc: the Hu-Adams-Shu limiter (positivity.hpp:1-8) activates and
+    keeps density and pressure positive."""
+    ov = ["mesh.nx1=24", "mesh.nx2=24", "mesh.nx3=1", "problem.pres_in=1e4", "problem.pres_out=1e-4", "mhd.fofc=false",
+          "mhd.flux_limiter=true", "time.cfl=0.4"]
+    o = po.OracleSim(deck("blast_3d.in"), ov)
+    o.step(10)
+    assert o.limiter_count > 0
+    w = o.get("w")[:, o.ks:o.ke + 1, o.js:o.je + 1, o.is_:o.ie + 1]
+    assert w[0].min() > 0 and w[4].min() > 0
+
+
+def test_gresho_vortex_setup():
+    """pgen.hpp:188-214: pressure equilibrium of the Gresho vortex at Mach 0.1, v_max = 1 at r = 0.2."""
+    text = "[mesh]\nnx1 = 40\nnx2 = 40\nx1min = -0.5\nx1max = 0.5\nx2min = -0.5\nx2max = 0.5\n[problem]\nname = gresho\n"
+    o = po.OracleSim(text, [], pgen_only=True)
+    w = o.get("w")
+    g, r_ = o.geometry()
+    dx = r_[0]
+    x = -0.5 + (np.arange(w.shape[3]) - o.is_ + 0.5) * dx
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    R = np.hypot(X, Y)
+    p0 = 1.0 / (5.0 / 3.0 * 0.01)
+    pe = np.where(R < 0.2, p0 + 12.5 * R * R,
+                  np.where(R < 0.4, p0 + 12.5 * R * R - 20.0 * R + 4.0 * np.log(R / 0.2) + 4.0, p0 - 2.0 + 4.0 * np.log(2.0)))
+    assert np.max(np.abs(w[4][0] - pe)) < 1e-12 * p0
+    vphi = np.where(R < 0.2, 5.0 * R, np.where(R < 0.4, 2.0 - 5.0 * R, 0.0))
+    assert np.max(np.abs(np.hypot(w[1][0], w[2][0]) - vphi)) < 1e-14
+    # radial equilibrium dp/dr = v^2 / r of the sampled profile (centred differences in the ring)
+    ring = ((R > 0.06) & (R < 0.15)) | ((R > 0.25) & (R < 0.35))   # away from the kinks at 0.2, 0.4
+    dpdx, dpdy = np.gradient(w[4][0], dx, dx, edge_order=2)[1], np.gradient(w[4][0], dx, dx, edge_order=2)[0]
+    dpdr = (dpdx * X + dpdy * Y) / R
+    assert np.max(np.abs(dpdr[ring] - (vphi ** 2 / R)[ring])) < 0.02 * np.max(vphi ** 2 / R)
