@@ -40,13 +40,17 @@ class Config(C.Structure):
                 ("fofc", C.c_int), ("dual_energy", C.c_int), ("de_eta", C.c_double),
                 ("lhlld_phi", C.c_double), ("bc", C.c_int * 3), ("precision", C.c_int),
                 ("characteristic", C.c_int), ("nu", C.c_double), ("kappa", C.c_double),
-                ("eta", C.c_double), ("sts", C.c_int)]
+                ("eta", C.c_double), ("sts", C.c_int), ("flux_limiter", C.c_int),
+                ("grav_uniform", C.c_int), ("grav_point", C.c_int), ("g1", C.c_double), ("g2", C.c_double),
+                ("g3", C.c_double), ("gm", C.c_double), ("gx0", C.c_double), ("gy0", C.c_double),
+                ("gz0", C.c_double), ("rsoft", C.c_double), ("cooling", C.c_int), ("lambda0", C.c_double),
+                ("cool_alpha", C.c_double), ("tfloor", C.c_double)]
 
 
 class Status(C.Structure):
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("tlim", C.c_double), ("min_dt", C.c_double),
                 ("cycle", C.c_int), ("nlim", C.c_int), ("active", C.c_int), ("nbad_last", C.c_int),
-                ("fofc_total", C.c_longlong), ("sts_stages", C.c_longlong)]
+                ("fofc_total", C.c_longlong), ("sts_stages", C.c_longlong), ("limiter_total", C.c_longlong)]
 
 
 _lib = None

@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-HIP_SOURCES = ["flux.hip", "flux32.hip", "flux_char.hip", "stage.hip", "layout.hip", "diffusion.hip"]
+HIP_SOURCES = ["flux.hip", "flux32.hip", "flux_char.hip", "stage.hip", "layout.hip", "diffusion.hip", "srcterms.hip"]
 EXTRA = {"flux32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
                         "-fgpu-flush-denormals-to-zero"]}
 HOST_SOURCES = ["capi.cpp", "batch.cpp", "server.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]

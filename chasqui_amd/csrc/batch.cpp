@@ -399,6 +399,8 @@ int chq_batch_add(chq_batch *b, chq_ctx *c) {
   if (c->cfg.characteristic && c->cfg.recon != 0)
     return fail(CHQ_EUNSUP, "batched runs need primitive-variable reconstruction");
   if (c->diffusive() || c->cfg.sts) return fail(CHQ_EUNSUP, "batched runs need diffusion off");
+  if (c->cfg.flux_limiter || c->gravity() || c->cfg.cooling)
+    return fail(CHQ_EUNSUP, "batched runs need the flux limiter and source terms off");
   if (c->device != b->device) return fail(CHQ_EINVAL, "context and batch are on different devices");
   const int prec = c->mixed ? 1 : 0;
   if (!b->keyed) {

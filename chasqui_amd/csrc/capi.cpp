@@ -150,6 +150,8 @@ int check_cfg(const chq_mesh_desc *m, const chq_config *c) {
     return fail(CHQ_EUNSUP, "mixed precision with characteristic reconstruction is not supported");
   if (c->precision == 1 && (c->nu > 0.0 || c->kappa > 0.0 || c->eta > 0.0 || c->sts))
     return fail(CHQ_EUNSUP, "mixed precision with diffusion is not supported");
+  if (c->precision == 1 && (c->flux_limiter || c->grav_uniform || c->grav_point || c->cooling))
+    return fail(CHQ_EUNSUP, "mixed precision with the flux limiter or source terms is not supported");
   if (c->rsolver < 0 || c->rsolver > 2) return fail(CHQ_EINVAL, "unknown Riemann solver");
   if (c->nstage < 1 || c->nstage > 4) return fail(CHQ_EINVAL, "nstage must be 1..4");
   const int ngmin = (c->recon == 2 || c->recon == 3 || c->recon == 4) ? 3 : 2;   // main.cpp:169-171
@@ -179,6 +181,7 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   c->cfg = *cfg;
   Geom &g = c->g;
   fill_geom(m, g);
+  c->x1min = m->x1min; c->x2min = m->x2min; c->x3min = m->x3min;
   c->eos.gamma = cfg->gamma;
   c->eos.gm1 = cfg->gamma - 1.0;
   c->eos.dfloor = cfg->dfloor;
@@ -574,13 +577,17 @@ int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt
   const Recon rc = Recon(c->cfg.recon);
   const RSolver rs = RSolver(c->cfg.rsolver);
   const DiffArgs da = c->diff_args(cur, c->wcur, k, true);
+  const SrcArgs sa = c->src_args(out, c->wcur, k, beta_dt, false);
   for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_flux(c->flux_args(d, cur, c->wcur, k), d, rc, rs, true, c->stream));
   if (int rc_ = enqueue_diff_fluxes(c, da, false)) return rc_;
+  if (c->cfg.flux_limiter)
+    for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_limit_fluxes(sa, d, c->stream));
   HIPCHK(launch_corner<double>(c->corner_args(c->wcur, k, 0), c->stream));
   if (da.eta > 0.0) HIPCHK(launch_resistive_emf(da, c->stream));
   UpdateArgs ua = c->update_args(out, g0, g1, 6, k, 0);
   ua.beta_dt = beta_dt;
   HIPCHK(launch_update<double>(ua, c->stream));
+  if (c->gravity()) HIPCHK(launch_gravity(sa, c->stream));
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 0, c->stream));
   if (c->cfg.fofc) {
     HIPCHK(launch_flag(c->c2p_args(out, c->wcur, c->ctrl_api), c->stream));
@@ -595,6 +602,11 @@ int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt
     UpdateArgs ub = c->update_args(out, g0, g1, 6, k, 1);
     ub.beta_dt = beta_dt;
     HIPCHK(launch_update<double>(ub, c->stream));
+    if (c->gravity()) {
+      SrcArgs sb = sa;
+      sb.only_if_bad = 1;
+      HIPCHK(launch_gravity(sb, c->stream));
+    }
   }
   HIPCHK(c->boundaries(out, 6, k, 0));
   if (nbad) {
@@ -709,8 +721,10 @@ static int enqueue_step(chq_ctx *c) {
   const RSolver rs = RSolver(c->cfg.rsolver);
   Ctrl *k = c->ctrl;
   const int nst = c->cfg.nstage;
-  // dual energy, diffusion and super-time-stepping run in the reference kernel order
-  const bool fused = !c->eos.dual_energy && !c->diffusive() && !c->cfg.sts;
+  // dual energy, diffusion, super-time-stepping, the flux limiter and the source terms run in
+  // the reference kernel order
+  const bool split = c->cfg.sts || c->cfg.cooling;   // main.cpp:273-290: dt taken after the splits
+  const bool fused = !c->eos.dual_energy && !c->diffusive() && !split && !c->cfg.flux_limiter && !c->gravity();
   const int nvar = c->entropy ? 6 : 5;
   HIPCHK(launch_ctrl_begin_step(k, c->stream));
   for (int s = 1; s <= nst; ++s) {
@@ -725,18 +739,22 @@ static int enqueue_step(chq_ctx *c) {
     } else {
       const int wc = c->wcur;
       const DiffArgs da = c->diff_args(cf.cur, wc, k, true);
+      const SrcArgs sa = c->src_args(cf.out, wc, k, cf.beta, true);
       for (int d = 0; d < c->ndir(); ++d) {
         FluxArgs fa = c->flux_args(d, cf.cur, wc, k);
         fa.entropy = c->entropy;
         HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
       }
       if (int rc_ = enqueue_diff_fluxes(c, da, false)) return rc_;        // main.cpp:55-59
+      if (c->cfg.flux_limiter)                                             // main.cpp:60-64
+        for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_limit_fluxes(sa, d, c->stream));
       HIPCHK(launch_corner<double>(c->corner_args(wc, k, 0), c->stream));
       if (da.eta > 0.0) HIPCHK(launch_resistive_emf(da, c->stream));     // main.cpp:66
       UpdateArgs ua = c->update_args(cf.out, cf.g0, cf.g1, nvar, k, 0);
       ua.use_ctrl_dt = 1;
       ua.beta = cf.beta;
       HIPCHK(launch_update<double>(ua, c->stream));
+      if (c->gravity()) HIPCHK(launch_gravity(sa, c->stream));           // main.cpp:76
       if (c->cfg.fofc) {
         HIPCHK(launch_flag(c->c2p_args(cf.out, wc, k), c->stream));
         for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_repair(c->flux_args(d, cf.cur, wc, k), d, true, c->stream));
@@ -751,15 +769,20 @@ static int enqueue_step(chq_ctx *c) {
         ub.use_ctrl_dt = 1;
         ub.beta = cf.beta;
         HIPCHK(launch_update<double>(ub, c->stream));
+        if (c->gravity()) {                                                // main.cpp:89
+          SrcArgs sb = sa;
+          sb.only_if_bad = 1;
+          HIPCHK(launch_gravity(sb, c->stream));
+        }
       }
       HIPCHK(c->boundaries(cf.out, nvar, k, 0));
       C2pArgs ca = c->c2p_args(cf.out, wc, k);
-      ca.compute_dt = last && !c->cfg.sts;   // main.cpp:273-274: split steps take dt after the STS
+      ca.compute_dt = last && !split;   // main.cpp:273-274: split steps take dt after the splits
       ca.slot = 0;
       HIPCHK(launch_c2p<double>(ca, c->stream));
     }
     HIPCHK(launch_ctrl_end_stage(k, last, fused ? 1 : 0, c->stream));
-    if (last && !fused && !c->cfg.sts && c->diffusive()) {   // main.cpp:275
+    if (last && !fused && !split && c->diffusive()) {   // main.cpp:275
       HIPCHK(launch_rho_min(c->f64.w[c->wcur], c->g, k, true, c->stream));
       HIPCHK(launch_diff_dt_fold(k, c->diff_args(0, c->wcur, k, false), c->stream));
     }
@@ -776,12 +799,26 @@ static int enqueue_step(chq_ctx *c) {
       if (int rc_ = sts_advance(c, k, c->wcur, h.dt, parabolic_dt(c, rho), &s)) return rc_;
       c->sts_stages += s;
       HIPCHK(c->boundaries(0, NVAR, k, 0));
-      HIPCHK(launch_ctrl_begin_stage(k, 1, c->stream));
-      C2pArgs ca = c->c2p_args(0, c->wcur, k);
-      ca.compute_dt = 1;
-      ca.slot = 0;
-      HIPCHK(launch_c2p<double>(ca, c->stream));
-      HIPCHK(launch_ctrl_take_min(k, 0, c->stream));
+    }
+  }
+  if (c->cfg.cooling) {   // main.cpp:282-286, with the step's dt (the kernels exit when the loop has ended)
+    C2pArgs ca = c->c2p_args(0, c->wcur, k);
+    ca.compute_dt = 0;
+    ca.slot = 0;
+    HIPCHK(launch_c2p<double>(ca, c->stream));
+    HIPCHK(launch_cooling(c->src_args(0, c->wcur, k, 1.0, true), c->stream));
+    HIPCHK(c->boundaries(0, NVAR, k, 0));
+  }
+  if (split) {            // main.cpp:287-290
+    HIPCHK(launch_ctrl_begin_stage(k, 1, c->stream));
+    C2pArgs ca = c->c2p_args(0, c->wcur, k);
+    ca.compute_dt = 1;
+    ca.slot = 0;
+    HIPCHK(launch_c2p<double>(ca, c->stream));
+    HIPCHK(launch_ctrl_take_min(k, 0, c->stream));
+    if (!c->cfg.sts && c->diffusive()) {
+      HIPCHK(launch_rho_min(c->f64.w[c->wcur], c->g, k, true, c->stream));
+      HIPCHK(launch_diff_dt_fold(k, c->diff_args(0, c->wcur, k, false), c->stream));
     }
   }
   HIPCHK(launch_ctrl_end_step(k, c->stream));
@@ -855,6 +892,7 @@ int chq_get_status(chq_ctx *c, chq_status *st) {
   st->nbad_last = h.nbad_last;
   st->fofc_total = h.fofc_total;
   st->sts_stages = c->sts_stages;
+  st->limiter_total = h.limiter_total;
   return CHQ_OK;
 }
 

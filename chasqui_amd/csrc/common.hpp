@@ -51,6 +51,7 @@ struct Ctrl {
   int wcur;                      // current primitive set (batched runs; solo contexts track it on the host)
   long long fofc_total;
   unsigned long long rho_bits;   // minimum active density of the last con2prim (parabolic dt)
+  long long limiter_total;       // positivity flux-limiter activations
 };
 
 inline __host__ __device__ int div_up(int a, int b) { return (a + b - 1) / b; }

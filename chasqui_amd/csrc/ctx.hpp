@@ -74,6 +74,7 @@ struct chq_ctx {
   double *sts_buf = nullptr;
   double *sy[3]{}, *sm0 = nullptr, *smj = nullptr, *syb[3][3]{}, *smb0[3]{}, *smbj[3]{};
   long long sts_stages = 0;  // RKL2 stages taken so far
+  double x1min = 0.0, x2min = 0.0, x3min = 0.0;   // domain origin (point-mass gravity)
   // per-kernel-class timing: event pairs recorded around launches of one class
   int timing_class = -1;
   std::vector<hipEvent_t> tev;   // begin/end pairs
@@ -198,6 +199,27 @@ struct chq_ctx {
     return diff_args(bf[cur][0], bf[cur][1], bf[cur][2], wset, c, in_stage);
   }
   bool diff_any(const DiffArgs &a) const { return a.nu > 0.0 || a.kappa > 0.0 || a.eta > 0.0; }
+  bool gravity() const { return cfg.grav_uniform || cfg.grav_point; }
+  // flux limiter / gravity / cooling launch description: sources act on u[out] with the
+  // primitives of set `wset`; the step weight is beta * ctrl->dt (use_ctrl_dt) or beta itself
+  SrcArgs src_args(int out, int wset, const Ctrl *c, double beta, bool use_ctrl_dt) const {
+    SrcArgs a{};
+    a.g = g;
+    a.eos = eos;
+    a.x1min = x1min; a.x2min = x2min; a.x3min = x3min;
+    a.grav_uniform = cfg.grav_uniform; a.grav_point = cfg.grav_point;
+    a.g1 = cfg.g1; a.g2 = cfg.g2; a.g3 = cfg.g3;   // added whenever gravity is on (srcterms.hpp:27)
+    a.gm = cfg.gm; a.gx0 = cfg.gx0; a.gy0 = cfg.gy0; a.gz0 = cfg.gz0; a.rsoft = cfg.rsoft;
+    a.lambda0 = cfg.lambda0; a.cool_alpha = cfg.cool_alpha; a.tfloor = cfg.tfloor;
+    a.beta = beta;
+    a.use_ctrl_dt = use_ctrl_dt ? 1 : 0;
+    a.u = u[out];
+    a.w = f64.w[wset];
+    a.bcc = f64.bcc[wset];
+    for (int d = 0; d < 3; ++d) a.f[d] = f64.flx[d];
+    a.ctrl = const_cast<Ctrl *>(c);
+    return a;
+  }
 
   template <class T> FluxArgsT<T> flux_args_t(const FieldSet<T> &f, int d, int cur, int wset, const Ctrl *c) const {
     FluxArgs a0 = flux_args(d, cur, wset, c);

@@ -155,6 +155,7 @@ int main(int argc, char *argv[]) {
   std::cout << "# zone-cycles/second = " << std::scientific << zones * (cycle - cycle0) / wall << "\n";
   if (cfg.fofc) std::cout << "# first-order flux corrections applied: " << st.fofc_total << "\n";
   if (cfg.sts) std::cout << "# super-time-stepping stages (total): " << st.sts_stages << "\n";
+  if (cfg.flux_limiter) std::cout << "# positivity flux limiter activations: " << st.limiter_total << "\n";
 
   if (compute_error) {   // main.cpp:310-327
     std::vector<double> u(u_init.size()), b(b_init.size());

@@ -250,6 +250,72 @@ void shock_tube(Params &p, Fields &h) {
   });
 }
 
+// uniform state (gravity and cooling tests), pgen.hpp:216-229
+void uniform(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const double d = p.get_real("problem", "dens", 1.0), pr = p.get_real("problem", "pres", 1.0);
+  const double vx = p.get_real("problem", "vx", 0.0), vy = p.get_real("problem", "vy", 0.0);
+  const double vz = p.get_real("problem", "vz", 0.0);
+  const double bx = p.get_real("problem", "bx", 0.0), by = p.get_real("problem", "by", 0.0);
+  const double bz = p.get_real("problem", "bz", 0.0);
+  for_rows(m.nk + 1, m.nj + 1, m.ni + 1, [&](int k, int j) {
+      for (int i = 0; i < m.ni + 1; ++i) {
+        const size_t f = m.f(k, j, i);
+        h.b1[f] = bx;
+        h.b2[f] = by;
+        h.b3[f] = bz;
+      }
+  });
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
+      for (int i = 0; i < m.ni; ++i) {
+        h.W(0, k, j, i) = d;
+        h.W(1, k, j, i) = vx;
+        h.W(2, k, j, i) = vy;
+        h.W(3, k, j, i) = vz;
+        h.W(4, k, j, i) = pr;
+      }
+  });
+}
+
+// Gresho vortex at Mach `mach` (pressure equilibrium of a rotating ring), pgen.hpp:188-214
+void gresho(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const double mach = p.get_real("problem", "mach", 0.1);
+  const double gamma = p.get_real("mhd", "gamma", 5.0 / 3.0);
+  const double p0 = 1.0 / (gamma * mach * mach);
+  for_rows(m.nk + 1, m.nj + 1, m.ni + 1, [&](int k, int j) {
+      for (int i = 0; i < m.ni + 1; ++i) {
+        const size_t f = m.f(k, j, i);
+        h.b1[f] = 0.0;
+        h.b2[f] = 0.0;
+        h.b3[f] = 0.0;
+      }
+  });
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
+      for (int i = 0; i < m.ni; ++i) {
+        const double x = m.x1v(i), y = m.two_d ? m.x2v(j) : 0.0;
+        const double r = std::sqrt(x * x + y * y);
+        double vphi, pr;
+        if (r < 0.2) {
+          vphi = 5.0 * r;
+          pr = p0 + 12.5 * r * r;
+        } else if (r < 0.4) {
+          vphi = 2.0 - 5.0 * r;
+          pr = p0 + 12.5 * r * r - 20.0 * r + 4.0 * std::log(r / 0.2) + 4.0;
+        } else {
+          vphi = 0.0;
+          pr = p0 - 2.0 + 4.0 * std::log(2.0);
+        }
+        const double ir = r > 1e-14 ? 1.0 / r : 0.0;
+        h.W(0, k, j, i) = 1.0;
+        h.W(1, k, j, i) = -vphi * y * ir;
+        h.W(2, k, j, i) = vphi * x * ir;
+        h.W(3, k, j, i) = 0.0;
+        h.W(4, k, j, i) = pr;
+      }
+  });
+}
+
 // decay of a sinusoid in B, v or T along one axis under one diffusive term, pgen.hpp:235-266
 void diffusion_test(Params &p, Fields &h) {
   const HostGrid &m = h.g;
@@ -324,14 +390,25 @@ void Deck::configure() {
   const std::string pr = p.get_string("mhd", "precision", "fp64");
   cfg.precision = pr == "fp64" ? 0 : pr == "mixed" ? 1 : -1;
   if (cfg.precision < 0) throw std::runtime_error("unknown precision: " + pr + " (fp64 | mixed)");
-  if (p.get_bool("mhd", "flux_limiter", false)) throw std::runtime_error("mhd.flux_limiter is not supported");
+  cfg.flux_limiter = p.get_bool("mhd", "flux_limiter", false) ? 1 : 0;   // main.cpp:141
   cfg.sts = p.get_bool("diffusion", "sts", false) ? 1 : 0;   // main.cpp:142-146
   cfg.nu = p.get_real("diffusion", "nu", 0.0);
   cfg.kappa = p.get_real("diffusion", "kappa", 0.0);
   cfg.eta = p.get_real("diffusion", "eta", 0.0);
-  if (p.get_bool("gravity", "uniform", false) || p.get_bool("gravity", "point_mass", false) ||
-      p.get_bool("cooling", "enabled", false))
-    throw std::runtime_error("gravity and cooling source terms are not supported");
+  cfg.grav_uniform = p.get_bool("gravity", "uniform", false) ? 1 : 0;   // main.cpp:148-162
+  cfg.g1 = p.get_real("gravity", "g1", 0.0);
+  cfg.g2 = p.get_real("gravity", "g2", 0.0);
+  cfg.g3 = p.get_real("gravity", "g3", 0.0);
+  cfg.grav_point = p.get_bool("gravity", "point_mass", false) ? 1 : 0;
+  cfg.gm = p.get_real("gravity", "gm", 0.0);
+  cfg.gx0 = p.get_real("gravity", "x0", 0.0);
+  cfg.gy0 = p.get_real("gravity", "y0", 0.0);
+  cfg.gz0 = p.get_real("gravity", "z0", 0.0);
+  cfg.rsoft = p.get_real("gravity", "rsoft", 0.0);
+  cfg.cooling = p.get_bool("cooling", "enabled", false) ? 1 : 0;
+  cfg.lambda0 = p.get_real("cooling", "lambda0", 0.0);
+  cfg.cool_alpha = p.get_real("cooling", "alpha", 1.0);
+  cfg.tfloor = p.get_real("cooling", "tfloor", 0.0);
   g.nx1 = p.get_int("mesh", "nx1");
   g.nx2 = p.get_int("mesh", "nx2", 1);
   g.nx3 = p.get_int("mesh", "nx3", 1);
@@ -370,6 +447,8 @@ void Deck::initialize() {
   else if (problem == "blast") blast(p, h);
   else if (problem == "shock_tube") shock_tube(p, h);
   else if (problem == "diffusion_test") diffusion_test(p, h);
+  else if (problem == "uniform") uniform(p, h);
+  else if (problem == "gresho") gresho(p, h);
   else throw std::runtime_error("unknown problem: " + problem);
   // conserved variables from primitives and face fields, pgen.hpp:286-301
   const double gm1 = cfg.gamma - 1.0;

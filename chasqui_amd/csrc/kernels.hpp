@@ -254,6 +254,26 @@ hipError_t launch_sts_div(const Geom &g, const double *f1, const double *f2, con
 struct StsTerm { const double *x; double a; };
 hipError_t launch_sts_combine(const Geom &g, double *out, const StsTerm t[5], int ncomp, bool faces, hipStream_t s);
 hipError_t launch_ctrl_take_min(Ctrl *c, int slot, hipStream_t s);   // ctrl->min_dt = running minimum `slot`
+
+// positivity flux limiter and source terms (FP64 single-mesh path), srcterms.hip
+struct SrcArgs {
+  Geom g;
+  Eos eos;
+  double x1min, x2min, x3min;
+  int grav_uniform, grav_point;
+  double g1, g2, g3, gm, gx0, gy0, gz0, rsoft;
+  double lambda0, cool_alpha, tfloor;
+  double beta;                           // stage weight: beta * ctrl->dt, or beta itself
+  int use_ctrl_dt;
+  double *u;                             // conserved state the sources act on
+  const double *w, *bcc;
+  double *f[3];
+  Ctrl *ctrl;
+  int only_if_bad;                       // fofc branch: skip unless cells were flagged
+};
+hipError_t launch_limit_fluxes(const SrcArgs &a, int dir, hipStream_t s);
+hipError_t launch_gravity(const SrcArgs &a, hipStream_t s);
+hipError_t launch_cooling(const SrcArgs &a, hipStream_t s);
 template <class T> hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s);
 hipError_t launch_flag(const C2pArgs &a, hipStream_t s);   // standalone FlagBadCells
 // control-block kernels

@@ -254,6 +254,10 @@ int chq_server_submit(chq_server *s, const char *deck_text, const char *override
     chq_deck_close(j.deck);
     return fail(CHQ_EUNSUP, "served runs need diffusion off");
   }
+  if (j.cfg.flux_limiter || j.cfg.grav_uniform || j.cfg.grav_point || j.cfg.cooling) {
+    chq_deck_close(j.deck);
+    return fail(CHQ_EUNSUP, "served runs need the flux limiter and source terms off");
+  }
   j.bytes = chq_bytes_for(&j.mesh, &j.cfg);
   if (j.bytes == 0) { chq_deck_close(j.deck); return fail(CHQ_EINVAL, "invalid mesh or configuration"); }
   if (j.bytes > s->budget) { chq_deck_close(j.deck); return fail(CHQ_ENOMEM, "run larger than the HBM budget"); }

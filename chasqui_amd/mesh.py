@@ -16,7 +16,7 @@ from ._capi import ARRAY_IDS, CELL_ARRAYS, FACE_MULTI, check, lib
 def make_config(recon="ppm", rsolver="hlld", integrator="rk3", gamma=5.0 / 3.0, dfloor=1e-10,
                 pfloor=1e-12, cfl=0.3, flatten=True, low_mach=False, fofc=True, dual_energy=False,
                 de_eta=1e-3, lhlld_phi=-1.0, bc=("periodic", "periodic", "periodic"), precision="fp64",
-                characteristic=False, nu=0.0, kappa=0.0, eta=0.0, sts=False):
+                characteristic=False, nu=0.0, kappa=0.0, eta=0.0, sts=False, **extra):
     c = _capi.Config()
     c.recon = _capi.RECON[recon]
     c.rsolver = _capi.RSOLVER[rsolver]
@@ -29,6 +29,11 @@ def make_config(recon="ppm", rsolver="hlld", integrator="rk3", gamma=5.0 / 3.0, 
     c.precision = _capi.PRECISION[precision]
     c.characteristic = int(characteristic)
     c.nu, c.kappa, c.eta, c.sts = nu, kappa, eta, int(sts)
+    c.cool_alpha = 1.0
+    for k, v in extra.items():   # flux limiter, gravity and cooling fields of chq_config
+        if k not in dict(_capi.Config._fields_):
+            raise TypeError(f"unknown configuration field {k}")
+        setattr(c, k, v)
     return c
 
 
@@ -189,7 +194,7 @@ class Mesh:
         check(self.L.chq_get_status(self.h, C.byref(s)), "chq_get_status")
         return {"t": s.t, "dt": s.dt, "tlim": s.tlim, "min_dt": s.min_dt, "cycle": s.cycle,
                 "nlim": s.nlim, "active": s.active, "nbad_last": s.nbad_last,
-                "fofc_total": s.fofc_total, "sts_stages": s.sts_stages}
+                "fofc_total": s.fofc_total, "sts_stages": s.sts_stages, "limiter_total": s.limiter_total}
 
     def sync(self):
         check(self.L.chq_sync(self.h), "chq_sync")

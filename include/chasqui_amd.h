@@ -52,7 +52,12 @@ typedef struct {
  * nu, kappa, eta: viscosity, thermal conduction and Ohmic resistivity (diffusion::Coeffs,
  * diffusion.hpp:18-21; main.cpp:144-146), added in every stage, or, with sts = 1, advanced by
  * RKL2 super-time-stepping after each step (sts.hpp:111-157; main.cpp:142, 278-290). FP64
- * single-mesh path only. */
+ * single-mesh path only.
+ * flux_limiter: the Hu-Adams-Shu positivity flux limiter (positivity.hpp:24-89, main.cpp:60-64).
+ * grav_*, g1..g3, gm, gx0..gz0, rsoft: uniform and point-mass gravity (srcterms::Gravity,
+ * srcterms.hpp:11-40, main.cpp:148-157); cooling, lambda0, cool_alpha, tfloor: exact power-law
+ * cooling after each step (srcterms::Cooling, 45-71, main.cpp:158-162, 282-286). FP64 single-mesh
+ * path only. */
 typedef struct {
   int recon, rsolver, nstage;
   double gamma, dfloor, pfloor, cfl;
@@ -63,6 +68,11 @@ typedef struct {
   int characteristic;
   double nu, kappa, eta;
   int sts;
+  int flux_limiter;
+  int grav_uniform, grav_point;
+  double g1, g2, g3, gm, gx0, gy0, gz0, rsoft;
+  int cooling;
+  double lambda0, cool_alpha, tfloor;
 } chq_config;
 
 /* Array identifiers for chq_upload / chq_download / chq_array_count. */
@@ -82,6 +92,7 @@ typedef struct {
   int cycle, nlim, active, nbad_last;
   long long fofc_total;
   long long sts_stages;   /* RKL2 stages taken (super-time-stepping, main.cpp:279, 308) */
+  long long limiter_total;   /* positivity flux-limiter activations (main.cpp:307) */
 } chq_status;
 
 const char *chq_last_error(void);

@@ -56,7 +56,14 @@ def gpu_mesh_for(text, overrides=()):
                       precision=g("mhd.precision", "fp64"),
                       characteristic=_b(g("mhd.characteristic", "false")),
                       nu=float(g("diffusion.nu", 0.0)), kappa=float(g("diffusion.kappa", 0.0)),
-                      eta=float(g("diffusion.eta", 0.0)), sts=_b(g("diffusion.sts", "false")))
+                      eta=float(g("diffusion.eta", 0.0)), sts=_b(g("diffusion.sts", "false")),
+                      flux_limiter=int(_b(g("mhd.flux_limiter", "false"))),
+                      grav_uniform=int(_b(g("gravity.uniform", "false"))), grav_point=int(_b(g("gravity.point_mass", "false"))),
+                      g1=float(g("gravity.g1", 0.0)), g2=float(g("gravity.g2", 0.0)), g3=float(g("gravity.g3", 0.0)),
+                      gm=float(g("gravity.gm", 0.0)), gx0=float(g("gravity.x0", 0.0)), gy0=float(g("gravity.y0", 0.0)),
+                      gz0=float(g("gravity.z0", 0.0)), rsoft=float(g("gravity.rsoft", 0.0)),
+                      cooling=int(_b(g("cooling.enabled", "false"))), lambda0=float(g("cooling.lambda0", 0.0)),
+                      cool_alpha=float(g("cooling.alpha", 1.0)), tfloor=float(g("cooling.tfloor", 0.0)))
     return Mesh(md, cfg), kv
 
 

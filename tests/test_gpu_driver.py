@@ -120,3 +120,46 @@ def test_driver_diffusion_decay_like_the_reference(tmp_path):
             x = (np.arange(prof.size) + 0.5) / prof.size
             A = np.sum(prof * np.sin(k * x)) / np.sum(np.sin(k * x) ** 2)
             assert abs(A / exact_amplitude(D, t) - 1) < tol, (name, A)
+
+
+UNI_TEXT = """[mesh]
+nx1 = 16
+nx2 = 16
+nx3 = 16
+[time]
+tlim = 1.0
+[mhd]
+reconstruct = plm
+[problem]
+name = uniform
+dens = 2.0
+pres = 1.0
+[output]
+basename = uni
+hst_dt = 1.0
+"""
+
+
+def test_driver_gravity_and_cooling_like_the_reference(tmp_path):
+    """tests/physics.sh:38-54, 84-86 as the reference runs them: uniform gravity read back from the
+    history file (mom1 = 1, mom3 = 2 to 1e-9), exact cooling from the final VTK (T(1) to 1e-6)."""
+    (tmp_path / "uni.in").write_text(UNI_TEXT)
+    _run(["-i", "uni.in", "gravity.uniform=true", "gravity.g1=0.5", "gravity.g2=-0.25", "gravity.g3=1.0",
+          "output.basename=grav", "time.ndiag=0"], tmp_path)
+    m = (tmp_path / "grav.hst").read_text().strip().split("\n")[-1].split()
+    assert abs(float(m[3]) - 1.0) < 1e-9 and abs(float(m[5]) - 2.0) < 1e-9, m
+    _run(["-i", "uni.in", "cooling.enabled=true", "cooling.lambda0=0.1", "cooling.alpha=0.5", "output.basename=cool",
+          "time.ndiag=0", "output.vtk_dt=1.0"], tmp_path)
+    d = _read_vtk(tmp_path / "cool.0001.vtk")
+    T = (d["prs"] / d["rho"]).mean()
+    c = (5.0 / 3.0 - 1) * 2.0 * 0.1
+    assert abs(T / (np.sqrt(0.5) - 0.5 * c) ** 2 - 1) < 1e-6
+    out = _run(["-i", os.path.join(DECKS, "blast_3d.in"), "mesh.nx1=24", "mesh.nx2=24", "mesh.nx3=1",
+                "problem.pres_in=1e4", "problem.pres_out=1e-4", "mhd.fofc=false", "mhd.flux_limiter=true",
+                "time.cfl=0.4", "time.nlim=10", "time.ndiag=0", "output.basename=lim"], tmp_path)
+    n = int(re.search(r"# positivity flux limiter activations: (\d+)", out).group(1))
+    o = po.OracleSim(deck("blast_3d.in"), ["mesh.nx1=24", "mesh.nx2=24", "mesh.nx3=1", "problem.pres_in=1e4",
+                                            "problem.pres_out=1e-4", "mhd.fofc=false", "mhd.flux_limiter=true",
+                                            "time.cfl=0.4"])
+    o.step(10)
+    assert n == o.limiter_count > 0

@@ -78,8 +78,6 @@ def test_overrides_win_over_file():
     (["time.integrator=rk7"], "unknown integrator"),
     (["mesh.nghost=2"], "nghost too small"),
     (["problem.name=nothere"], "unknown problem"),
-    (["gravity.uniform=true"], "not supported"),
-    (["mhd.flux_limiter=true"], "not supported"),
 ])
 def test_bad_decks_raise(ov, msg):
     with pytest.raises(DeckError, match=msg):
@@ -106,3 +104,27 @@ def test_characteristic_key_and_its_limits():
     cfg.recon = 0   # donor cell ignores the switch (physics.hpp:71)
     assert L.chq_bytes_for(C.byref(md), C.byref(cfg)) > 0
 
+
+
+def test_limiter_and_source_keys_and_their_limits():
+    """mhd.flux_limiter, [gravity] and [cooling] (main.cpp:141, 148-162) reach the configuration;
+    the device accepts them in FP64 and refuses them in mixed precision."""
+    from chasqui_amd import _capi
+    d = Deck(deck("blast_3d.in"), ["mhd.flux_limiter=true", "gravity.uniform=true", "gravity.g3=-0.5",
+                                   "gravity.point_mass=true", "gravity.gm=2", "gravity.x0=0.1", "gravity.rsoft=0.05",
+                                   "cooling.enabled=true", "cooling.lambda0=3", "cooling.alpha=0.5",
+                                   "cooling.tfloor=0.01"])
+    c = d.config
+    assert (c.flux_limiter, c.grav_uniform, c.grav_point, c.cooling) == (1, 1, 1, 1)
+    assert (c.g1, c.g2, c.g3, c.gm, c.gx0, c.rsoft) == (0.0, 0.0, -0.5, 2.0, 0.1, 0.05)
+    assert (c.lambda0, c.cool_alpha, c.tfloor) == (3.0, 0.5, 0.01)
+    L = _capi.lib()
+    md = d.mesh_desc
+    assert L.chq_bytes_for(C.byref(md), C.byref(c)) > 0
+    for field in ("flux_limiter", "grav_uniform", "grav_point", "cooling"):
+        one = type(c).from_buffer_copy(c)
+        for f in ("flux_limiter", "grav_uniform", "grav_point", "cooling"):
+            setattr(one, f, 1 if f == field else 0)
+        one.precision = 1
+        assert L.chq_bytes_for(C.byref(md), C.byref(one)) == 0, field
+        assert "source terms" in L.chq_last_error().decode()
