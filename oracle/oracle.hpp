@@ -11,7 +11,8 @@
 // so results do not depend on the OpenMP thread count.
 //
 // Every physics option of the reference's Stage() and time loop is restated.
-// Requesting any of them raises an error instead of silently running something else.
+// Parity unpinned (no golden vector, known answer or invariant of its own): the fofc flag count
+// of a real run and the summation order of the history file.
 #pragma once
 #include <cstdint>
 #include <cstdio>

@@ -292,3 +292,127 @@ def test_charac_limits_a_single_wave_like_the_primitive_scheme():
     for side, cell in ((0, 2), (1, 3)):
         out = po.charac_side("ppm", st[None], np.array([bx]), side)[0]
         assert np.max(np.abs(out - st[:, cell])) < 1e-13
+
+
+# ---------------------------------------------------------------- LHLLD and low-Mach pins
+def test_lhlld_supersonic_upwind():
+    """riemann.hpp:104-113: the LHLLD speeds are clamped at zero, so a supersonic face takes
+    S_L = 0 and returns the left physical flux exactly."""
+    L = np.array([[1.0, 20.0, 0.1, -0.2, 0.5, 0.3, 0.2]])
+    R = np.array([[0.5, 19.0, 0.0, 0.1, 0.4, -0.2, 0.1]])
+    assert np.array_equal(po.riemann("lhlld", L, R, 0.4, theta=0.3, phi=0.7), po.phys_flux(L, 0.4))
+
+
+def test_lhlld_consistent_for_any_theta_phi():
+    """theta and phi scale jump terms only (riemann.hpp:123-124): F(U, U) = F(U) for any
+    theta, phi in [0, 1]."""
+    rng = np.random.default_rng(3)
+    n = 500
+    S = np.stack([rng.uniform(0.1, 2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n),
+                  rng.uniform(0.1, 2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)], axis=1)
+    bn = rng.uniform(-1, 1, n)
+    f = po.riemann("lhlld", S, S, bn, theta=rng.uniform(0, 1, n), phi=rng.uniform(0, 1, n))
+    assert np.allclose(f, po.phys_flux(S, bn), rtol=1e-12, atol=1e-13)
+
+
+def test_lhlld_theta_scales_the_pressure_term_of_the_contact_speed():
+    """Gas at rest with a pressure jump (B = 0): with theta = 0 the contact speed
+    ((dsr uR - dsl uL) + theta (ptL - ptR)) / (dsr - dsl) is exactly 0 and so is the mass
+    flux; theta = 1 gives the usual pressure-driven mass flux toward the low side."""
+    L = np.array([[1.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0]])
+    R = np.array([[1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0]])
+    f0 = po.riemann("lhlld", L, R, 0.0, theta=0.0, phi=1.0)[0]
+    f1 = po.riemann("lhlld", L, R, 0.0, theta=1.0, phi=1.0)[0]
+    assert f0[0] == 0.0
+    assert f1[0] > 1e-3
+
+
+def test_lhlld_phi_scales_the_velocity_term_of_the_star_pressure():
+    """A symmetric collision at equal pressure (B = 0): the contact is at rest, so the normal
+    momentum flux is the star pressure pts = P + phi rho^2 (sR - uR)(sL - uL)(uR - uL) /
+    (rho (sR - uR) - rho (sL - uL)) (riemann.hpp:124), with the LHLLD speeds
+    sL = min(0, min(uL, uR) - cmax), sR = max(0, max(uL, uR) + cmax)."""
+    rho, P, a, g = 1.0, 1.0, 0.3, 5.0 / 3.0
+    L = np.array([[rho, a, 0.0, 0.0, P, 0.0, 0.0]])
+    R = np.array([[rho, -a, 0.0, 0.0, P, 0.0, 0.0]])
+    cmax = np.sqrt(g * P / rho)
+    sl, sr = min(0.0, -a - cmax), max(0.0, a + cmax)
+    for phi in (0.0, 0.4, 1.0):
+        f = po.riemann("lhlld", L, R, 0.0, theta=1.0, phi=phi)[0]
+        pts = P + phi * rho * rho * (sr + a) * (sl - a) * (-2 * a) / (rho * (sr + a) - rho * (sl - a))
+        assert abs(f[1] - pts) <= 1e-14 * pts, (phi, f[1], pts)
+        assert abs(f[0]) <= 1e-15
+
+
+def _flux_grid(text, setup):
+    o = po.OracleSim(text, [])
+    w, bcc = o.view("w"), o.view("bcc")
+    setup(o, w, bcc)
+    o.fluxes(0, 0)
+    return o, o.get("f1")
+
+
+def _state7(w, bcc, k, j, i):
+    return np.array([[w[0, k, j, i], w[1, k, j, i], w[2, k, j, i], w[3, k, j, i], w[4, k, j, i],
+                      bcc[1, k, j, i], bcc[2, k, j, i]]])
+
+
+def test_lhlld_shock_detector_wiring():
+    """physics.hpp:182-211 on a grid: an x-face with a pressure jump whose two cells sit on a
+    compressive jump of v_y along y gets theta = (cf / (2A + cf))^4 (the transverse one-sided
+    differences of the face's two cells, normal jump du = 0); the same face one row away gets
+    theta = 1. Donor-cell states, phi forced to 1: the stored flux is the LHLLD flux of the
+    two cell states with that theta, bitwise."""
+    text = ("[mesh]\nnx1 = 12\nnx2 = 12\n[mhd]\nreconstruct = dc\nrsolver = lhlld\nlhlld_phi = 1.0\n"
+            "flatten = false\n[problem]\nname = uniform\ndens = 1.0\npres = 1.0\n")
+    A = 0.75
+
+    def setup(o, w, bcc):
+        w[0] = 1.0
+        w[1] = 0.0
+        w[3] = 0.0
+        w[2] = np.where(np.arange(o.nj)[None, :, None] >= o.js + 6, -A, A)   # v_y jumps down across row js+6
+        w[4] = np.where(np.arange(o.ni)[None, None, :] >= o.is_ + 5, 2.0, 1.0)   # p jumps across x-face is+5
+        bcc[:] = 0.0
+
+    o, f1 = _flux_grid(text, setup)
+    w, bcc = o.get("w"), o.get("bcc")
+    i = o.is_ + 5
+    for j, sheared in ((o.js + 5, True), (o.js + 6, True), (o.js + 2, False), (o.js + 9, False)):
+        L, R = _state7(w, bcc, 0, j, i - 1), _state7(w, bcc, 0, j, i)
+        cf = max(po.fast_speed(L[0], 0.0, 5.0 / 3.0), po.fast_speed(R[0], 0.0, 5.0 / 3.0))
+        ratio = min(1.0, cf / (2 * A + cf)) if sheared else 1.0
+        theta = (ratio * ratio) * (ratio * ratio)
+        f = po.riemann("lhlld", L, R, 0.0, theta=theta, phi=1.0)[0]
+        assert np.array_equal(f1[:5, 0, j, i], f[:5]), (j, f1[:5, 0, j, i], f[:5])
+        if sheared:
+            assert not np.array_equal(f[:5], po.riemann("lhlld", L, R, 0.0, theta=1.0, phi=1.0)[0][:5])
+
+
+def test_low_mach_blend_wiring():
+    """physics.hpp:170-178 on a grid: the face's normal velocities are blended with
+    z = min(1, max(|uL| / cL, |uR| / cR)) before the HLLD solve (um + ud, um - ud with
+    um = (uL + uR) / 2, ud = z (uL - uR) / 2); a supersonic pair (z = 1) and a subsonic one."""
+    for uL, uR in ((0.05, -0.02), (3.0, 2.5)):
+        text = ("[mesh]\nnx1 = 12\n[mhd]\nreconstruct = dc\nrsolver = hlld\nlow_mach = true\n"
+                "[problem]\nname = uniform\ndens = 1.0\npres = 1.0\nbx = 0.3\nby = 0.2\n")
+
+        def setup(o, w, bcc, uL=uL, uR=uR):
+            w[1] = np.where(np.arange(o.ni)[None, None, :] >= o.is_ + 6, uR, uL)
+            w[4] = np.where(np.arange(o.ni)[None, None, :] >= o.is_ + 6, 0.8, 1.0)
+
+        o, f1 = _flux_grid(text, setup)
+        w, bcc = o.get("w"), o.get("bcc")
+        i = o.is_ + 6
+        bn = o.get("b1f0")[0, 0, i]
+        L, R = _state7(w, bcc, 0, 0, i - 1), _state7(w, bcc, 0, 0, i)
+        g = 5.0 / 3.0
+        cl, cr = po.fast_speed(L[0], bn, g), po.fast_speed(R[0], bn, g)
+        z = min(1.0, max(abs(L[0, 1]) / cl, abs(R[0, 1]) / cr))
+        um, ud = 0.5 * (L[0, 1] + R[0, 1]), 0.5 * (z * (L[0, 1] - R[0, 1]))
+        Lb, Rb = L.copy(), R.copy()
+        Lb[0, 1], Rb[0, 1] = um + ud, um - ud
+        f = po.riemann("hlld", Lb, Rb, bn)[0]
+        assert np.array_equal(f1[:5, 0, 0, i], f[:5]), (uL, uR)
+        if z < 1.0:
+            assert not np.array_equal(f[:5], po.riemann("hlld", L, R, bn)[0][:5])
