@@ -12,7 +12,7 @@
 //
 // Every physics option of the reference's Stage() and time loop is restated.
 // Parity unpinned (no golden vector, known answer or invariant of its own): the fofc flag count
-// of a real run and the summation order of the history file.
+// of a real run beyond its closed-form cases, and the summation order of the history file.
 #pragma once
 #include <cstdint>
 #include <cstdio>

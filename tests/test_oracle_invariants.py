@@ -124,3 +124,16 @@ def test_dual_energy_switch_uses_the_entropy_pressure():
     # E = kin + B^2/2 + p/(gamma-1) in switched cells
     e_re = kin[sw] + 0.5 * (bc[:, sw] ** 2).sum(0) + w[4][sw] / (gamma - 1.0)
     assert np.max(np.abs(u[4][sw] - e_re) / np.abs(u[4][sw])) < 1e-13
+
+
+def test_fofc_flag_count_closed_forms():
+    """fofc.hpp:24-40, known counts: a pressure floor above every state flags every active cell
+    of the tentative state; Brio-Wu with dfloor = 0.5 flags exactly the right half (its uniform
+    cells keep rho = 0.125 through a stage, the cells next to the jump stay far from 0.5), and a
+    negligible floor flags nothing."""
+    o = po.OracleSim(deck("blast_3d.in"), ["mesh.nx1=8", "mesh.nx2=6", "mesh.nx3=4", "mhd.pfloor=1e6"])
+    assert o.stage(0, 1, 1.0, 0.0, 0.1 * o.min_dt) == 8 * 6 * 4
+    o = po.OracleSim(deck("brio_wu.in"), ["mesh.nx1=64", "mhd.dfloor=0.5"])
+    assert o.stage(0, 1, 1.0, 0.0, 0.5 * o.min_dt) == 32
+    o = po.OracleSim(deck("brio_wu.in"), ["mesh.nx1=64"])
+    assert o.stage(0, 1, 1.0, 0.0, 0.5 * o.min_dt) == 0
