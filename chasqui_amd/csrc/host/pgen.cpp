@@ -5,6 +5,7 @@
 // out of the x-y plane), orszag_tang problem.pres / problem.b0, blast problem.perturb /
 // problem.seed (counter-based density noise). Absent keys reproduce the reference bitwise.
 #include <cmath>
+#include <random>
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
@@ -220,7 +221,32 @@ void blast(Params &p, Fields &h) {
   });
 }
 
-// Brio-Wu shock tube, pgen.hpp:158-184 (density noise is not provided)
+// field loop advection, pgen.hpp:138-155: A_z = max(amp (radius - r), 0), uniform flow
+void field_loop(Params &p, Fields &h) {
+  const HostGrid &m = h.g;
+  const double a0 = p.get_real("problem", "amp", 1.0e-3);
+  const double rad = p.get_real("problem", "radius", 0.3);
+  const double vx = p.get_real("problem", "vx", 1.0), vy = p.get_real("problem", "vy", 0.5);
+  const double vz = p.get_real("problem", "vz", 0.0);
+  curl_potential(h, [=](int c, double x, double y, double) {
+    if (c != 2) return 0.0;
+    const double r = std::sqrt(x * x + y * y);
+    return std::fmax(a0 * (rad - r), 0.0);
+  });
+  for_rows(m.nk, m.nj, m.ni, [&](int k, int j) {
+      for (int i = 0; i < m.ni; ++i) {
+        h.W(0, k, j, i) = 1.0;
+        h.W(1, k, j, i) = vx;
+        h.W(2, k, j, i) = vy;
+        h.W(3, k, j, i) = vz;
+        h.W(4, k, j, i) = 1.0;
+      }
+  });
+}
+
+// Brio-Wu shock tube, pgen.hpp:158-184. problem.perturb: density times (1 + perturb U), U from
+// std::mt19937(42) through std::uniform_real_distribution<double>(-1, 1), one draw per cell in
+// (k, j, i) order over all cells, ghosts included (so that pass runs on one thread).
 void shock_tube(Params &p, Fields &h) {
   const HostGrid &m = h.g;
   const double dl = p.get_real("problem", "dl", 1.0), pl = p.get_real("problem", "pl", 1.0);
@@ -229,7 +255,7 @@ void shock_tube(Params &p, Fields &h) {
   const double bx = p.get_real("problem", "bx", 0.75);
   const double x0 = p.get_real("problem", "x0", 0.0);
   const double ul = p.get_real("problem", "ul", 0.0), ur = p.get_real("problem", "ur", 0.0);
-  if (p.get_real("problem", "perturb", 0.0) != 0.0) throw std::runtime_error("shock_tube: problem.perturb is not supported");
+  const double perturb = p.get_real("problem", "perturb", 0.0);
   for_rows(m.nk + 1, m.nj + 1, m.ni + 1, [&](int k, int j) {
       for (int i = 0; i < m.ni + 1; ++i) {
         const size_t f = m.f(k, j, i);
@@ -248,6 +274,16 @@ void shock_tube(Params &p, Fields &h) {
         h.W(4, k, j, i) = left ? pl : pr;
       }
   });
+  if (perturb != 0.0) {   // (with perturb = 0 every factor is 1 + 0 U = 1, as above)
+    std::mt19937 rng(42);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (int k = 0; k < m.nk; ++k)
+      for (int j = 0; j < m.nj; ++j)
+        for (int i = 0; i < m.ni; ++i) {
+          const bool left = m.x1v(i) < x0;
+          h.W(0, k, j, i) = (left ? dl : dr) * (1.0 + perturb * U(rng));
+        }
+  }
 }
 
 // uniform state (gravity and cooling tests), pgen.hpp:216-229
@@ -445,6 +481,7 @@ void Deck::initialize() {
   if (problem == "cpaw") cpaw(p, h);
   else if (problem == "orszag_tang") orszag_tang(p, h);
   else if (problem == "blast") blast(p, h);
+  else if (problem == "field_loop") field_loop(p, h);
   else if (problem == "shock_tube") shock_tube(p, h);
   else if (problem == "diffusion_test") diffusion_test(p, h);
   else if (problem == "uniform") uniform(p, h);

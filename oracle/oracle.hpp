@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <fstream>
@@ -1258,8 +1259,28 @@ inline void pg_blast(Deck &p, const Grid &m, Init &h) {
       }
 }
 
-// Brio-Wu shock tube, pgen.hpp:158-184 (problem.perturb must stay 0: the reference draws
-// it from std::mt19937 in loop order, which is not restated here).
+// field loop advection, pgen.hpp:138-155: A_z = max(amp (radius - r), 0), uniform flow
+inline void pg_field_loop(Deck &p, const Grid &m, Init &h) {
+  const Real a0 = p.real("problem", "amp", 1.0e-3);
+  const Real rad = p.real("problem", "radius", 0.3);
+  const Real vx = p.real("problem", "vx", 1.0), vy = p.real("problem", "vy", 0.5), vz = p.real("problem", "vz", 0.0);
+  from_potential(m, h, [=](int c, Real x, Real y, Real) -> Real {
+    if (c != 2) return 0.0;
+    const Real r = std::sqrt(x * x + y * y);
+    return std::fmax(a0 * (rad - r), 0.0);
+  });
+  for (int k = 0; k < m.nk; ++k)
+    for (int j = 0; j < m.nj; ++j)
+      for (int i = 0; i < m.ni; ++i) {
+        h.w(IDN, k, j, i) = 1.0; h.w(IVX, k, j, i) = vx; h.w(IVY, k, j, i) = vy; h.w(IVZ, k, j, i) = vz;
+        h.w(IPR, k, j, i) = 1.0;
+      }
+}
+
+// Brio-Wu shock tube, pgen.hpp:158-184. problem.perturb: density times (1 + perturb U) with U
+// drawn from std::mt19937(42) through std::uniform_real_distribution<double>(-1, 1), one draw
+// per cell in (k, j, i) order over all cells, ghosts included (the draw is made for every
+// cell, also when perturb is 0).
 inline void pg_shock_tube(Deck &p, const Grid &m, Init &h) {
   const Real dl = p.real("problem", "dl", 1.0), pl = p.real("problem", "pl", 1.0);
   const Real dr = p.real("problem", "dr", 0.125), pr = p.real("problem", "pr", 0.1);
@@ -1267,7 +1288,9 @@ inline void pg_shock_tube(Deck &p, const Grid &m, Init &h) {
   const Real bx = p.real("problem", "bx", 0.75);
   const Real x0 = p.real("problem", "x0", 0.0);
   const Real ul = p.real("problem", "ul", 0.0), ur = p.real("problem", "ur", 0.0);
-  if (p.real("problem", "perturb", 0.0) != 0.0) throw std::runtime_error("shock_tube: perturb is not supported");
+  const Real perturb = p.real("problem", "perturb", 0.0);
+  std::mt19937 rng(42);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
   for (int k = 0; k < m.nk + 1; ++k)
     for (int j = 0; j < m.nj + 1; ++j)
       for (int i = 0; i < m.ni + 1; ++i) {
@@ -1279,7 +1302,7 @@ inline void pg_shock_tube(Deck &p, const Grid &m, Init &h) {
     for (int j = 0; j < m.nj; ++j)
       for (int i = 0; i < m.ni; ++i) {
         const bool left = m.x1v(i) < x0;
-        h.w(IDN, k, j, i) = (left ? dl : dr) * (1.0 + 0.0);
+        h.w(IDN, k, j, i) = (left ? dl : dr) * (1.0 + perturb * U(rng));
         h.w(IVX, k, j, i) = left ? ul : ur;
         h.w(IVY, k, j, i) = 0.0; h.w(IVZ, k, j, i) = 0.0;
         h.w(IPR, k, j, i) = left ? pl : pr;
@@ -1369,6 +1392,7 @@ inline void initialize(const std::string &name, Deck &p, Grid &m, const Cfg &cfg
   if (name == "cpaw") pg_cpaw(p, m, h);
   else if (name == "orszag_tang") pg_orszag_tang(p, m, h);
   else if (name == "blast") pg_blast(p, m, h);
+  else if (name == "field_loop") pg_field_loop(p, m, h);
   else if (name == "shock_tube") pg_shock_tube(p, m, h);
   else if (name == "diffusion_test") pg_diffusion_test(p, m, h);
   else if (name == "uniform") pg_uniform(p, m, h);

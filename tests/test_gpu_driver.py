@@ -22,6 +22,8 @@ EXE = os.path.join(ROOT, "chasqui_amd", "_lib", "chasqui_amd")
     ("orszag_tang.in", ["mesh.nx1=32", "mesh.nx2=32", "problem.pres=0.14"], 8),
     ("cpaw_3d_oblique.in", ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12"], 3),
     ("brio_wu.in", [], 40),
+    ("brio_wu.in", ["problem.perturb=0.05"], 30),
+    ("field_loop.in", [], 12),
 ])
 def test_deck_pipeline_bitwise(name, ov, steps):
     from chasqui_amd.deck import Deck

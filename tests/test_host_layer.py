@@ -24,6 +24,11 @@ CASES = [
     ("blast_3d.in", ["mesh.nx1=64", "mesh.nx2=64", "mesh.nx3=64"]),
     ("orszag_tang.in", ["mesh.nx1=384", "mesh.nx2=384"]),
     ("cpaw_3d_oblique.in", ["mesh.nx1=48", "mesh.nx2=48", "mesh.nx3=48"]),
+    ("field_loop.in", []),
+    ("field_loop.in", ["mesh.nx1=12", "mesh.nx2=10", "mesh.nx3=8", "mesh.x3min=-0.5", "mesh.x3max=0.5",
+                       "problem.vz=0.25"]),
+    ("brio_wu.in", ["mesh.nx1=100", "problem.perturb=0.05"]),
+    ("brio_wu.in", ["mesh.nx1=40", "mesh.nx2=6", "problem.perturb=0.2"]),
 ]
 
 

@@ -211,3 +211,72 @@ def test_gresho_vortex_setup():
     dpdx, dpdy = np.gradient(w[4][0], dx, dx, edge_order=2)[1], np.gradient(w[4][0], dx, dx, edge_order=2)[0]
     dpdr = (dpdx * X + dpdy * Y) / R
     assert np.max(np.abs(dpdr[ring] - (vphi ** 2 / R)[ring])) < 0.02 * np.max(vphi ** 2 / R)
+
+
+# ---------------------------------------------------------------- shock-tube noise, field loop
+def _mt19937_raw(seed, n):
+    """Raw 32-bit outputs of std::mt19937(seed) (numpy's legacy seeding of MT19937 is the
+    engine's own init_genrand)."""
+    return np.random.RandomState(seed)._bit_generator.random_raw(n)
+
+
+def _std_uniform_pm1(raw):
+    """libstdc++ std::uniform_real_distribution<double>(-1, 1) over mt19937: generate_canonical
+    draws two words, sum = x1 + x2 2^32 in double, divides by 2^64 (clamped below 1), then
+    a + (b - a) u."""
+    out = []
+    for t in range(0, len(raw) - 1, 2):
+        s = float(raw[t]) + float(raw[t + 1]) * 4294967296.0
+        u = s / 18446744073709551616.0
+        if u >= 1.0:
+            u = float(np.nextafter(1.0, 0.0))
+        out.append(u * 2.0 + -1.0)
+    return np.array(out)
+
+
+def test_mt19937_reference_value():
+    """The C++ standard's check value ([rand.predef]): the 10000th output of a default-seeded
+    std::mt19937 (seed 5489) is 4123659995; pins the engine the noise restatement below uses."""
+    assert int(_mt19937_raw(5489, 10000)[-1]) == 4123659995
+
+
+def test_shock_tube_noise_is_the_reference_stream():
+    """pgen.hpp:165-179: rho = (dl | dr) (1 + perturb U) with U from std::mt19937(42) through
+    std::uniform_real_distribution<double>(-1, 1), one draw per cell in (k, j, i) order over
+    all cells, ghosts included: the oracle's densities equal that stream bitwise (1D and 2D)."""
+    for ov, pert in ((["mesh.nx1=100"], 0.05), (["mesh.nx1=40", "mesh.nx2=6"], 0.2)):
+        o = po.OracleSim(deck("brio_wu.in"), ov + [f"problem.perturb={pert}"], pgen_only=True)
+        rho = o.get("w")[0]
+        n = rho.size
+        U = _std_uniform_pm1(_mt19937_raw(42, 2 * n)).reshape(rho.shape)
+        g, r = o.geometry()
+        x = -0.5 + (np.arange(o.ni) - o.is_ + 0.5) * r[0]
+        base = np.where(x < 0.0, 1.0, 0.125)[None, None, :]
+        assert np.array_equal(rho, base * (1.0 + pert * U))
+
+
+def test_field_loop_setup():
+    """pgen.hpp:138-155: B = curl(A_z) with A_z = amp max(R - r, 0) on the edges: the face
+    divergence vanishes to round-off, the field is exactly zero clear of the loop and has
+    magnitude amp inside it (to the truncation of the edge differences)."""
+    o = po.OracleSim(deck("field_loop.in"), [], pgen_only=True)
+    g, r = o.geometry()
+    dx, dy = r[0], r[1]
+    b1, b2 = o.get("b1f0")[0], o.get("b2f0")[0]
+    js, je, is_, ie = o.js, o.je, o.is_, o.ie
+    div = (b1[js:je + 1, is_ + 1:ie + 2] - b1[js:je + 1, is_:ie + 1]) / dx + \
+          (b2[js + 1:je + 2, is_:ie + 1] - b2[js:je + 1, is_:ie + 1]) / dy
+    assert np.max(np.abs(div)) * dx <= 1e-15 * 1e-3
+    bx = 0.5 * (b1[:o.nj, :o.ni] + b1[:o.nj, 1:o.ni + 1])
+    by = 0.5 * (b2[:o.nj, :o.ni] + b2[1:o.nj + 1, :o.ni])
+    x = -1.0 + (np.arange(o.ni) - is_ + 0.5) * dx
+    y = -0.5 + (np.arange(o.nj) - js + 0.5) * dy
+    X, Y = np.meshgrid(x, y)
+    R = np.hypot(X, Y)
+    mag = np.hypot(bx, by)
+    outside = R > 0.3 + 2 * dx
+    assert np.all(mag[outside] == 0.0)
+    ring = (R > 0.1) & (R < 0.3 - 2 * dx)
+    assert np.max(np.abs(mag[ring] / 1e-3 - 1.0)) < 0.05
+    w = o.get("w")
+    assert np.all(w[1] == 1.0) and np.all(w[2] == 0.5) and np.all(w[0] == 1.0)
