@@ -83,3 +83,24 @@ def test_blast256_twenty_steps_invariants(prec):
     k, j, i = ACT
     assert np.min(w[0][k, j, i]) > 0 and np.min(w[4][k, j, i]) > 0
     m.close()
+
+
+def test_blast_mirror_symmetry_on_the_device():
+    """README.md:150-152 on the device time loop: the noise-free blast (64^3, 20 steps, the
+    fused stage with PPM + HLLD + flattening + fofc) keeps its mirror symmetries bitwise: the
+    z-mirror (B lies in the x-y plane) and the combined (x, y) -> (-x, -y) flip."""
+    from chasqui_amd.deck import Deck
+    print("case blast 64^3 mirror symmetry, 20 steps", flush=True)
+    d = Deck(bench.DECK, ["mesh.nx1=64", "mesh.nx2=64", "mesh.nx3=64", "problem.perturb=0"])
+    m = d.make_mesh()
+    d.close()
+    m.init(0.0, 1.0e9, -1)
+    m.step(20)
+    u = m.download("u0")
+    a = (slice(None), slice(NG, NG + 64), slice(NG, NG + 64), slice(NG, NG + 64))
+    u = u[a]
+    assert np.array_equal(u[0], u[0][::-1, :, :]) and np.array_equal(u[4], u[4][::-1, :, :])
+    assert np.array_equal(u[3], -u[3][::-1, :, :]) and np.array_equal(u[1], u[1][::-1, :, :])
+    assert np.array_equal(u[0], u[0][:, ::-1, ::-1]) and np.array_equal(u[4], u[4][:, ::-1, ::-1])
+    assert np.array_equal(u[1], -u[1][:, ::-1, ::-1]) and np.array_equal(u[2], -u[2][:, ::-1, ::-1])
+    m.close()
