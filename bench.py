@@ -100,9 +100,10 @@ def cpu_baseline(n, steps):
                       f"(oracle/, -O3 -ffp-contract=off, OMP_PROC_BIND=spread)"}
 
 
-def run(n, precision, steps, warmup, breakdown=False):
+def run(n, precision, steps, warmup, breakdown=False, state=None):
     """K timed steps of the device-resident pipeline on the n^3 blast (after W warm-up steps);
-    returns (ms, flux kernel ms, flux launches, status, per-kernel breakdown)."""
+    returns (ms, flux kernel ms, flux launches, status, per-kernel breakdown). With `state` (a
+    dict), the final conserved state of the active cells is left in state["u0"]."""
     from chasqui_amd.deck import Deck
     deck = Deck(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}", f"mhd.precision={precision}"])
     mesh = deck.make_mesh()
@@ -122,6 +123,13 @@ def run(n, precision, steps, warmup, breakdown=False):
             kms, kn = mesh.kernel_timing_read()
             kernels[cls] = {"ms_per_step": kms / 2, "launches_per_step": kn // 2}
         mesh.kernel_timing(None)
+    if state is not None:
+        u = mesh.download("u0")
+        ng = (u.shape[3] - n) // 2
+        act = slice(ng, ng + n)
+        state["u0"] = u[:5, act, act, act].copy()
+        state["cycle"] = mesh.status()["cycle"]
+        del u
     mesh.close()
     return ms, flux_ms, flux_launches, st, kernels
 
@@ -332,7 +340,8 @@ def main():
         print(json.dumps(serve(args.serve_jobs, rate_factor=args.serve_rate, chunk=args.serve_chunk)))
         return
     n = args.n
-    ms, flux_ms, flux_launches, st, breakdown = run(n, "fp64", args.steps, args.warmup, breakdown=True)
+    fstate = {} if not args.no_mixed else None
+    ms, flux_ms, flux_launches, st, breakdown = run(n, "fp64", args.steps, args.warmup, breakdown=True, state=fstate)
     zone_updates = float(n) ** 3 * args.steps
     value = zone_updates / (ms * 1e-3)
     avg_flux_s = flux_ms * 1e-3 / max(flux_launches, 1)
@@ -368,12 +377,21 @@ def main():
     }
     if not args.no_mixed:
         # the same workload with FP32 reconstruction/Riemann/EMF and FP64 con2prim/RK/CT
-        mms, mflux, mlaunch, mst, mk = run(n, "mixed", args.steps, args.warmup, breakdown=True)
+        mstate = {}
+        mms, mflux, mlaunch, mst, mk = run(n, "mixed", args.steps, args.warmup, breakdown=True, state=mstate)
         mval = zone_updates / (mms * 1e-3)
+        # relative L1 distance of the final mixed state from the FP64 one (same step count)
+        import numpy as np
+        a, b = mstate["u0"], fstate["u0"]
+        l1 = {f: float(np.sum(np.abs(a[q] - b[q])) / np.sum(np.abs(b[q])))
+              for f, q in (("rho", 0), ("E", 4))}
+        l1["momentum"] = float(np.sum(np.abs(a[1:4] - b[1:4])) / np.sum(np.abs(b[1:4])))
+        del mstate, fstate
         out["mixed"] = {"value": mval, "unit": "zone-updates/s", "dtype": "f32 flux/EMF, f64 state",
                         "ms_per_step": mms / args.steps, "speedup_vs_fp64": mval / value,
                         "flux_avg_launch_ms": mflux / max(mlaunch, 1),
-                        "fofc_corrections_in_run": mst["fofc_total"], "kernels": mk}
+                        "fofc_corrections_in_run": mst["fofc_total"], "kernels": mk,
+                        "l1_rel_vs_fp64": l1, "l1_cycles": mst["cycle"]}
     if args.ensemble_runs > 0:
         out["batch"] = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
     if args.serve_jobs > 0:
