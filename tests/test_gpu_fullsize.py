@@ -104,3 +104,27 @@ def test_blast_mirror_symmetry_on_the_device():
     assert np.array_equal(u[0], u[0][:, ::-1, ::-1]) and np.array_equal(u[4], u[4][:, ::-1, ::-1])
     assert np.array_equal(u[1], -u[1][:, ::-1, ::-1]) and np.array_equal(u[2], -u[2][:, ::-1, ::-1])
     m.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_blast256_runs_are_deterministic(prec):
+    """Two independent contexts advanced the same 20 steps at 256^3 end bitwise equal: the
+    atomically built fofc list (its order varies from run to run), the atomic CFL minimum and
+    the grid-wide launches leave no trace of scheduling in the results."""
+    from chasqui_amd.deck import Deck
+    print(f"case blast 256^3 {prec} determinism, 2 x 20 steps", flush=True)
+    out = []
+    for _ in range(2):
+        d = Deck(bench.DECK, [f"mhd.precision={prec}", "mhd.pfloor=0.05"])   # fofc fires every stage
+        m = d.make_mesh()
+        d.close()
+        m.init(0.0, 1.0e9, -1)
+        m.step(20)
+        st = m.status()
+        out.append((m.download("u0"), [m.download(f"b{q}f0") for q in (1, 2, 3)], st["t"], st["fofc_total"]))
+        m.close()
+    (u_a, b_a, t_a, n_a), (u_b, b_b, t_b, n_b) = out
+    assert n_a == n_b and n_a > 0 and t_a == t_b
+    assert np.array_equal(u_a, u_b)
+    for x, y in zip(b_a, b_b):
+        assert np.array_equal(x, y)
