@@ -134,3 +134,16 @@ def test_diffusion_needs_fp64_single_mesh():
         b.add(m)
     b.close()
     m.close()
+
+
+def test_stiff_diffusion_step_counts_on_the_device():
+    """README.md:126-127 on the device loop: eta = 0.5 on the 64-cell deck to t = 0.05 takes 683
+    explicit steps, or 14 RKL2 super-time-steps with 111 stages in total."""
+    ov = ["mesh.nx1=64", "problem.var=b", "problem.axis=1", "diffusion.eta=0.5"]
+    for sts, steps, stages in ((False, 683, 0), (True, 14, 111)):
+        o, gm, _ = _pair(DIFF_DECK, ov + (["diffusion.sts=true"] if sts else []))
+        gm.init(0.0, 0.05, -1)
+        gm.step(1000)
+        st = gm.status()
+        assert st["cycle"] == steps and st["t"] == 0.05 and st["sts_stages"] == stages, st
+        gm.close()

@@ -190,3 +190,32 @@ def test_cpaw_1d_full_run_known_answer():
     total_without_entropy = err.sum() - err[5]
     assert f"{total_without_entropy:.2e}" == "6.65e-04"
     gm.close()
+
+
+GRESHO_DECK = ("[mesh]\nnx1 = 64\nnx2 = 64\nx1min = -0.5\nx1max = 0.5\nx2min = -0.5\nx2max = 0.5\n"
+               "[time]\nintegrator = rk3\ncfl = 0.4\ntlim = 1.0\n[mhd]\nreconstruct = ppmx\nrsolver = hlld\n"
+               "[problem]\nname = gresho\n")
+
+
+@pytest.mark.parametrize("mach,ov,percent", [
+    (0.1, [], "99.75"), (0.1, ["mhd.low_mach=true"], "99.89"), (0.1, ["mhd.rsolver=lhlld"], "99.88"),
+    (0.01, [], "99.08"), (0.01, ["mhd.low_mach=true"], "99.89"), (0.01, ["mhd.rsolver=lhlld"], "99.87"),
+], ids=["M0.1-hlld", "M0.1-low_mach", "M0.1-lhlld", "M0.01-hlld", "M0.01-low_mach", "M0.01-lhlld"])
+def test_gresho_kinetic_energy_retained_on_the_device(mach, ov, percent):
+    """README.md:103-109 on the device loop: kinetic energy of the Gresho vortex (64^2, PPMX,
+    t = 1) retained with HLLD, HLLD + low-Mach blend and LHLLD at Mach 0.1 and 0.01, to the
+    quoted digits (the Mach 0.01 runs take ~17600 steps)."""
+    from chasqui_amd.deck import Deck
+    print(f"case gresho mach={mach} {ov}", flush=True)
+    d = Deck(GRESHO_DECK, [f"problem.mach={mach}"] + ov)
+    m = d.make_mesh()
+    d.close()
+    def kinetic():
+        u = m.download("u0")[:, 0, 3:67, 3:67]
+        return float(np.sum(0.5 * (u[1] ** 2 + u[2] ** 2) / u[0]))
+    k0 = kinetic()
+    m.init(0.0, 1.0, -1)
+    m.step(40000)
+    assert m.status()["t"] == 1.0
+    assert f"{100 * kinetic() / k0:.2f}" == percent
+    m.close()

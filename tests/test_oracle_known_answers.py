@@ -280,3 +280,41 @@ def test_field_loop_setup():
     assert np.max(np.abs(mag[ring] / 1e-3 - 1.0)) < 0.05
     w = o.get("w")
     assert np.all(w[1] == 1.0) and np.all(w[2] == 0.5) and np.all(w[0] == 1.0)
+
+
+# ---------------------------------------------------------------- README robustness numbers
+def test_stiff_diffusion_step_counts():
+    """README.md:126-127: stiff resistivity (eta = 0.5 on the 64-cell diffusion deck, so
+    dt_par = dx^2 / (2 eta) = 2.4e-4) to t = 0.05: 683 explicit steps, or 14 RKL2
+    super-time-steps with 111 stages in total."""
+    from diffusion_cases import DIFF_DECK
+    ov = ["mesh.nx1=64", "problem.var=b", "problem.axis=1", "diffusion.eta=0.5", "time.tlim=0.05"]
+    o = po.OracleSim(DIFF_DECK, ov)
+    assert abs(o.diffusion_dt() / (1.0 / 64) ** 2 - 1.0) < 1e-15
+    o.run()
+    assert o.cycle == 683 and o.time == 0.05
+    o = po.OracleSim(DIFF_DECK, ov + ["diffusion.sts=true"])
+    o.run()
+    assert o.cycle == 14 and o.sts_count == 111 and o.time == 0.05
+
+
+GRESHO_DECK = ("[mesh]\nnx1 = 64\nnx2 = 64\nx1min = -0.5\nx1max = 0.5\nx2min = -0.5\nx2max = 0.5\n"
+               "[time]\nintegrator = rk3\ncfl = 0.4\ntlim = 1.0\n[mhd]\nreconstruct = ppmx\nrsolver = hlld\n"
+               "[problem]\nname = gresho\n")
+
+
+def _kinetic(o):
+    u = o.get("u0")[:, 0, o.js:o.je + 1, o.is_:o.ie + 1]
+    return float(np.sum(0.5 * (u[1] ** 2 + u[2] ** 2) / u[0]))
+
+
+@pytest.mark.parametrize("ov,percent", [([], "99.75"), (["mhd.low_mach=true"], "99.89"),
+                                        (["mhd.rsolver=lhlld"], "99.88")], ids=["hlld", "low_mach", "lhlld"])
+def test_gresho_kinetic_energy_retained(ov, percent):
+    """README.md:103-109: the Mach 0.1 Gresho vortex (64^2, PPMX, t = 1) keeps 99.75% of its
+    kinetic energy with HLLD, 99.89% with the low-Mach blend (physics.hpp:170-178) and 99.88%
+    with LHLLD (its phi factor, physics.hpp:182-211): each reproduced to the quoted digits."""
+    o = po.OracleSim(GRESHO_DECK, ["problem.mach=0.1"] + ov)
+    k0 = _kinetic(o)
+    o.run()
+    assert f"{100 * _kinetic(o) / k0:.2f}" == percent
