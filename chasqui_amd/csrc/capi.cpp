@@ -663,6 +663,8 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
     HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
     c->tmark(CHQ_K_FLUX, false);
   }
+  // gravity (FP64 contexts only) reads the stage-input primitives, set wc
+  constexpr bool fp64 = std::is_same<T, double>::value;
   c->tmark(CHQ_K_CORNER, true);
   HIPCHK(launch_corner<T>(c->corner_args_t(fs, wc, k, 0), c->stream));
   c->tmark(CHQ_K_CORNER, false);
@@ -672,6 +674,7 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   c->tmark(CHQ_K_UPDATE, true);
   HIPCHK(launch_update<T>(ua, c->stream));
   c->tmark(CHQ_K_UPDATE, false);
+  if (fp64 && c->gravity()) HIPCHK(launch_gravity(c->src_args(cf.out, wc, k, cf.beta, true), c->stream));   // main.cpp:76
   c->tmark(CHQ_K_BC, true);
   HIPCHK(c->boundaries(cf.out, nvar, k, 0));
   c->tmark(CHQ_K_BC, false);
@@ -697,6 +700,11 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
     ub.use_ctrl_dt = 1;
     ub.beta = cf.beta;
     HIPCHK(launch_update<T>(ub, c->stream));
+    if (fp64 && c->gravity()) {   // main.cpp:89
+      SrcArgs sb = c->src_args(cf.out, wc, k, cf.beta, true);
+      sb.only_if_bad = 1;
+      HIPCHK(launch_gravity(sb, c->stream));
+    }
     HIPCHK(c->boundaries(cf.out, nvar, k, 1));
     C2pArgsT<T> cb = ca;
     cb.flag_mode = 0;
@@ -721,10 +729,13 @@ static int enqueue_step(chq_ctx *c) {
   const RSolver rs = RSolver(c->cfg.rsolver);
   Ctrl *k = c->ctrl;
   const int nst = c->cfg.nstage;
-  // dual energy, diffusion, super-time-stepping, the flux limiter and the source terms run in
-  // the reference kernel order
+  // dual energy, diffusion, super-time-stepping, cooling and the flux limiter run in the
+  // reference kernel order; gravity also fits the fused stage. (The limiter leaves the faces
+  // that a periodic boundary fill pairs up unequal — it limits active faces only, not the
+  // ghost-column faces the corner EMF also reads — and the reference flags cells before that
+  // fill, which the fused stage, flagging inside con2prim after it, would not reproduce.)
   const bool split = c->cfg.sts || c->cfg.cooling;   // main.cpp:273-290: dt taken after the splits
-  const bool fused = !c->eos.dual_energy && !c->diffusive() && !split && !c->cfg.flux_limiter && !c->gravity();
+  const bool fused = !c->eos.dual_energy && !c->diffusive() && !split && !c->cfg.flux_limiter;
   const int nvar = c->entropy ? 6 : 5;
   HIPCHK(launch_ctrl_begin_step(k, c->stream));
   for (int s = 1; s <= nst; ++s) {
