@@ -681,7 +681,7 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   C2pArgsT<T> ca = c->c2p_args_t(fs, cf.out, wn, k);
   ca.write_went = c->entropy;
   ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
-  ca.compute_dt = last;
+  ca.compute_dt = last && !(c->cfg.sts || c->cfg.cooling);   // split steps take dt after the splits
   ca.slot = 0;
   ca.flag_mode = c->cfg.fofc ? 2 : 0;
   c->tmark(CHQ_K_C2P, true);
@@ -729,13 +729,14 @@ static int enqueue_step(chq_ctx *c) {
   const RSolver rs = RSolver(c->cfg.rsolver);
   Ctrl *k = c->ctrl;
   const int nst = c->cfg.nstage;
-  // dual energy, diffusion, super-time-stepping, cooling and the flux limiter run in the
-  // reference kernel order; gravity also fits the fused stage. (The limiter leaves the faces
-  // that a periodic boundary fill pairs up unequal — it limits active faces only, not the
-  // ghost-column faces the corner EMF also reads — and the reference flags cells before that
-  // fill, which the fused stage, flagging inside con2prim after it, would not reproduce.)
+  // dual energy, explicit diffusion and the flux limiter run in the reference kernel order;
+  // gravity and the operator-split steps (super-time-stepping, cooling: the stages carry no
+  // diffusion then) also fit the fused stage. (The limiter leaves the faces that a periodic
+  // boundary fill pairs up unequal — it limits active faces only, not the ghost-column faces
+  // the corner EMF also reads — and the reference flags cells before that fill, which the
+  // fused stage, flagging inside con2prim after it, would not reproduce.)
   const bool split = c->cfg.sts || c->cfg.cooling;   // main.cpp:273-290: dt taken after the splits
-  const bool fused = !c->eos.dual_energy && !c->diffusive() && !split && !c->cfg.flux_limiter;
+  const bool fused = !c->eos.dual_energy && !(c->diffusive() && !c->cfg.sts) && !c->cfg.flux_limiter;
   const int nvar = c->entropy ? 6 : 5;
   HIPCHK(launch_ctrl_begin_step(k, c->stream));
   for (int s = 1; s <= nst; ++s) {
