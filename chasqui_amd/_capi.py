@@ -91,6 +91,9 @@ def lib():
         "chq_diffusion_fluxes": (C.c_int, [vp, C.c_int, C.c_int]),
         "chq_resistive_emf": (C.c_int, [vp, C.c_int]),
         "chq_diffusion_dt": (C.c_int, [vp, C.POINTER(C.c_double)]),
+        "chq_limit_fluxes": (C.c_int, [vp, C.c_int, C.c_double, C.POINTER(C.c_long)]),
+        "chq_apply_gravity": (C.c_int, [vp, C.c_int, C.c_double]),
+        "chq_apply_cooling": (C.c_int, [vp, C.c_int, C.c_double]),
         "chq_stage": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_long)]),
         "chq_init": (C.c_int, [vp, C.c_double, C.c_double, C.c_int]),
@@ -157,7 +160,8 @@ def exported_symbols():
         "chq_last_error", "chq_version", "chq_create", "chq_destroy", "chq_geometry",
         "chq_device_bytes", "chq_array_count", "chq_upload", "chq_download", "chq_fluxes",
         "chq_corner_emf", "chq_update", "chq_boundaries", "chq_cons_to_prim", "chq_fofc_flag",
-        "chq_fofc_repair", "chq_diffusion_fluxes", "chq_resistive_emf", "chq_diffusion_dt", "chq_stage", "chq_init", "chq_step", "chq_step_timed",
+        "chq_fofc_repair", "chq_diffusion_fluxes", "chq_resistive_emf", "chq_diffusion_dt", "chq_limit_fluxes",
+        "chq_apply_gravity", "chq_apply_cooling", "chq_stage", "chq_init", "chq_step", "chq_step_timed",
         "chq_get_status", "chq_sync", "chq_stream", "chq_kernel_timing", "chq_kernel_timing_read",
         "chq_deck_open", "chq_deck_open_lazy", "chq_deck_prepare", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
         "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",

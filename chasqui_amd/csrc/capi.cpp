@@ -567,6 +567,41 @@ int chq_diffusion_dt(chq_ctx *c, double *dt) {
   return CHQ_OK;
 }
 
+int chq_limit_fluxes(chq_ctx *c, int dir, double beta_dt, long *limited) {
+  if (!c || dir < 0 || dir > 2) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_limit_fluxes: FP64 contexts only");
+  if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
+  long long before = 0, after = 0;
+  if (limited) {
+    HIPCHK(hipMemcpyAsync(&before, &c->ctrl_api->limiter_total, sizeof(before), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  HIPCHK(launch_limit_fluxes(c->src_args(0, c->wcur, c->ctrl_api, beta_dt, false), dir, c->stream));
+  if (limited) {
+    HIPCHK(hipMemcpyAsync(&after, &c->ctrl_api->limiter_total, sizeof(after), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *limited = long(after - before);
+  }
+  return CHQ_OK;
+}
+
+int chq_apply_gravity(chq_ctx *c, int out, double beta_dt) {
+  if (!c || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_apply_gravity: FP64 contexts only");
+  if (c->gravity()) HIPCHK(launch_gravity(c->src_args(out, c->wcur, c->ctrl_api, beta_dt, false), c->stream));
+  return CHQ_OK;
+}
+
+int chq_apply_cooling(chq_ctx *c, int buf, double dt) {
+  if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
+  if (int rc_ = sync_wcur(c)) return rc_;
+  if (c->mixed) return fail(CHQ_EUNSUP, "chq_apply_cooling: FP64 contexts only");
+  if (c->cfg.cooling) HIPCHK(launch_cooling(c->src_args(buf, c->wcur, c->ctrl_api, dt, false), c->stream));
+  return CHQ_OK;
+}
+
 // Stage(), main.cpp:46-93, reference order (flag before boundaries; fofc branch taken on the
 // device when the flag kernel counted cells)
 int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt, long *nbad) {

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(TPB) k_gravity(const SrcArgs a, const Box b) {
   u[IEN * cs] += bdt * d * (a.w[IVX * cs + c] * a1 + a.w[IVY * cs + c] * a2 + a.w[IVZ * cs + c] * a3);
 }
 
-// srcterms::ApplyCooling (srcterms.hpp:55-71) over the active cells with the step's dt
+// srcterms::ApplyCooling (srcterms.hpp:55-71) over the active cells
 __global__ void __launch_bounds__(TPB) k_cooling(const SrcArgs a, const Box b) {
   if (!gated(a)) return;
   const long long t = (long long)blockIdx.x * TPB + threadIdx.x;
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(TPB) k_cooling(const SrcArgs a, const Box b) {
   b.at(t, k, j, i);
   const Geom &g = a.g;
   const long long c = g.at(k, j, i), cs = g.cstride;
-  const double dt = a.ctrl->dt, gm1 = a.eos.gm1;
+  const double dt = beta_dt(a), gm1 = a.eos.gm1;   // the step's dt in the time loop (beta = 1)
   const double d = a.w[IDN * cs + c], p = a.w[IPR * cs + c];
   const double t0 = p / d;
   const double cc = gm1 * d * a.lambda0;

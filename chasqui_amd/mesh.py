@@ -151,6 +151,17 @@ class Mesh:
         check(self.L.chq_diffusion_dt(self.h, C.byref(v)), "chq_diffusion_dt")
         return v.value
 
+    def limit_fluxes(self, d, beta_dt):
+        n = C.c_long(0)
+        check(self.L.chq_limit_fluxes(self.h, d, beta_dt, C.byref(n)), "chq_limit_fluxes")
+        return n.value
+
+    def apply_gravity(self, out, beta_dt):
+        check(self.L.chq_apply_gravity(self.h, out, beta_dt), "chq_apply_gravity")
+
+    def apply_cooling(self, buf, dt):
+        check(self.L.chq_apply_cooling(self.h, buf, dt), "chq_apply_cooling")
+
     def stage(self, cur, out, g0, g1, beta_dt):
         n = C.c_long(0)
         check(self.L.chq_stage(self.h, cur, out, g0, g1, beta_dt, C.byref(n)), "chq_stage")

@@ -141,6 +141,16 @@ int chq_diffusion_fluxes(chq_ctx *ctx, int dir, int cur);
 int chq_resistive_emf(chq_ctx *ctx, int cur);
 /* diffusion::TimeStep (diffusion.hpp:159-174) from the current primitives (synchronises). */
 int chq_diffusion_dt(chq_ctx *ctx, double *dt);
+/* positivity::LimitFluxes<DIR> (positivity.hpp:24-89): blends the current dir fluxes toward the
+ * local Lax-Friedrichs flux where the half-updates with beta_dt would leave rho or p below the
+ * floors; *limited (may be null; then no synchronisation) = faces limited. */
+int chq_limit_fluxes(chq_ctx *ctx, int dir, double beta_dt, long *limited);
+/* srcterms::ApplyGravity (srcterms.hpp:21-40): gravity sources into u[out] with beta_dt from the
+ * current primitives (no-op unless [gravity] is on). */
+int chq_apply_gravity(chq_ctx *ctx, int out, double beta_dt);
+/* srcterms::ApplyCooling (srcterms.hpp:55-71): exact power-law cooling of u[buf] over dt from
+ * the current primitives (no-op unless [cooling] is on). */
+int chq_apply_cooling(chq_ctx *ctx, int buf, double dt);
 /* Stage() (main.cpp:46-93): fluxes, corner EMF, update, fofc, boundaries; *nbad (may be
  * NULL) receives the fofc count (synchronises when non-NULL). */
 int chq_stage(chq_ctx *ctx, int cur, int out, double g0, double g1, double beta_dt, long *nbad);

@@ -140,6 +140,20 @@ int orc_fofc_repair(void *p, int dir, int cur) {
   return 0;
 }
 long orc_limiter_count(void *p) { return S(p)->limiter_count; }
+long orc_limit_fluxes(void *p, int dir, double bdt) {
+  Sim &s = *S(p);
+  if (dir == 0) return limit_fluxes<0>(s.m, s.cfg, bdt);
+  if (dir == 1) return limit_fluxes<1>(s.m, s.cfg, bdt);
+  return limit_fluxes<2>(s.m, s.cfg, bdt);
+}
+void orc_apply_gravity(void *p, int out, double bdt) {
+  Sim &s = *S(p);
+  apply_gravity(s.m, s.cfg, s.m.u(out), bdt);
+}
+void orc_apply_cooling(void *p, int buf, double dt) {
+  Sim &s = *S(p);
+  apply_cooling(s.m, s.cfg, s.m.u(buf), dt);
+}
 long orc_stage(void *p, int cur, int out, double g0, double g1, double bdt) {
   Sim &s = *S(p);
   return stage(s.m, s.cfg, cur, out, g0, g1, bdt);

@@ -66,6 +66,10 @@ def lib():
     L.orc_sts_count.argtypes = [vp]
     L.orc_limiter_count.restype = C.c_long
     L.orc_limiter_count.argtypes = [vp]
+    L.orc_limit_fluxes.restype = C.c_long
+    L.orc_limit_fluxes.argtypes = [vp, C.c_int, C.c_double]
+    L.orc_apply_gravity.argtypes = [vp, C.c_int, C.c_double]
+    L.orc_apply_cooling.argtypes = [vp, C.c_int, C.c_double]
     L.orc_stage.restype = C.c_long
     L.orc_stage.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
     L.orc_set_floors.argtypes = [vp, C.c_double, C.c_double]
@@ -226,6 +230,15 @@ class OracleSim:
 
     def diffusion_dt(self):
         return self.L.orc_diffusion_dt(self.h)
+
+    def limit_fluxes(self, d, bdt):
+        return self.L.orc_limit_fluxes(self.h, d, bdt)
+
+    def apply_gravity(self, out, bdt):
+        self.L.orc_apply_gravity(self.h, out, bdt)
+
+    def apply_cooling(self, buf, dt):
+        self.L.orc_apply_cooling(self.h, buf, dt)
 
     @property
     def sts_count(self):

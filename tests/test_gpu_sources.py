@@ -220,3 +220,47 @@ def test_sources_need_fp64_single_mesh():
             b.add(m)
         b.close()
         m.close()
+
+
+KERNEL_CASES = [
+    ("blast_3d.in", LIMITED_BLAST + UNIFORM),
+    ("blast_3d.in", ["mesh.nx1=12", "mesh.nx2=10", "mesh.nx3=8", "mhd.flux_limiter=true", "mhd.pfloor=1e-3",
+                     "problem.pres_out=1e-3"] + POINT),
+    ("orszag_tang.in", ["mesh.nx1=24", "mesh.nx2=20", "mhd.flux_limiter=true", "mhd.dual_energy=true",
+                        "cooling.enabled=true", "cooling.lambda0=2.0", "cooling.alpha=0.5"] + UNIFORM),
+]
+
+
+@pytest.mark.parametrize("name,ov", KERNEL_CASES, ids=[f"{c[0]}-{len(c[1])}" for c in KERNEL_CASES])
+def test_source_entry_points(name, ov):
+    """chq_limit_fluxes / chq_apply_gravity / chq_apply_cooling (positivity::LimitFluxes<DIR>,
+    srcterms::ApplyGravity, ApplyCooling) against the oracle's functions on the same state: the
+    limited fluxes and the gravity update bitwise with equal activation counts, cooling within
+    COOL_RTOL (exp/pow)."""
+    print(f"case source entry points {name} {ov}", flush=True)
+    o, gm, _ = _pair(deck(name), ov)
+    gm.upload("u1", o.get("u1"))
+    bdt = 2.0 * o.min_dt
+    limited = 0
+    for d in range(1 + o.two_d + o.three_d):
+        o.fluxes(d, 0)
+        gm.fluxes(d, 0)
+        n = gm.limit_fluxes(d, bdt)
+        assert n == o.limit_fluxes(d, bdt)
+        limited += n
+        f = f"f{d + 1}"
+        a, b = gm.download(f), o.get(f)
+        _same(np.delete(a, 5, axis=0), np.delete(b, 5, axis=0), f + " limited")
+    if ov[:len(LIMITED_BLAST)] == LIMITED_BLAST:
+        assert limited > 0
+    o.apply_gravity(1, bdt)
+    gm.apply_gravity(1, bdt)
+    u, ou = gm.download("u1"), o.get("u1")
+    for n in range(5):
+        _same(u[n], ou[n], f"u1[{n}] after gravity")
+    o.apply_cooling(0, bdt)
+    gm.apply_cooling(0, bdt)
+    u, ou = gm.download("u0"), o.get("u0")
+    for n in range(5):
+        _close(u[n], ou[n], COOL_RTOL, f"u0[{n}] after cooling")
+    gm.close()
