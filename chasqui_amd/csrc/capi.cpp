@@ -698,8 +698,15 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
     HIPCHK(launch_flux(fa, d, rc, rs, c->entropy, c->stream));
     c->tmark(CHQ_K_FLUX, false);
   }
-  // gravity (FP64 contexts only) reads the stage-input primitives, set wc
+  // flux limiter and gravity (FP64 contexts only) read the stage-input primitives, set wc
   constexpr bool fp64 = std::is_same<T, double>::value;
+  const bool lim = fp64 && c->cfg.flux_limiter;
+  if constexpr (fp64) {
+    if (lim) {   // main.cpp:60-64
+      const SrcArgs sa = c->src_args(cf.out, wc, k, cf.beta, true);
+      for (int d = 0; d < c->ndir(); ++d) HIPCHK(launch_limit_fluxes(sa, d, c->stream));
+    }
+  }
   c->tmark(CHQ_K_CORNER, true);
   HIPCHK(launch_corner<T>(c->corner_args_t(fs, wc, k, 0), c->stream));
   c->tmark(CHQ_K_CORNER, false);
@@ -710,15 +717,30 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   HIPCHK(launch_update<T>(ua, c->stream));
   c->tmark(CHQ_K_UPDATE, false);
   if (fp64 && c->gravity()) HIPCHK(launch_gravity(c->src_args(cf.out, wc, k, cf.beta, true), c->stream));   // main.cpp:76
+  if constexpr (fp64) {
+    // with the limiter the cells are flagged here, before the boundary fill, as the reference
+    // does (fofc.hpp:17-44, main.cpp:78-79): the limiter can leave the two copies of a periodic
+    // shared face unequal, and the fill would replace one of them
+    if (lim && c->cfg.fofc) {
+      C2pArgs fl = c->c2p_args(cf.out, wc, k);
+      fl.flag_mode = 2;
+      HIPCHK(launch_flag(fl, c->stream));
+    }
+  }
+  // With the limiter, the first boundary fill and con2prim run only when no cell was flagged:
+  // the fill would rewrite a periodic shared face whose two copies the limiter can leave
+  // unequal, and the fofc re-update reads the un-filled tentative faces (main.cpp:78-91).
+  const int first = lim && c->cfg.fofc ? 2 : 0;
   c->tmark(CHQ_K_BC, true);
-  HIPCHK(c->boundaries(cf.out, nvar, k, 0));
+  HIPCHK(c->boundaries(cf.out, nvar, k, first));
   c->tmark(CHQ_K_BC, false);
   C2pArgsT<T> ca = c->c2p_args_t(fs, cf.out, wn, k);
+  ca.only_if_bad = first;
   ca.write_went = c->entropy;
   ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
   ca.compute_dt = last && !(c->cfg.sts || c->cfg.cooling);   // split steps take dt after the splits
   ca.slot = 0;
-  ca.flag_mode = c->cfg.fofc ? 2 : 0;
+  ca.flag_mode = c->cfg.fofc && !lim ? 2 : 0;
   c->tmark(CHQ_K_C2P, true);
   HIPCHK(launch_c2p<T>(ca, c->stream));
   c->tmark(CHQ_K_C2P, false);
@@ -744,7 +766,7 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
     C2pArgsT<T> cb = ca;
     cb.flag_mode = 0;
     cb.slot = 1;
-    cb.only_if_bad = 1;
+    cb.only_if_bad = 1;   // (after ca's gate 2: exactly one of the two runs)
     HIPCHK(launch_c2p<T>(cb, c->stream));
     c->tmark(CHQ_K_REPAIR, false);
   }
@@ -764,14 +786,11 @@ static int enqueue_step(chq_ctx *c) {
   const RSolver rs = RSolver(c->cfg.rsolver);
   Ctrl *k = c->ctrl;
   const int nst = c->cfg.nstage;
-  // dual energy, explicit diffusion and the flux limiter run in the reference kernel order;
+  // dual energy and explicit diffusion run in the reference kernel order; the flux limiter,
   // gravity and the operator-split steps (super-time-stepping, cooling: the stages carry no
-  // diffusion then) also fit the fused stage. (The limiter leaves the faces that a periodic
-  // boundary fill pairs up unequal — it limits active faces only, not the ghost-column faces
-  // the corner EMF also reads — and the reference flags cells before that fill, which the
-  // fused stage, flagging inside con2prim after it, would not reproduce.)
+  // diffusion then) also fit the fused stage
   const bool split = c->cfg.sts || c->cfg.cooling;   // main.cpp:273-290: dt taken after the splits
-  const bool fused = !c->eos.dual_energy && !(c->diffusive() && !c->cfg.sts) && !c->cfg.flux_limiter;
+  const bool fused = !c->eos.dual_energy && !(c->diffusive() && !c->cfg.sts);
   const int nvar = c->entropy ? 6 : 5;
   HIPCHK(launch_ctrl_begin_step(k, c->stream));
   for (int s = 1; s <= nst; ++s) {

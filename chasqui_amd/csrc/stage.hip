@@ -19,9 +19,12 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
+// only_if_bad: 0 always, 1 only when the stage flagged cells (the fofc branch), 2 only when it
+// flagged none (the fused stage's first boundary fill and con2prim when the flux limiter is on)
 __device__ __forceinline__ bool gate(const Ctrl *c, int only_if_bad) {
   if (!c->active) return false;
-  if (only_if_bad && c->nbad == 0) return false;
+  if (only_if_bad == 1 && c->nbad == 0) return false;
+  if (only_if_bad == 2 && c->nbad != 0) return false;
   return true;
 }
 
@@ -589,7 +592,8 @@ __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
   c2p_block_merge<DT>(a, mdt, nbad_local, smin, ssum);
 }
 
-// standalone FlagBadCells over the active cells (fofc.hpp:17-44); count into ctrl->nbad
+// standalone FlagBadCells over the active cells (fofc.hpp:17-44); count into ctrl->nbad, with
+// the flag array (flag_mode 1 or 0) or appending to the flagged-cell list (flag_mode 2)
 __global__ void __launch_bounds__(BX * BY) k_flag(const C2pArgs a) {
   if (!gate(a.ctrl, 0)) return;
   const Geom &g = a.g;
@@ -613,9 +617,14 @@ __global__ void __launch_bounds__(BX * BY) k_flag(const C2pArgs a) {
       if (e.dual_energy && ei < e.de_eta * fabs(u[IEN * cs])) p = (u[IENT * cs] / d) * pow(d, e.gamma);
       bad = p <= e.pfloor;
     }
-    a.flag[c] = bad ? 1 : 0;
-    bad_count = bad;
+    if (a.flag_mode == 2) {   // append to the flagged-cell list (the fused stage's repair reads it)
+      if (bad) a.badlist[atomicAdd(&a.ctrl->nbad, 1)] = int(c);
+    } else {
+      a.flag[c] = bad ? 1 : 0;
+      bad_count = bad;
+    }
   }
+  if (a.flag_mode == 2) return;
   const int ws = wave_sum(bad_count);
   if ((int(threadIdx.x) & 63) == 0 && ws) atomicAdd(&a.ctrl->nbad, ws);
 }
@@ -938,8 +947,8 @@ void launch_c2p_de(const C2pArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
   const bool ent = a.entropy_out || a.eos.dual_energy;
   const VB v = c2p_vb(g);
-  const dim3 grid(grid_of(v, a.only_if_bad)), block(256);
-  if (a.only_if_bad) {
+  const dim3 grid(grid_of(v, a.only_if_bad == 1)), block(256);
+  if (a.only_if_bad == 1) {
     if (a.compute_dt) {
       if (ent) hipLaunchKernelGGL((k_c2p<T, true, true, DE, true>), grid, block, 0, s, a, v);
       else hipLaunchKernelGGL((k_c2p<T, true, false, DE, true>), grid, block, 0, s, a, v);
