@@ -717,20 +717,21 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   HIPCHK(launch_update<T>(ua, c->stream));
   c->tmark(CHQ_K_UPDATE, false);
   if (fp64 && c->gravity()) HIPCHK(launch_gravity(c->src_args(cf.out, wc, k, cf.beta, true), c->stream));   // main.cpp:76
+  // Strict order (flux limiter or dual energy on): the cells are flagged here, before the
+  // boundary fill, as the reference does (fofc.hpp:17-44, main.cpp:78-79), and the first fill
+  // and con2prim run only when no cell was flagged, because the fofc re-update must read the
+  // tentative state as the update left it (main.cpp:78-91): the limiter can leave the two
+  // copies of a periodic shared face unequal (the fill would replace one), and dual-energy
+  // con2prim rewrites u[IEN] of switched cells (physics.hpp:463-468).
+  const bool strict = fp64 && c->cfg.fofc && (lim || c->eos.dual_energy);
   if constexpr (fp64) {
-    // with the limiter the cells are flagged here, before the boundary fill, as the reference
-    // does (fofc.hpp:17-44, main.cpp:78-79): the limiter can leave the two copies of a periodic
-    // shared face unequal, and the fill would replace one of them
-    if (lim && c->cfg.fofc) {
+    if (strict) {
       C2pArgs fl = c->c2p_args(cf.out, wc, k);
       fl.flag_mode = 2;
       HIPCHK(launch_flag(fl, c->stream));
     }
   }
-  // With the limiter, the first boundary fill and con2prim run only when no cell was flagged:
-  // the fill would rewrite a periodic shared face whose two copies the limiter can leave
-  // unequal, and the fofc re-update reads the un-filled tentative faces (main.cpp:78-91).
-  const int first = lim && c->cfg.fofc ? 2 : 0;
+  const int first = strict ? 2 : 0;
   c->tmark(CHQ_K_BC, true);
   HIPCHK(c->boundaries(cf.out, nvar, k, first));
   c->tmark(CHQ_K_BC, false);
@@ -740,7 +741,7 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
   ca.compute_dt = last && !(c->cfg.sts || c->cfg.cooling);   // split steps take dt after the splits
   ca.slot = 0;
-  ca.flag_mode = c->cfg.fofc && !lim ? 2 : 0;
+  ca.flag_mode = c->cfg.fofc && !strict ? 2 : 0;
   c->tmark(CHQ_K_C2P, true);
   HIPCHK(launch_c2p<T>(ca, c->stream));
   c->tmark(CHQ_K_C2P, false);
@@ -778,19 +779,20 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
 // con2prim of the tentative state, whose primitives go to the second (w, bcc, ecc) set, so
 // the stage-input primitives the repair needs stay intact; the repair branch (list-driven
 // repair and corner EMF, update, boundaries, con2prim) always launches and exits at once on
-// the device when nothing was flagged. Entropy (IENT) is not transported: with dual energy
-// off con2prim recomputes u[IENT] from rho and p in every stage, so its transport cannot
-// change any result. With dual energy on, the reference kernel order is kept.
+// the device when nothing was flagged. Entropy (IENT) is not transported unless dual energy is
+// on: without it con2prim recomputes u[IENT] from rho and p in every stage, so its transport
+// cannot change any result. With dual energy or the flux limiter on, the flag runs before the
+// boundary fill (see enqueue_fused_stage); explicit diffusion keeps the reference kernel order.
 static int enqueue_step(chq_ctx *c) {
   const Recon rc = Recon(c->cfg.recon);
   const RSolver rs = RSolver(c->cfg.rsolver);
   Ctrl *k = c->ctrl;
   const int nst = c->cfg.nstage;
-  // dual energy and explicit diffusion run in the reference kernel order; the flux limiter,
+  // explicit diffusion runs in the reference kernel order; dual energy, the flux limiter,
   // gravity and the operator-split steps (super-time-stepping, cooling: the stages carry no
   // diffusion then) also fit the fused stage
   const bool split = c->cfg.sts || c->cfg.cooling;   // main.cpp:273-290: dt taken after the splits
-  const bool fused = !c->eos.dual_energy && !(c->diffusive() && !c->cfg.sts);
+  const bool fused = !(c->diffusive() && !c->cfg.sts);
   const int nvar = c->entropy ? 6 : 5;
   HIPCHK(launch_ctrl_begin_step(k, c->stream));
   for (int s = 1; s <= nst; ++s) {
