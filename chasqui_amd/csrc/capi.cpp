@@ -422,6 +422,9 @@ int sts_alloc(chq_ctx *c) {
   if (c->sts_buf) return CHQ_OK;
   const size_t cs = size_t(c->g.cstride);
   HIPCHK(hipMalloc(&c->sts_buf, (5 * NVAR + 15) * cs * sizeof(double)));
+  // the face combination covers [is..ie+1] x [js..je+1] x [ks..ke+1] on every component, beyond
+  // where the operator writes mb?: those entries stay zero for the life of the context
+  HIPCHK(launch_fill_zero(c->sts_buf, (long long)((5 * NVAR + 15) * cs), c->stream));
   double *p = c->sts_buf;
   for (int n = 0; n < 3; ++n, p += NVAR * cs) c->sy[n] = p;
   c->sm0 = p; p += NVAR * cs;
