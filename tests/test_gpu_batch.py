@@ -158,3 +158,75 @@ def test_batch_rejects_mixed_schemes():
     b.close()
     m1.close()
     m2.close()
+
+
+def _batch_cases(seed=7, n=8):
+    """Seeded schemes (shared by the batch) and per-run options (kept per run)."""
+    import random
+    rng = random.Random(seed)
+    decks = ["blast_3d.in", "orszag_tang.in", "brio_wu.in", "cpaw_1d.in", "field_loop.in"]
+    out = []
+    for _ in range(n):
+        recon = rng.choice(["dc", "plm", "ppm", "ppmx", "wenoz"])
+        scheme = [f"mhd.reconstruct={recon}", f"mhd.rsolver={rng.choice(['hlle', 'hlld', 'lhlld'])}",
+                  f"time.integrator={rng.choice(['rk1', 'rk2', 'rk3', 'rk4'])}"]
+        prec = rng.choice(["fp64", "fp64", "mixed"])
+        runs = []
+        for _ in range(rng.randint(2, 5)):
+            name = rng.choice(decks)
+            dims = 3 if name == "blast_3d.in" else rng.choice([1, 2])
+            nx = [rng.randint(4, 18) for _ in range(3)]
+            ov = [f"mesh.nx1={nx[0]}", f"mesh.nx2={nx[1] if dims > 1 else 1}", f"mesh.nx3={nx[2] if dims > 2 else 1}",
+                  f"time.cfl={rng.choice([0.2, 0.3, 0.4])}", f"mhd.fofc={rng.choice(['true', 'false'])}",
+                  f"mhd.flatten={rng.choice(['true', 'false'])}"]
+            for ax in range(dims):
+                ov.append(f"mesh.bc_x{ax + 1}={rng.choice(['periodic', 'outflow', 'reflect'])}")
+            if rng.random() < 0.3:
+                ov.append("mhd.low_mach=true")
+            if rng.random() < 0.3:
+                ov.append("mhd.pfloor=0.11")   # fofc fires where it is on
+            if rng.random() < 0.3:
+                ov.append(f"time.nlim={rng.randint(1, 5)}")   # finishes before the others
+            runs.append((name, ov))
+        out.append((scheme, prec, runs, rng.randint(3, 8)))
+    return out
+
+
+BATCH_CASES = _batch_cases()
+
+
+@pytest.mark.parametrize("scheme,prec,runs,steps", BATCH_CASES,
+                         ids=[f"{i}-{c[1]}-{c[0][0].split('=')[1]}" for i, c in enumerate(BATCH_CASES)])
+def test_batch_seeded_schemes_equal_solo(scheme, prec, runs, steps):
+    """Every scheme the batch accepts, with runs that differ in problem, dimensionality, size,
+    boundaries, CFL, fofc, flattening, the low-Mach blend, a pressure floor and their step limit:
+    each batched run ends bitwise where it ends stepped alone."""
+    from chasqui_amd import Batch
+    from chasqui_amd.deck import Deck
+    print(f"case batch {scheme} {prec} {runs} steps={steps}", flush=True)
+
+    def mk(name, ov):
+        d = Deck(deck(name), scheme + ["time.tlim=1e9", "time.nlim=-1"] + ov + [f"mhd.precision={prec}"])
+        m = d.make_mesh()
+        d.close()
+        return m
+
+    solo = []
+    for name, ov in runs:
+        m = mk(name, ov)
+        m.step(steps)
+        solo.append(_state(m))
+        m.close()
+    b = Batch(8)
+    meshes = [mk(name, ov) for name, ov in runs]
+    for m in meshes:
+        b.add(m)
+    b.step(steps)
+    b.sync()
+    for i, m in enumerate(meshes):
+        nlim = [int(o.split("=")[1]) for o in runs[i][1] if o.startswith("time.nlim=")]
+        assert solo[i]["status"][1] == min([steps] + nlim), f"run {i}: cycle {solo[i]['status'][1]}"
+        _same(_state(m), solo[i], f"run {i} {runs[i]}")
+    b.close()
+    for m in meshes:
+        m.close()
