@@ -147,3 +147,36 @@ def test_stiff_diffusion_step_counts_on_the_device():
         st = gm.status()
         assert st["cycle"] == steps and st["t"] == 0.05 and st["sts_stages"] == stages, st
         gm.close()
+
+
+def test_sts_scratch_starts_clear_after_other_contexts():
+    """Regression: the RKL2 face combination covers [is..ie+1] x [js..je+1] x [ks..ke+1] on every
+    face component, beyond the entries where the resistive operator writes its curl terms. A 3D
+    RKL2 run fills and frees its scratch; a 1D run created after it (whose scratch may reuse that
+    memory) must still match the oracle bitwise, b2 at i = ie+1 included."""
+    from chasqui_amd.deck import Deck
+    sts = ["diffusion.eta=0.002", "diffusion.sts=true", "time.nlim=-1", "time.tlim=1e9"]
+    print("case RKL2 3D blast 12^3, then RKL2 1D Brio-Wu 8 cells", flush=True)
+    d = Deck(deck("blast_3d.in"), sts + ["mesh.nx1=12", "mesh.nx2=12", "mesh.nx3=12"])
+    m = d.make_mesh()
+    d.close()
+    m.step(3)
+    assert m.status()["sts_stages"] > 0
+    m.close()
+    ov = sts + ["mesh.nx1=8", "mesh.nx2=1", "mesh.nx3=1", "mhd.reconstruct=dc", "mhd.rsolver=hlld",
+                "time.integrator=rk3", "time.cfl=0.2", "mesh.bc_x1=outflow", "mhd.pfloor=0.11"]
+    d = Deck(deck("brio_wu.in"), ov)
+    m = d.make_mesh()
+    d.close()
+    m.step(5)
+    o = po.OracleSim(deck("brio_wu.in"), ov)
+    o.step(5)
+    assert m.status()["sts_stages"] == o.sts_count > 0
+    u, ou = m.download("u0"), o.get("u0")
+    for q in range(5):
+        assert n_diff(u[q], ou[q]) == 0, f"u0[{q}]"
+    for dd in range(3):
+        b = f"b{dd + 1}f0"
+        r = face_region(o, dd)
+        assert n_diff(m.download(b)[r], o.get(b)[r]) == 0, b
+    m.close()
