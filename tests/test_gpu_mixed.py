@@ -192,3 +192,64 @@ def test_mixed_rejects_reference_order_entries():
     with pytest.raises(RuntimeError, match="FP64 contexts only"):
         gm.flag_bad_cells(0)
     gm.close()
+
+
+def _mixed_cases(seed=31, n=16):
+    import random
+    rng = random.Random(seed)
+    decks = ["blast_3d.in", "orszag_tang.in", "brio_wu.in", "cpaw_1d.in", "field_loop.in", "cpaw_3d_oblique.in"]
+    out = []
+    for _ in range(n):
+        name = rng.choice(decks)
+        dims = 3 if name in ("blast_3d.in", "cpaw_3d_oblique.in") else rng.choice([1, 2])
+        recon = rng.choice(["dc", "plm", "ppm", "ppmx", "wenoz"])
+        nx = [rng.randint(6, 24) for _ in range(3)]
+        ov = [f"mesh.nx1={nx[0]}", f"mesh.nx2={nx[1] if dims > 1 else 1}", f"mesh.nx3={nx[2] if dims > 2 else 1}",
+              f"mhd.reconstruct={recon}", f"mhd.rsolver={rng.choice(['hlle', 'hlld', 'lhlld'])}",
+              f"time.integrator={rng.choice(['rk1', 'rk2', 'rk3', 'rk4'])}", "time.nlim=-1", "time.tlim=1e9",
+              f"time.cfl={rng.choice([0.2, 0.3, 0.4])}", f"mhd.fofc={rng.choice(['true', 'false'])}",
+              f"mhd.flatten={rng.choice(['true', 'false'])}"]
+        for ax in range(dims):
+            ov.append(f"mesh.bc_x{ax + 1}={rng.choice(['periodic', 'outflow', 'reflect'])}")
+        if rng.random() < 0.3:
+            ov.append("mhd.low_mach=true")
+        out.append((name, ov, rng.randint(3, 8)))
+    return out
+
+
+MIXED_CASES = _mixed_cases()
+
+
+@pytest.mark.parametrize("name,ov,steps", MIXED_CASES,
+                         ids=[f"{i:02d}-{c[0].split('.')[0]}" for i, c in enumerate(MIXED_CASES)])
+def test_mixed_seeded_options_close_to_fp64(name, ov, steps):
+    """Seeded problems, meshes, schemes, boundaries and switches (fofc, flattening, low-Mach
+    blend; the mixed path takes neither dual energy nor characteristic variables) in mixed
+    precision against the same run in FP64:
+    the same number of cycles, t within 1e-4, each conserved field within 1e-3 relative L1
+    (the scale of a field that stays at zero by symmetry is 1e-6 of the total energy), positive
+    density and pressure, div B unchanged to round-off."""
+    print(f"case mixed {name} {ov} steps={steps}", flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        d = _open(name, ov + [f"mhd.precision={prec}"])
+        m = d.make_mesh()
+        div0 = _divb(m, d)
+        m.step(steps)
+        md = d.mesh_desc
+        bmax = max(float(np.max(np.abs(b))) for b in _faces(m))
+        div_change = float(np.max(np.abs(_divb(m, d) - div0))) * (md.x1max - md.x1min) / md.nx1 / max(bmax, 1e-300)
+        out[prec] = (m.download("u0"), m.download("w"), m.status(), div_change, _active(md))
+        m.close()
+        d.close()
+    u64, w64, s64, _, (k, j, i) = out["fp64"]
+    u32, w32, s32, div32, _ = out["mixed"]
+    assert s32["cycle"] == s64["cycle"] == steps
+    assert abs(s32["t"] - s64["t"]) <= 1e-4 * s64["t"]
+    e_scale = float(np.sum(np.abs(u64[4][k, j, i])))
+    for n in range(5):
+        a, b = u32[n][k, j, i], u64[n][k, j, i]
+        rel = np.sum(np.abs(a - b)) / max(float(np.sum(np.abs(b))), 1e-6 * e_scale)
+        assert rel <= 1e-3, (n, rel)
+    assert np.min(w32[0][k, j, i]) > 0 and np.min(w32[4][k, j, i]) > 0
+    assert div32 <= 5e-13
