@@ -679,6 +679,7 @@ int chq_init_resume(chq_ctx *c, double t, int cycle, double tlim, int nlim) {
   }
   h.active = 0;
   h.cycle = cycle;
+  h.cycle0 = cycle;   // time.nlim counts from here (main.cpp:241-243)
   h.wcur = c->wcur;
   HIPCHK(hipMemcpyAsync(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -44,8 +44,9 @@ struct Ctrl {
   double t, dt, tlim, cfl;
   double min_dt;                 // CFL minimum of the last completed step
   unsigned long long mdt_bits[2];  // running minima (bit patterns of positive doubles)
-  int active;                    // this step runs (t < tlim and cycle < nlim)
+  int active;                    // this step runs (t < tlim and cycle - cycle0 < nlim)
   int cycle, nlim;
+  int cycle0;                    // cycle the loop started from (a restart's cycle): nlim counts from it
   int nbad;                      // fofc flags of the current stage
   int nbad_last;                 // flags of the last stage (selects the repaired minimum)
   int wcur;                      // current primitive set (batched runs; solo contexts track it on the host)

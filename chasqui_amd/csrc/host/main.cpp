@@ -8,6 +8,8 @@
 // files, restart from a .rst dump.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -57,7 +59,9 @@ int main(int argc, char *argv[]) {
   const double dt_vtk = std::stod(chq_deck_get(deck, "output", "vtk_dt"));
   const double dt_hst = std::stod(chq_deck_get(deck, "output", "hst_dt"));
   const double dt_rst = std::stod(chq_deck_get(deck, "output", "rst_dt"));
-  const std::string ce = chq_deck_get(deck, "output", "compute_error");
+  // GetOrAddBool semantics (params.hpp:67-72): the value is compared in lower case
+  std::string ce = chq_deck_get(deck, "output", "compute_error");
+  std::transform(ce.begin(), ce.end(), ce.begin(), [](unsigned char ch) { return char(std::tolower(ch)); });
   const bool compute_error = ce == "true" || ce == "1" || ce == "yes";
 
   std::cout << "# Chasqui MHD\n";
@@ -151,7 +155,7 @@ int main(int argc, char *argv[]) {
   if (dt_vtk > 0 || dt_hst > 0 || dt_rst > 0) do_outputs(0.0, true, true);
 
   const double zones = double(mesh.nx1) * mesh.nx2 * mesh.nx3;
-  std::cout << "# cycles = " << cycle << ", time = " << t << ", wall = " << wall << " s\n";
+  std::cout << "# cycles = " << cycle - cycle0 << ", time = " << t << ", wall = " << wall << " s\n";
   std::cout << "# zone-cycles/second = " << std::scientific << zones * (cycle - cycle0) / wall << "\n";
   if (cfg.fofc) std::cout << "# first-order flux corrections applied: " << st.fofc_total << "\n";
   if (cfg.sts) std::cout << "# super-time-stepping stages (total): " << st.sts_stages << "\n";

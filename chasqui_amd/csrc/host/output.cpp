@@ -63,14 +63,14 @@ extern "C" {
 // WriteVTK (output.hpp:32-85): BINARY RECTILINEAR_GRID, big-endian float32 cell data
 // rho prs vx1 vx2 vx3 Bx1 Bx2 Bx3 over the active cells, i fastest.
 int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, double time, int cycle) {
-  if (!ctx || !mesh || !path) return CHQ_EINVAL;
+  if (!ctx || !mesh || !path) return chq::fail(CHQ_EINVAL, "chq_write_vtk: null context, mesh or path");
   const Geo g = geo_of(ctx);
   std::vector<double> w(chq_array_count(ctx, CHQ_W)), b(chq_array_count(ctx, CHQ_BCC));
   int rc = chq_download(ctx, CHQ_W, w.data(), w.size());
   if (!rc) rc = chq_download(ctx, CHQ_BCC, b.data(), b.size());
   if (rc) return rc;
   std::ofstream f(path, std::ios::binary);
-  if (!f) return CHQ_EINVAL;
+  if (!f) return chq::fail(CHQ_EINVAL, std::string("cannot write ") + path);
   const double dx1 = (mesh->x1max - mesh->x1min) / g.nx1, dx2 = (mesh->x2max - mesh->x2min) / g.nx2,
                dx3 = (mesh->x3max - mesh->x3min) / g.nx3;
   f << "# vtk DataFile Version 2.0\n" << "Chasqui MHD VTK Data\n" << "BINARY\n" << "DATASET RECTILINEAR_GRID\n";
@@ -120,7 +120,7 @@ int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, dou
 // the active cells), max |div B| and its mean, from u and face B of set `buf` and the
 // current primitives.
 int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]) {
-  if (!ctx || !mesh || !out || (buf & ~1)) return CHQ_EINVAL;
+  if (!ctx || !mesh || !out || (buf & ~1)) return chq::fail(CHQ_EINVAL, "chq_history: null argument or bad buffer index");
   const Geo g = geo_of(ctx);
   std::vector<double> u(chq_array_count(ctx, CHQ_U0)), w(chq_array_count(ctx, CHQ_W)),
       bc(chq_array_count(ctx, CHQ_BCC)), b1(chq_array_count(ctx, CHQ_B1F0)), b2(b1.size()), b3(b1.size());
@@ -165,7 +165,7 @@ int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]
 
 // restart::Write (restart.hpp:19-34): header, then u0 and b1f0..b3f0 in the reference layout
 int chq_restart_write(chq_ctx *ctx, const char *path, double time, int cycle) {
-  if (!ctx || !path) return CHQ_EINVAL;
+  if (!ctx || !path) return chq::fail(CHQ_EINVAL, "chq_restart_write: null context or path");
   int iv[16];
   if (int e = chq_geometry(ctx, iv, nullptr)) return e;
   RstHeader h;
@@ -190,7 +190,7 @@ int chq_restart_write(chq_ctx *ctx, const char *path, double time, int cycle) {
 // restart::Read (restart.hpp:37-58) into u0 and b1f0..b3f0; the caller then runs chq_init_resume
 // (boundaries + con2prim, main.cpp:196-201)
 int chq_restart_read(chq_ctx *ctx, const char *path, double *time, int *cycle) {
-  if (!ctx || !path) return CHQ_EINVAL;
+  if (!ctx || !path) return chq::fail(CHQ_EINVAL, "chq_restart_read: null context or path");
   std::ifstream f(path, std::ios::binary);
   if (!f) return chq::fail(CHQ_EINVAL, std::string("cannot open restart file ") + path);
   RstHeader h;

@@ -633,9 +633,12 @@ __global__ void __launch_bounds__(BX * BY) k_flag(const C2pArgs a) {
 __device__ __forceinline__ unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// main.cpp:246-248
+// main.cpp:243-248 (nlim counts the cycles run since the loop started, main.cpp:243)
+__device__ __forceinline__ bool ctrl_running(const Ctrl *c) {
+  return c->t < c->tlim && (c->nlim < 0 || c->cycle - c->cycle0 < c->nlim);
+}
 __device__ __forceinline__ void ctrl_begin_step(Ctrl *c) {
-  if (!(c->t < c->tlim && (c->nlim < 0 || c->cycle < c->nlim))) { c->active = 0; return; }
+  if (!ctrl_running(c)) { c->active = 0; return; }
   c->active = 1;
   double dt = c->cfl * c->min_dt;
   if (c->t + dt > c->tlim) dt = c->tlim - c->t;
@@ -878,7 +881,7 @@ __global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *n
   else if (op == CTRL_END_STAGE) ctrl_end_stage(c, arg, 1);
   else {
     if (op == CTRL_END_STEP) ctrl_end_step(c);
-    live[r] = (c->t < c->tlim && (c->nlim < 0 || c->cycle < c->nlim)) ? 1 : 0;
+    live[r] = ctrl_running(c) ? 1 : 0;
   }
 }
 

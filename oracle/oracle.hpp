@@ -1429,7 +1429,7 @@ struct Sim {
   Cfg cfg;
   Grid m;
   Real t = 0.0, tlim = 1.0, min_dt = 0.0, last_dt = 0.0;
-  int cycle = 0, nlim = -1;
+  int cycle = 0, cycle0 = 0, nlim = -1;   // nlim counts from cycle0 (main.cpp:241-243)
   long fofc_count = 0, sts_count = 0, limiter_count = 0;
   StsWork sts_work;
   std::string problem;
@@ -1506,7 +1506,7 @@ struct Sim {
 
   // one step of the time loop, main.cpp:247-292 (returns false when the loop would stop)
   bool step() {
-    if (!(t < tlim && (nlim < 0 || cycle < nlim))) return false;
+    if (!(t < tlim && (nlim < 0 || cycle - cycle0 < nlim))) return false;
     Real dt = cfg.cfl * min_dt;
     if (t + dt > tlim) dt = tlim - t;
     for (int s = 1; s <= cfg.nstage; ++s) {
@@ -1569,6 +1569,7 @@ struct Sim {
     if (!f) throw std::runtime_error("restart file is truncated: " + path);
     t = h.time;
     cycle = h.cycle;
+    cycle0 = h.cycle;
     boundaries(m, cfg, 0);
     min_dt = cons_to_prim(m, cfg, 0, true);
   }

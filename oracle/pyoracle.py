@@ -172,6 +172,12 @@ class OracleSim:
         p = self.L.orc_flag(self.h, C.byref(n))
         return np.ctypeslib.as_array(p, shape=(n.value,)).reshape(self.nk, self.nj, self.ni).copy()
 
+    def flag_view(self):
+        """A live numpy view of the oracle's fofc flag array (cells, reference layout)."""
+        n = C.c_long()
+        p = self.L.orc_flag(self.h, C.byref(n))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).reshape(self.nk, self.nj, self.ni)
+
     def step(self, n=1):
         return self.L.orc_step(self.h, int(n))
 

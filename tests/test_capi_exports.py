@@ -50,8 +50,9 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 
 def test_null_handles_are_rejected_without_a_device():
     """Every entry point that takes a context, batch or server handle and returns a status
-    rejects a NULL handle with CHQ_EINVAL and a message, before any device call (this runs with
-    no GPU); chq_batch_size, a count, gives 0."""
+    rejects a NULL handle with CHQ_EINVAL and a message of its own, before any device call (this
+    runs with no GPU); chq_batch_size, a count, gives 0. Before each call the thread's message is
+    set to a known one (chq_create's), so a call that returns without setting its own fails."""
     import ctypes as C
     from chasqui_amd import _capi
     L = _capi.lib()
@@ -60,14 +61,18 @@ def test_null_handles_are_rejected_without_a_device():
         f = getattr(L, n)
         if f.restype is not C.c_int or not f.argtypes or f.argtypes[0] is not C.c_void_p:
             continue
-        if n.startswith("chq_deck") or n.endswith("_destroy") or n in ("chq_write_vtk", "chq_history"):
+        if n.startswith("chq_deck") or n.endswith("_destroy"):
             continue   # deck calls report through chq_deck_last_error; destroy(NULL) is a no-op
+        assert L.chq_create(None, None, None) == -1
+        marker = L.chq_last_error()
+        assert marker
         args = [0 if t in (C.c_int, C.c_long, C.c_size_t) else 0.0 if t is C.c_double else None for t in f.argtypes]
         rc = f(*args)
         if n == "chq_batch_size":
             assert rc == 0
             continue
         assert rc == -1, (n, rc)          # CHQ_EINVAL
-        assert L.chq_last_error(), n
+        msg = L.chq_last_error()
+        assert msg and msg != marker, (n, msg)
         checked += 1
     assert checked >= 40

@@ -56,6 +56,28 @@ def test_oracle_restart_is_bitwise(tmp_path, name, ov):
     _same(_state(b), ref, "resumed")
 
 
+def test_oracle_nlim_counts_from_the_restart_cycle(tmp_path):
+    """main.cpp:241-243: the loop runs while cycle - cycle0 < nlim, cycle0 being the cycle it
+    started from, so a run resumed at cycle 4 with time.nlim=3 stops at cycle 7 (and a dump
+    whose cycle is beyond nlim still runs nlim steps)."""
+    name, ov = OT
+    text = deck(name)
+    a = po.OracleSim(text, ov)
+    a.step(4)
+    path = tmp_path / "a.rst"
+    a.restart_write(path)
+    full = po.OracleSim(text, ov)
+    full.step(7)
+    for nlim, end in ((3, 7), (2, 6)):
+        b = po.OracleSim(text, ov + [f"time.nlim={nlim}"])
+        b.restart_read(path)
+        assert b.run() == nlim and b.cycle == end
+    b = po.OracleSim(text, ov + ["time.nlim=3"])
+    b.restart_read(path)
+    b.run()
+    _same(_state(b), _state(full), "resumed with nlim")
+
+
 def test_oracle_restart_rejects_mismatch(tmp_path):
     a = po.OracleSim(deck(OT[0]), OT[1])
     path = tmp_path / "a.rst"
@@ -121,6 +143,42 @@ def test_gpu_restart_is_bitwise_and_interoperates(tmp_path, name, ov):
     g.step(3)
     _same(_gpu_state(g), ref, "device resumed from an oracle dump", ent_rtol=1e-14)
     g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_nlim_counts_from_the_restart_cycle(tmp_path):
+    """A device run resumed at cycle 4 with nlim = 3 ends at cycle 7, bitwise the uninterrupted
+    7-step run (main.cpp:241-243); the driver reports the cycles it ran, not the absolute one."""
+    name, ov = OT
+    print(f"case nlim from restart {name} {ov}", flush=True)
+    m, tl, nl = _mesh(name, ov)
+    m.step(7)
+    ref = _gpu_state(m)
+    m.close()
+    m, tl, nl = _mesh(name, ov)
+    m.step(4)
+    st = m.status()
+    dump = tmp_path / "gpu.rst"
+    m.restart_write(dump, st["t"], st["cycle"])
+    m.close()
+    r, _, _ = _mesh(name, ov)
+    t, cyc = r.restart_read(dump)
+    r.init(t, tl, 3, cyc)
+    r.step(10)
+    rs = r.status()
+    assert rs["cycle"] == 7 and rs["active"] == 0
+    _same(_gpu_state(r), ref, "device resume with nlim")
+    r.close()
+    ot = os.path.join(DECKS, "orszag_tang.in")
+    common = ["mesh.nx1=32", "mesh.nx2=24", "time.ndiag=0", "output.hst_dt=-1", "output.vtk_dt=-1"]
+    r1 = subprocess.run([EXE, "-i", ot] + common + ["time.nlim=4", "output.basename=a", "output.rst_dt=1e-9"],
+                        cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r1.returncode == 0, r1.stderr
+    dumps = sorted(p for p in os.listdir(tmp_path) if p.startswith("a.") and p.endswith(".rst"))
+    r2 = subprocess.run([EXE, "-i", ot, "-r", dumps[-1]] + common + ["time.nlim=3", "output.basename=b"],
+                        cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    assert "# cycles = 3," in r2.stdout, r2.stdout
 
 
 @pytest.mark.gpu
