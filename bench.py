@@ -6,14 +6,20 @@ multiplicative noise, p = 0.1 ambient / 10 for r < 0.1, B = (1/sqrt2, 1/sqrt2, 0
 PPM + HLLD + RK3 + CT + fofc + flattening, CFL 0.3 (synthetic seeded initial state from the
 library's own problem generator). A step is one full RK3 step (three stages) of the
 device-resident pipeline; the timed region is K steps between two events on the
-pipeline's stream, after W warm-up steps, bracketed by stream synchronisation.
+pipeline's stream, after W warm-up steps, bracketed by stream synchronisation, with no other
+event inside it.
 
-Output: one JSON line. `roofline` is for the dominant kernel (the face-flux kernel),
-timed with HIP events on its stream inside the timed region; `step_roofline` applies the
-reference's traffic model (1040 B per cell per RK stage, README.md:45-47) to the whole step;
-`cpu_baseline` times the CPU oracle on this host's cores on one step of the same workload;
-`mixed` is the same workload on the mixed-precision path (FP32 reconstruction, Riemann solve
-and EMFs; FP64 con2prim, RK combine, CT), reported beside the FP64 headline.
+Output: one JSON line. `kernels` is a second pass over the same W + K steps (a fresh mesh
+follows the same trajectory bitwise, fofc firings included) with HIP events around every
+launch on the pipeline's stream: per-class milliseconds per step, the fofc branch included.
+`roofline` is for the dominant kernel (the face-flux kernel), its average launch time from
+that pass; its `traffic` is read from the committed rocprofv3 counter summary of the same
+workload. `step_roofline` applies the reference's traffic model (1040 B per cell per RK stage,
+README.md:45-47) to the whole step; `stream_probe` is the device's streaming rate with the
+stage kernels' access pattern (chq_stream_probe); `cpu_baseline` times the CPU oracle on this
+host's cores on a few steps of the same workload; `mixed` is the same workload on the
+mixed-precision path (FP32 reconstruction, Riemann solve and EMFs; FP64 con2prim, RK combine,
+CT); `batch` the batched Orszag-Tang ensemble; `serve` the served stream.
 """
 import argparse
 import json
@@ -33,11 +39,12 @@ MODEL_BYTES_PER_ZONE_STAGE = 1040  # README.md:45-47 (130 doubles per cell per R
 # 52.0 / 3 doubles per active cell per launch.
 FLUX_DOUBLES_PER_CELL_STAGE = 52.0
 
-# HBM bytes per flux-kernel launch (mean of the x, y and z launches), measured with rocprofv3
-# --pmc FETCH_SIZE / WRITE_SIZE and scaled by the reading of a plain 2 GiB device copy
-# (profiles/blast256_fp64_hbm_traffic.md): read 1252 / 1470 / 1471 MiB, write 940 / 962 / 962
-# MiB per x / y / z launch.
-FLUX_TRAFFIC_PER_LAUNCH = (1252 + 940 + 1470 + 962 + 1471 + 962) / 3 * 2 ** 20
+# Memory traffic per flux-kernel launch (mean of the x, y and z launches) comes from the
+# committed rocprofv3 counter summary of this workload (read + write MiB per launch, scaled by
+# the counters' reading of the streaming probe's known bytes).
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "blast256_fp64_hbm_traffic.md")
+FLUX_KERNELS = ("void chq::fluxk::k_flux_x<double", "void chq::fluxk::k_flux_m<double")
+STREAM_PROBE = os.path.join(HERE, "chasqui_amd", "_lib", "chq_stream_probe")
 
 DECK = """
 [mesh]
@@ -100,38 +107,71 @@ def cpu_baseline(n, steps):
                       f"(oracle/, -O3 -ffp-contract=off, OMP_PROC_BIND=spread)"}
 
 
-def run(n, precision, steps, warmup, breakdown=False, state=None):
-    """K timed steps of the device-resident pipeline on the n^3 blast (after W warm-up steps);
-    returns (ms, flux kernel ms, flux launches, status, per-kernel breakdown). With `state` (a
-    dict), the final conserved state of the active cells is left in state["u0"]."""
+def _blast_mesh(n, precision):
     from chasqui_amd.deck import Deck
     deck = Deck(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={n}", f"mhd.precision={precision}"])
     mesh = deck.make_mesh()
     deck.close()
+    return mesh
+
+
+def run(n, precision, steps, warmup, breakdown=False, state=None):
+    """K timed steps of the device-resident pipeline on the n^3 blast (after W warm-up steps),
+    with no per-launch events in the timed region; returns (ms, status, breakdown). With
+    `breakdown`, the same W + K steps run again on a fresh mesh (the same trajectory, bitwise:
+    the same fofc firings) with HIP events around every launch, giving per-class times on the
+    pipeline's stream: flux, corner, update, bc, c2p, and fofc_branch (the repair, edge EMFs,
+    re-update, boundaries and con2prim of stages that flagged cells; near zero when none did).
+    With `state` (a dict), the final conserved state of the active cells is left in
+    state["u0"]."""
+    mesh = _blast_mesh(n, precision)
     mesh.step(warmup)
     mesh.sync()
-    mesh.kernel_timing("flux")
     ms = mesh.step_timed(steps)
-    flux_ms, flux_launches = mesh.kernel_timing_read()
-    mesh.kernel_timing(None)
     st = mesh.status()
-    kernels = {}
-    if breakdown:   # per-kernel breakdown on two further (untimed) steps
-        for cls in ("flux", "corner", "update", "bc", "c2p", "repair"):
-            mesh.kernel_timing(cls)
-            mesh.step(2)
-            kms, kn = mesh.kernel_timing_read()
-            kernels[cls] = {"ms_per_step": kms / 2, "launches_per_step": kn // 2}
-        mesh.kernel_timing(None)
     if state is not None:
         u = mesh.download("u0")
         ng = (u.shape[3] - n) // 2
         act = slice(ng, ng + n)
         state["u0"] = u[:5, act, act, act].copy()
-        state["cycle"] = mesh.status()["cycle"]
+        state["cycle"] = st["cycle"]
         del u
     mesh.close()
-    return ms, flux_ms, flux_launches, st, kernels
+    kernels = {}
+    if breakdown:
+        mesh = _blast_mesh(n, precision)
+        mesh.step(warmup)
+        mesh.sync()
+        mesh.kernel_timing("all")
+        ims = mesh.step_timed(steps)
+        bd = mesh.kernel_timing_breakdown()
+        mesh.kernel_timing(None)
+        st2 = mesh.status()
+        mesh.close()
+        if st2["fofc_total"] != st["fofc_total"] or st2["t"] != st["t"]:
+            raise RuntimeError("the instrumented pass did not follow the timed trajectory")
+        for cls, (kms, kn) in bd.items():
+            kernels["fofc_branch" if cls == "repair" else cls] = {
+                "ms_per_step": kms / steps, "launches_per_step": kn / steps, "total_ms": kms, "launches": kn}
+        kernels["instrumented_ms_per_step"] = ims / steps
+    return ms, st, kernels
+
+
+def profile_traffic(path, kernel_prefixes):
+    """Mean read + write MiB per launch of the kernels whose names start with one of
+    `kernel_prefixes`, from a committed rocprofv3 traffic summary (markdown table rows
+    | `kernel` | launches | read MiB | write MiB |), in bytes; None if absent."""
+    import re
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    vals = []
+    for m in re.finditer(r"^\|\s*`([^`]+)`\s*\|\s*(\d+)\s*\|\s*([\d.]+)\s*\|\s*([\d.]+)\s*\|", text, re.M):
+        name = m.group(1)
+        if any(name.startswith(p) for p in kernel_prefixes):
+            vals.append((float(m.group(3)) + float(m.group(4))) * 2 ** 20)
+    return sum(vals) / len(vals) if vals else None
 
 
 OT_DECK = """
@@ -277,17 +317,30 @@ def serve(njobs, rate_factor=8.0, seed=11, budget_gb=200.0, slots=64, chunk=8):
     seq_rate = njobs / seq_s
     rng = np.random.default_rng(seed + 1)
     gaps = rng.exponential(1.0 / (rate_factor * seq_rate), size=njobs)
-    arrivals = [(float(t), text, ov, "mixed") for t, (text, ov) in zip(np.cumsum(gaps) - gaps[0], jobs)]
-    srv = Server(int(budget_gb * 2 ** 30), slots=slots, chunk=chunk)
-    t0 = time.perf_counter()
-    recs = srv.play(arrivals)
-    span = time.perf_counter() - t0
-    phases = srv.stats()
-    srv.close()
+    t_arr = np.cumsum(gaps) - gaps[0]
+
+    def play(precision):
+        arrivals = [(float(t), text, ov, precision) for t, (text, ov) in zip(t_arr, jobs)]
+        srv = Server(int(budget_gb * 2 ** 30), slots=slots, chunk=chunk)
+        t0 = time.perf_counter()
+        recs = srv.play(arrivals)
+        span = time.perf_counter() - t0
+        phases = srv.stats()
+        srv.close()
+        return recs, span, phases
+
+    # the same arrivals served in FP64: the batching and serving gain without the precision gain
+    recs64, span64, _ = play("fp64")
+    recs, span, phases = play("mixed")
     tts = np.array([r["tts"] for r in recs])
+    tts64 = np.array([r["tts"] for r in recs64])
     return {"jobs": njobs, "sequential_fp64_runs_per_s": seq_rate, "sequential_s": seq_s,
             "served_runs_per_s": njobs / span, "served_makespan_s": span,
             "speedup_vs_sequential_fp64": (njobs / span) / seq_rate,
+            "served_fp64_runs_per_s": njobs / span64,
+            "served_fp64_speedup_vs_sequential_fp64": (njobs / span64) / seq_rate,
+            "served_fp64_tts_p50_s": float(np.percentile(tts64, 50)),
+            "served_fp64_tts_p99_s": float(np.percentile(tts64, 99)),
             "arrival_rate_per_s": rate_factor * seq_rate,
             "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
             "fell_back": int(sum(r["fell_back"] for r in recs)),
@@ -306,16 +359,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6, help="RK3 steps of the CPU baseline sample (~10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision side measurement")
-    ap.add_argument("--ensemble-runs", type=int, default=64,
+    ap.add_argument("--ensemble-runs", type=int, default=512,
                     help="runs of the batched Orszag-Tang ensemble measured beside the headline (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=100,
                     help="jobs of the served stream measured beside the headline (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0,
                     help="Poisson arrival rate of the served stream, in multiples of the sequential FP64 rate")
     ap.add_argument("--serve-chunk", type=int, default=8, help="steps per serving-loop iteration")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a plain device copy of 2 GiB (5 times): the reference reading for the "
-                         "FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--no-stream-probe", action="store_true",
+                    help="skip the HBM streaming probe (chasqui_amd/_lib/chq_stream_probe) reported beside the roofline")
     ap.add_argument("--only", choices=["batch", "serve"], default=None,
                     help="print only the batched-ensemble or served-stream measurement")
     args = ap.parse_args()
@@ -324,15 +376,13 @@ def main():
               "sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    if args.copy_calibration:
-        import torch
-        src = torch.ones(2 ** 28, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "launches": 5}))
-        return
+    stream = None
+    if not args.no_stream_probe and args.only is None and os.path.exists(STREAM_PROBE):
+        # child process, before this process touches the GPU
+        import subprocess
+        r = subprocess.run([STREAM_PROBE], capture_output=True, text=True, timeout=120)
+        if r.returncode == 0:
+            stream = json.loads(r.stdout)
     if args.only == "batch":
         print(json.dumps({p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}))
         return
@@ -341,17 +391,19 @@ def main():
         return
     n = args.n
     fstate = {} if not args.no_mixed else None
-    ms, flux_ms, flux_launches, st, breakdown = run(n, "fp64", args.steps, args.warmup, breakdown=True, state=fstate)
+    ms, st, breakdown = run(n, "fp64", args.steps, args.warmup, breakdown=True, state=fstate)
     zone_updates = float(n) ** 3 * args.steps
     value = zone_updates / (ms * 1e-3)
-    avg_flux_s = flux_ms * 1e-3 / max(flux_launches, 1)
+    flux = breakdown["flux"]
+    avg_flux_s = flux["total_ms"] * 1e-3 / max(flux["launches"], 1)
     flux_bytes = FLUX_DOUBLES_PER_CELL_STAGE / 3 * 8 * float(n) ** 3
     achieved = flux_bytes / avg_flux_s
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK,
-                "traffic": FLUX_TRAFFIC_PER_LAUNCH,
+                "traffic": profile_traffic(TRAFFIC_PROFILE, FLUX_KERNELS) if n == 256 else None,
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, HERE),
                 "kernel": "k_flux_x / k_flux_m (face flux, PPM + flattening + HLLD, one direction per launch)",
-                "bytes_per_launch": flux_bytes, "avg_launch_ms": avg_flux_s * 1e3, "launches": flux_launches}
+                "bytes_per_launch": flux_bytes, "avg_launch_ms": avg_flux_s * 1e3, "launches": flux["launches"]}
     model = value * 3 * MODEL_BYTES_PER_ZONE_STAGE
     out = {
         "metric": METRIC,
@@ -375,10 +427,13 @@ def main():
                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": model / HBM_PEAK},
         "kernels": breakdown,
     }
+    if stream is not None:
+        out["stream_probe"] = {k: v["gb_per_s"] for k, v in stream.items() if isinstance(v, dict)}
+        out["stream_probe"]["unit"] = "GB/s"
     if not args.no_mixed:
         # the same workload with FP32 reconstruction/Riemann/EMF and FP64 con2prim/RK/CT
         mstate = {}
-        mms, mflux, mlaunch, mst, mk = run(n, "mixed", args.steps, args.warmup, breakdown=True, state=mstate)
+        mms, mst, mk = run(n, "mixed", args.steps, args.warmup, breakdown=True, state=mstate)
         mval = zone_updates / (mms * 1e-3)
         # relative L1 distance of the final mixed state from the FP64 one (same step count)
         import numpy as np
@@ -389,7 +444,7 @@ def main():
         del mstate, fstate
         out["mixed"] = {"value": mval, "unit": "zone-updates/s", "dtype": "f32 flux/EMF, f64 state",
                         "ms_per_step": mms / args.steps, "speedup_vs_fp64": mval / value,
-                        "flux_avg_launch_ms": mflux / max(mlaunch, 1),
+                        "flux_avg_launch_ms": mk["flux"]["total_ms"] / max(mk["flux"]["launches"], 1),
                         "fofc_corrections_in_run": mst["fofc_total"], "kernels": mk,
                         "l1_rel_vs_fp64": l1, "l1_cycles": mst["cycle"]}
     if args.ensemble_runs > 0:

@@ -24,7 +24,8 @@ RSOLVER = {"hlle": 0, "hlld": 1, "lhlld": 2}
 INTEGRATOR = {"rk1": 1, "rk2": 2, "rk3": 3, "rk4": 4, "ssprk43": 4}
 BC = {"periodic": 0, "outflow": 1, "reflect": 2}
 PRECISION = {"fp64": 0, "mixed": 1}
-KERNEL_CLASSES = {"flux": 0, "corner": 1, "update": 2, "bc": 3, "c2p": 4, "repair": 5}
+KERNEL_CLASSES = {"flux": 0, "corner": 1, "update": 2, "bc": 3, "c2p": 4, "repair": 5, "all": 64}
+CLASS_NAMES = ["flux", "corner", "update", "bc", "c2p", "repair"]
 
 
 class MeshDesc(C.Structure):
@@ -107,6 +108,7 @@ def lib():
         "chq_stream": (vp, [vp]),
         "chq_kernel_timing": (C.c_int, [vp, C.c_int]),
         "chq_kernel_timing_read": (C.c_int, [vp, dp, ip]),
+        "chq_kernel_timing_breakdown": (C.c_int, [vp, dp, ip]),
         "chq_deck_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
         "chq_deck_open_lazy": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
         "chq_deck_prepare": (C.c_int, [vp]),
@@ -163,6 +165,7 @@ def exported_symbols():
         "chq_fofc_repair", "chq_diffusion_fluxes", "chq_resistive_emf", "chq_diffusion_dt", "chq_limit_fluxes",
         "chq_apply_gravity", "chq_apply_cooling", "chq_stage", "chq_init", "chq_step", "chq_step_timed",
         "chq_get_status", "chq_sync", "chq_stream", "chq_kernel_timing", "chq_kernel_timing_read",
+        "chq_kernel_timing_breakdown",
         "chq_deck_open", "chq_deck_open_lazy", "chq_deck_prepare", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
         "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",
         "chq_write_vtk", "chq_history", "chq_init_resume", "chq_restart_write", "chq_restart_read",

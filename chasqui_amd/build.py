@@ -19,6 +19,7 @@ OUT = os.path.join(HERE, "_lib")
 OBJ = os.path.join(OUT, "obj")
 LIB = os.path.join(OUT, "libchasqui_amd.so")
 EXE = os.path.join(OUT, "chasqui_amd")
+PROBE = os.path.join(OUT, "chq_stream_probe")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -27,7 +28,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-HIP_SOURCES = ["flux.hip", "flux32.hip", "flux_char.hip", "stage.hip", "layout.hip", "diffusion.hip", "srcterms.hip"]
+HIP_SOURCES = ["flux.hip", "flux32.hip", "flux_char.hip", "stage.hip", "layout.hip", "diffusion.hip", "srcterms.hip",
+               "history.hip"]
 EXTRA = {"flux32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
                         "-fgpu-flush-denormals-to-zero"]}
 HOST_SOURCES = ["capi.cpp", "batch.cpp", "server.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]
@@ -62,6 +64,13 @@ def build(verbose=True):
         cmd = [HIPCC] + COMMON + [main, "-o", tmp, "-L" + OUT, "-lchasqui_amd", "-Wl,-rpath,$ORIGIN"]
         subprocess.run(cmd, check=True)
         os.replace(tmp, EXE)
+    # the HBM streaming probe (one double per lane, like the stage kernels): device rate and
+    # memory-counter calibration
+    probe = os.path.join(CSRC, "probe", "stream.cpp")
+    tmp = PROBE + ".tmp"
+    cmd = [HIPCC, "-x", "hip"] + DEVICE + COMMON + [probe, "-o", tmp]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PROBE)
     if verbose:
         print("built", LIB)
     return LIB

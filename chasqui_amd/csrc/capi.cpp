@@ -108,6 +108,15 @@ int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
   return CHQ_OK;
 }
 
+}  // namespace
+
+int chq::ctx_ready(chq_ctx *c) {
+  if (c->batch_stream) HIPCHK(hipStreamSynchronize(c->batch_stream));
+  return sync_wcur(c);
+}
+
+namespace {
+
 // Mesh::Allocate, mesh.hpp:37-48, plus the padded device layout
 void fill_geom(const chq_mesh_desc *m, Geom &g) {
   g = Geom{};
@@ -243,6 +252,7 @@ int chq_destroy(chq_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->arena) (void)hipFree(c->arena);
   if (c->staging) (void)hipFree(c->staging);
+  if (c->hst_work) (void)hipFree(c->hst_work);
   if (c->sts_buf) (void)hipFree(c->sts_buf);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -264,7 +274,7 @@ int chq_geometry(const chq_ctx *c, int *iv, double *dx) {
   return CHQ_OK;
 }
 
-size_t chq_device_bytes(const chq_ctx *c) { return c ? c->arena_bytes + c->staging_bytes : 0; }
+size_t chq_device_bytes(const chq_ctx *c) { return c ? c->arena_bytes + c->staging_bytes + c->hst_bytes : 0; }
 
 size_t chq_bytes_for(const chq_mesh_desc *m, const chq_config *cfg) {
   if (check_cfg(m, cfg)) return 0;
@@ -924,7 +934,8 @@ int chq_step_timed(chq_ctx *c, int nsteps, double *ms) {
 }
 
 int chq_kernel_timing(chq_ctx *c, int kernel_class) {
-  if (!c || kernel_class < -1 || kernel_class >= CHQ_K_NCLASS) return fail(CHQ_EINVAL, "bad kernel class");
+  if (!c || kernel_class < -1 || (kernel_class >= CHQ_K_NCLASS && kernel_class != CHQ_K_ALL))
+    return fail(CHQ_EINVAL, "bad kernel class");
   HIPCHK(hipStreamSynchronize(c->stream));
   c->timing_class = kernel_class;
   c->tev_used = 0;
@@ -944,6 +955,20 @@ int chq_kernel_timing_read(chq_ctx *c, double *total_ms, int *launches) {
   }
   *total_ms = tot;
   *launches = n;
+  c->tev_used = 0;
+  return CHQ_OK;
+}
+
+int chq_kernel_timing_breakdown(chq_ctx *c, double *ms, int *launches) {
+  if (!c || !ms || !launches) return fail(CHQ_EINVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < CHQ_K_NCLASS; ++q) { ms[q] = 0.0; launches[q] = 0; }
+  for (size_t q = 0; q + 1 < c->tev_used; q += 2) {
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, c->tev[q], c->tev[q + 1]));
+    const int cls = c->tcls[q / 2];
+    if (cls >= 0 && cls < CHQ_K_NCLASS) { ms[cls] += f; launches[cls] += 1; }
+  }
   c->tev_used = 0;
   return CHQ_OK;
 }

@@ -18,6 +18,9 @@ int fail(int code, const std::string &msg);
     if (e_ != hipSuccess) return fail(CHQ_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// history columns (output.hpp:91-119): 8 volume sums, max |div B| (slot HST_MAX), sum |div B|
+constexpr int HST_NV = 10, HST_MAX = 8;
+
 // RK coefficient table (g0, g1, beta, cur, out), main.cpp:250-268
 struct Coef { double g0, g1, beta; int cur, out; };
 inline Coef rk_coef(int nstage, int s) {
@@ -33,6 +36,15 @@ inline Coef rk_coef(int nstage, int s) {
   if (s == 3) return {2.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 1, 1};
   return {0, 1, 0.5, 1, 0};
 }
+}  // namespace chq
+
+namespace chq {
+// waits for a batch advancing the context and refreshes the current primitive set (host reads)
+int ctx_ready(chq_ctx *c);
+// history reduction (history.hip): blocks of the first pass, and the passes over set `buf`
+// into `work` (history_blocks(g) partials + HST_NV results), scaled into out[HST_NV]
+int history_blocks(const Geom &g);
+int history_on(chq_ctx *c, int buf, double *work, double out[HST_NV]);
 }  // namespace chq
 
 using namespace chq;
@@ -64,6 +76,8 @@ struct chq_ctx {
   Ctrl *ctrl_api = nullptr;  // per-kernel entry points (always active)
   void *staging = nullptr;   // device staging buffer for layout conversion
   size_t staging_bytes = 0;
+  double *hst_work = nullptr;   // history partials and results (chq_history)
+  size_t hst_bytes = 0;
   int wcur = 0;              // which primitive set (w, bcc, ecc) is current
   bool wcur_stale = false;   // the time loop may have stopped early: the device copy (Ctrl::wcur) rules
   void *batch = nullptr;     // the batch advancing this context, if any
@@ -76,18 +90,25 @@ struct chq_ctx {
   long long sts_stages = 0;  // RKL2 stages taken so far
   double x1min = 0.0, x2min = 0.0, x3min = 0.0;   // domain origin (point-mass gravity)
   // per-kernel-class timing: event pairs recorded around launches of one class
-  int timing_class = -1;
+  int timing_class = -1;          // a chq_kernel_class, CHQ_K_ALL, or -1 (off)
   std::vector<hipEvent_t> tev;   // begin/end pairs
+  std::vector<int> tcls;         // class of each pair
   size_t tev_used = 0;
 
   void tmark(int cls, bool begin) {
-    if (cls != timing_class) return;
+    if (cls != timing_class && timing_class != CHQ_K_ALL) return;
     if (tev_used == tev.size()) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) return;
       tev.push_back(e);
     }
     if (!begin && (tev_used % 2) == 0) return;
+    if (begin && (tev_used % 2) == 1) return;   // unmatched begin: keep the open pair
+    if (begin) {
+      const size_t pair = tev_used / 2;
+      if (tcls.size() <= pair) tcls.resize(pair + 1);
+      tcls[pair] = cls;
+    }
     (void)hipEventRecord(tev[tev_used++], stream);
   }
 

@@ -1,5 +1,6 @@
-// Outputs: legacy-VTK dumps in the PLUTO layout (output.hpp:14-85) and the history sums of
-// the .hst file (output.hpp:87-119), from state downloaded off the device.
+// Outputs: legacy-VTK dumps in the PLUTO layout (output.hpp:14-85) and the restart files
+// (restart.hpp), from state downloaded off the device. The history sums of the .hst file are
+// reduced on the device (history.hip).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -114,53 +115,6 @@ int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, dou
   dump("Bx2", b, 1);
   dump("Bx3", b, 2);
   return f ? CHQ_OK : chq::fail(CHQ_EINVAL, std::string("write failed: ") + path);
-}
-
-// History (output.hpp:91-119): mass, mom1..3, etot, ekin, emag, eint (volume integrals over
-// the active cells), max |div B| and its mean, from u and face B of set `buf` and the
-// current primitives.
-int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]) {
-  if (!ctx || !mesh || !out || (buf & ~1)) return chq::fail(CHQ_EINVAL, "chq_history: null argument or bad buffer index");
-  const Geo g = geo_of(ctx);
-  std::vector<double> u(chq_array_count(ctx, CHQ_U0)), w(chq_array_count(ctx, CHQ_W)),
-      bc(chq_array_count(ctx, CHQ_BCC)), b1(chq_array_count(ctx, CHQ_B1F0)), b2(b1.size()), b3(b1.size());
-  int rc = chq_download(ctx, buf ? CHQ_U1 : CHQ_U0, u.data(), u.size());
-  if (!rc) rc = chq_download(ctx, CHQ_W, w.data(), w.size());
-  if (!rc) rc = chq_download(ctx, CHQ_BCC, bc.data(), bc.size());
-  if (!rc) rc = chq_download(ctx, buf ? CHQ_B1F1 : CHQ_B1F0, b1.data(), b1.size());
-  if (!rc) rc = chq_download(ctx, buf ? CHQ_B2F1 : CHQ_B2F0, b2.data(), b2.size());
-  if (!rc) rc = chq_download(ctx, buf ? CHQ_B3F1 : CHQ_B3F0, b3.data(), b3.size());
-  if (rc) return rc;
-  const double dx1 = (mesh->x1max - mesh->x1min) / g.nx1, dx2 = (mesh->x2max - mesh->x2min) / g.nx2,
-               dx3 = (mesh->x3max - mesh->x3min) / g.nx3;
-  const double idx1 = 1.0 / dx1, idx2 = 1.0 / dx2, idx3 = 1.0 / dx3;
-  const size_t nc = size_t(g.ni) * g.nj * g.nk;
-  auto C = [&](int k, int j, int i) { return (size_t(k) * g.nj + j) * g.ni + i; };
-  auto F = [&](int k, int j, int i) { return (size_t(k) * (g.nj + 1) + j) * (g.ni + 1) + i; };
-  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, dbmax = 0.0, dbl1 = 0.0;
-  const int ke = g.three_d ? g.ks + g.nx3 - 1 : 0, je = g.two_d ? g.js + g.nx2 - 1 : 0, ie = g.is + g.nx1 - 1;
-  for (int k = g.ks; k <= ke; ++k)
-    for (int j = g.js; j <= je; ++j)
-      for (int i = g.is; i <= ie; ++i) {
-        const size_t c = C(k, j, i);
-        for (int n = 0; n < 5; ++n) s[n] += u[n * nc + c];
-        const double d = w[c];
-        const double v2 = w[nc + c] * w[nc + c] + w[2 * nc + c] * w[2 * nc + c] + w[3 * nc + c] * w[3 * nc + c];
-        const double bb = bc[c] * bc[c] + bc[nc + c] * bc[nc + c] + bc[2 * nc + c] * bc[2 * nc + c];
-        s[5] += 0.5 * d * v2;
-        s[6] += 0.5 * bb;
-        s[7] += u[4 * nc + c] - 0.5 * d * v2 - 0.5 * bb;
-        double db = (b1[F(k, j, i + 1)] - b1[F(k, j, i)]) * idx1;
-        if (g.two_d) db += (b2[F(k, j + 1, i)] - b2[F(k, j, i)]) * idx2;
-        if (g.three_d) db += (b3[F(k + 1, j, i)] - b3[F(k, j, i)]) * idx3;
-        dbmax = std::fmax(dbmax, std::fabs(db));
-        dbl1 += std::fabs(db);
-      }
-  const double vol = dx1 * (g.two_d ? dx2 : 1.0) * (g.three_d ? dx3 : 1.0);
-  for (int n = 0; n < 8; ++n) out[n] = s[n] * vol;
-  out[8] = dbmax;
-  out[9] = dbl1 / (double(g.nx1) * g.nx2 * g.nx3);
-  return CHQ_OK;
 }
 
 // restart::Write (restart.hpp:19-34): header, then u0 and b1f0..b3f0 in the reference layout

@@ -190,8 +190,8 @@ class Mesh:
         return ms.value
 
     def kernel_timing(self, kernel_class):
-        """Select a kernel class ('flux', 'corner', 'update', 'bc', 'c2p', 'repair', or None)
-        whose launches inside step() are bracketed by HIP events."""
+        """Select a kernel class ('flux', 'corner', 'update', 'bc', 'c2p', 'repair' (the whole
+        fofc branch), 'all', or None) whose launches inside step() are bracketed by HIP events."""
         code = -1 if kernel_class is None else _capi.KERNEL_CLASSES[kernel_class]
         check(self.L.chq_kernel_timing(self.h, code), "chq_kernel_timing")
 
@@ -199,6 +199,21 @@ class Mesh:
         ms, n = C.c_double(0.0), C.c_int(0)
         check(self.L.chq_kernel_timing_read(self.h, C.byref(ms), C.byref(n)), "chq_kernel_timing_read")
         return ms.value, n.value
+
+    def history(self, buf=0):
+        """The .hst columns of set `buf` (output.hpp:91-119), reduced on the device: mass, mom1,
+        mom2, mom3, etot, ekin, emag, eint, divB_max, divB_L1."""
+        out = (C.c_double * 10)()
+        desc = _capi.MeshDesc()   # the geometry comes from the context
+        check(self.L.chq_history(self.h, C.byref(desc), int(buf), out), "chq_history")
+        return list(out)
+
+    def kernel_timing_breakdown(self):
+        """{class: (ms, launches)} since kernel_timing('all') (then starts over)."""
+        n = len(_capi.CLASS_NAMES)
+        ms, k = (C.c_double * n)(), (C.c_int * n)()
+        check(self.L.chq_kernel_timing_breakdown(self.h, ms, k), "chq_kernel_timing_breakdown")
+        return {c: (ms[q], k[q]) for q, c in enumerate(_capi.CLASS_NAMES)}
 
     def status(self):
         s = _capi.Status()

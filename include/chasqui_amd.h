@@ -171,12 +171,19 @@ int chq_sync(chq_ctx *ctx);
 /* Per-kernel timing of chq_step (no reference counterpart: measurement support). When a
  * class is selected, HIP events are recorded on the context stream around each launch of
  * that class; chq_kernel_timing_read synchronises and returns the summed duration and the
- * number of launches since selection (then starts over). kernel_class -1 turns it off. */
+ * number of launches since selection (then starts over). kernel_class -1 turns it off;
+ * CHQ_K_ALL brackets every class, read per class with chq_kernel_timing_breakdown.
+ * CHQ_K_REPAIR brackets the whole fofc branch of a stage (repair, edge EMFs, re-update,
+ * boundaries and con2prim of the repaired state; its launches exit at once when nothing was
+ * flagged). */
 enum chq_kernel_class {
-  CHQ_K_FLUX = 0, CHQ_K_CORNER, CHQ_K_UPDATE, CHQ_K_BC, CHQ_K_C2P, CHQ_K_REPAIR, CHQ_K_NCLASS
+  CHQ_K_FLUX = 0, CHQ_K_CORNER, CHQ_K_UPDATE, CHQ_K_BC, CHQ_K_C2P, CHQ_K_REPAIR, CHQ_K_NCLASS,
+  CHQ_K_ALL = 64
 };
 int chq_kernel_timing(chq_ctx *ctx, int kernel_class);
 int chq_kernel_timing_read(chq_ctx *ctx, double *total_ms, int *launches);
+/* per-class totals since selection (CHQ_K_ALL): ms[CHQ_K_NCLASS], launches[CHQ_K_NCLASS] */
+int chq_kernel_timing_breakdown(chq_ctx *ctx, double *ms, int *launches);
 void *chq_stream(chq_ctx *ctx);
 
 /* ---- host layer: decks, problem setups, outputs (host-only, no device needed) ---- */

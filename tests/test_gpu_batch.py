@@ -230,3 +230,61 @@ def test_batch_seeded_schemes_equal_solo(scheme, prec, runs, steps):
     b.close()
     for m in meshes:
         m.close()
+
+
+def _ensemble_runs(n, seed, sizes):
+    """Orszag-Tang ensemble members: size from `sizes`, per-run seeded +-5% perturbation of the
+    pressure and of the field amplitude (problem.pres / problem.b0 around pgen.hpp:94-111's
+    5/(12 pi) and 1/sqrt(4 pi))."""
+    rng = np.random.default_rng(seed)
+    p0, b0 = 5.0 / (12.0 * np.pi), 1.0 / np.sqrt(4.0 * np.pi)
+    out = []
+    for _ in range(n):
+        s = int(rng.choice(sizes))
+        fp, fb = 1.0 + 0.05 * rng.uniform(-1, 1), 1.0 + 0.05 * rng.uniform(-1, 1)
+        out.append([f"mesh.nx1={s}", f"mesh.nx2={s}", f"problem.pres={float(p0 * fp)!r}",
+                    f"problem.b0={float(b0 * fb)!r}"])
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_orszag_tang_ensemble_config(prec):
+    """The batched-ensemble workload's own scheme, PPMX + HLLD + RK3 on Orszag-Tang with sizes
+    from its set {64, 96, 128}^2 (the smaller end, so the solo runs finish in seconds) and the
+    per-run +-5% pressure / field amplitude: every member bitwise its solo run, and the
+    perturbations really differ between members."""
+    from chasqui_amd import Batch
+    from chasqui_amd.deck import Deck
+    scheme = ["mhd.reconstruct=ppmx", "mhd.rsolver=hlld", "time.integrator=rk3", "time.tlim=1e9", "time.nlim=-1"]
+    runs = _ensemble_runs(6, 3, [64, 96, 128])
+    steps = 8
+    print(f"case OT ensemble {prec} {runs}", flush=True)
+
+    def mk(ov):
+        d = Deck(deck("orszag_tang.in"), scheme + ov + [f"mhd.precision={prec}"])
+        m = d.make_mesh()
+        d.close()
+        return m
+
+    solo = []
+    for ov in runs:
+        m = mk(ov)
+        m.step(steps)
+        solo.append(_state(m))
+        m.close()
+    assert len({r[2] for r in runs}) == len(runs) and len({r[3] for r in runs}) == len(runs)
+    assert len({int(r[0].split("=")[1]) for r in runs}) >= 2
+    b = Batch(8)
+    meshes = [mk(ov) for ov in runs]
+    for m in meshes:
+        b.add(m)
+    b.step(2)
+    b.tune(reps=1)
+    b.step(steps - 2)
+    b.sync()
+    for i, m in enumerate(meshes):
+        assert solo[i]["status"][1] == steps
+        _same(_state(m), solo[i], f"member {i} {runs[i]}")
+    b.close()
+    for m in meshes:
+        m.close()
