@@ -123,6 +123,7 @@ def lib():
         "chq_deck_upload": (C.c_int, [vp, vp]),
         "chq_write_vtk": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_char_p, C.c_double, C.c_int]),
         "chq_history": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_int, dp]),
+        "chq_history_noalloc": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_int, dp]),
         "chq_batch_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
         "chq_batch_destroy": (C.c_int, [vp]),
         "chq_batch_add": (C.c_int, [vp, vp]),
@@ -168,7 +169,7 @@ def exported_symbols():
         "chq_kernel_timing_breakdown",
         "chq_deck_open", "chq_deck_open_lazy", "chq_deck_prepare", "chq_deck_close", "chq_deck_last_error", "chq_deck_mesh", "chq_deck_config",
         "chq_deck_run", "chq_deck_get", "chq_deck_initial", "chq_deck_print", "chq_deck_upload",
-        "chq_write_vtk", "chq_history", "chq_init_resume", "chq_restart_write", "chq_restart_read",
+        "chq_write_vtk", "chq_history", "chq_history_noalloc", "chq_init_resume", "chq_restart_write", "chq_restart_read",
         "chq_batch_create", "chq_batch_destroy", "chq_batch_add", "chq_batch_remove", "chq_batch_size",
         "chq_batch_run", "chq_batch_step", "chq_batch_step_timed", "chq_batch_sync", "chq_batch_unfinished",
         "chq_batch_use_graph", "chq_batch_tune", "chq_batch_grid", "chq_batch_set_grid",

@@ -203,6 +203,7 @@ int chq_create(const chq_mesh_desc *m, const chq_config *cfg, chq_ctx **out) {
   auto cleanup = [&](int code) { chq_destroy(c); return code; };
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) return cleanup(fail(CHQ_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
+  if (int rs = history_scratch(c->device, &c->hst_scratch)) return cleanup(rs);
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
     return cleanup(fail(CHQ_EHIP, std::string("stream/event: ") + hipGetErrorString(e)));

@@ -44,6 +44,7 @@ int ctx_ready(chq_ctx *c);
 // history reduction (history.hip): blocks of the first pass, and the passes over set `buf`
 // into `work` (history_blocks(g) partials + HST_NV results), scaled into out[HST_NV]
 int history_blocks(const Geom &g);
+int history_scratch(int dev, double **ptr);
 int history_on(chq_ctx *c, int buf, double *work, double out[HST_NV]);
 }  // namespace chq
 
@@ -78,6 +79,7 @@ struct chq_ctx {
   size_t staging_bytes = 0;
   double *hst_work = nullptr;   // history partials and results (chq_history)
   size_t hst_bytes = 0;
+  double *hst_scratch = nullptr;   // this device's static history scratch (chq_history_noalloc)
   int wcur = 0;              // which primitive set (w, bcc, ecc) is current
   bool wcur_stale = false;   // the time loop may have stopped early: the device copy (Ctrl::wcur) rules
   void *batch = nullptr;     // the batch advancing this context, if any

@@ -10,15 +10,30 @@
 // the ten results (80 bytes) cross PCIe; the host applies the cell volume and the cell count.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "ctx.hpp"
 
 namespace chq {
+
+// The partials and results of chq_history_noalloc: a file-scope device array, one copy per GPU
+// (320 KiB, held until the process ends), shared by every call on that GPU. Host code reaches
+// it only through the address hipGetSymbolAddress returns on each device (history_scratch);
+// its name in host code is a host-side stand-in, never a device address.
+static __device__ __align__(256) double scratch[(4096 + 1) * HST_NV];
+
 namespace {
 
 constexpr int HB = 256;        // threads per block
 constexpr int HST_ROWS = 32;   // active (k, j) rows per block of the first pass
+// chq_history_noalloc's limit: 4096 first-pass blocks, i.e. 131072 active (k, j) rows (a 256^3
+// mesh has 65536, a 1024^2 one 1024)
+constexpr int HST_MAX_BLOCKS = 4096;
+static_assert(sizeof(scratch) == (HST_MAX_BLOCKS + 1) * HST_NV * sizeof(double), "scratch size");
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};   // device address of `scratch`, per device ordinal
 
 template <class T>
 struct HstArgs {
@@ -105,6 +120,27 @@ __global__ void __launch_bounds__(HB) k_history_final(const double *part, int nb
 
 }  // namespace
 
+// Device address of the static scratch on device `dev`, looked up once per device with that
+// device current (chq_create calls it).
+int history_scratch(int dev, double **ptr) {
+  *ptr = nullptr;
+  if (dev < 0 || dev >= 64) return fail(CHQ_EINVAL, "device ordinal out of range for the history scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess)
+      return fail(CHQ_EHIP, std::string("hipGetSymbolAddress(history scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = q;
+  }
+  *ptr = static_cast<double *>(g_scratch_ptr[dev]);
+  return CHQ_OK;
+}
+
 int history_blocks(const Geom &g) {
   const int rows = (g.two_d ? g.nx2 : 1) * (g.three_d ? g.nx3 : 1);
   return (rows + HST_ROWS - 1) / HST_ROWS;
@@ -165,6 +201,20 @@ int chq_history(chq_ctx *c, const chq_mesh_desc *mesh, int buf, double out[10]) 
     c->hst_bytes = need;
   }
   return chq::history_on(c, buf, c->hst_work, out);
+}
+
+// chq_history without any allocation: the partials go to the static per-device scratch.
+int chq_history_noalloc(chq_ctx *c, const chq_mesh_desc *mesh, int buf, double out[10]) {
+  if (!c || !mesh || !out || (buf & ~1))
+    return chq::fail(CHQ_EINVAL, "chq_history_noalloc: null argument or bad buffer index");
+  const int nb = chq::history_blocks(c->g);
+  if (nb > chq::HST_MAX_BLOCKS)
+    return chq::fail(CHQ_EINVAL, "chq_history_noalloc: the mesh has " + std::to_string(nb * chq::HST_ROWS) +
+                                     " or more active (k, j) rows; the static scratch holds " +
+                                     std::to_string(chq::HST_MAX_BLOCKS * chq::HST_ROWS) + " (use chq_history)");
+  if (!c->hst_scratch) return chq::fail(CHQ_EHIP, "chq_history_noalloc: no history scratch on this device");
+  if (int rc = chq::ctx_ready(c)) return rc;
+  return chq::history_on(c, buf, c->hst_scratch, out);
 }
 
 }  // extern "C"

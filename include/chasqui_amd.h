@@ -219,6 +219,14 @@ int chq_write_vtk(chq_ctx *ctx, const chq_mesh_desc *mesh, const char *path, dou
 /* output::History (output.hpp:91-119) of set `buf`: out[10] = mass mom1 mom2 mom3 etot ekin
  * emag eint divB_max divB_L1 */
 int chq_history(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]);
+/* chq_history for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); no counterpart in the reference. The
+ * same kernels and the same bits as chq_history, with the partial sums in a static per-device
+ * scratch (320 KiB, resolved by chq_create) instead of a buffer the library allocates: it calls
+ * no hipMalloc / hipFree. Meshes of more than 131072 active (k, j) rows (nx2 * nx3) are refused
+ * with CHQ_EINVAL. All calls on one GPU share the scratch: calls that overlap on one GPU (two
+ * contexts on different streams or threads) race and must be serialised by the caller. */
+int chq_history_noalloc(chq_ctx *ctx, const chq_mesh_desc *mesh, int buf, double out[10]);
 /* restart::Write / restart::Read (restart.hpp:19-58): the reference's binary restart format
  * (header + u0 + b1f0..b3f0, FP64), readable by either code. After chq_restart_read, call
  * chq_init_resume with the returned time and cycle. */
