@@ -746,7 +746,18 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
       HIPCHK(launch_flag(fl, c->stream));
     }
   }
-  const int first = strict ? 2 : 0;
+  // Otherwise the order adapts on the device (ctrl_begin_stage): a stage whose counterpart in
+  // the previous step flagged cells flags here, before the fill (gate 3), and skips its first
+  // fill and con2prim when it flags again (gate 4); any other stage flags inside con2prim
+  // (flag mode 3).
+  const bool adaptive = c->cfg.fofc && !strict;
+  if (adaptive) {
+    C2pArgs fl = c->c2p_args(cf.out, wc, k);   // reads the conserved state and face B only
+    fl.flag_mode = 2;
+    fl.only_if_bad = 3;
+    HIPCHK(launch_flag(fl, c->stream));
+  }
+  const int first = strict ? 2 : adaptive ? 4 : 0;
   c->tmark(CHQ_K_BC, true);
   HIPCHK(c->boundaries(cf.out, nvar, k, first));
   c->tmark(CHQ_K_BC, false);
@@ -756,7 +767,7 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
   ca.compute_dt = last && !(c->cfg.sts || c->cfg.cooling);   // split steps take dt after the splits
   ca.slot = 0;
-  ca.flag_mode = c->cfg.fofc && !strict ? 2 : 0;
+  ca.flag_mode = adaptive ? 3 : 0;
   c->tmark(CHQ_K_C2P, true);
   HIPCHK(launch_c2p<T>(ca, c->stream));
   c->tmark(CHQ_K_C2P, false);
@@ -813,7 +824,7 @@ static int enqueue_step(chq_ctx *c) {
   for (int s = 1; s <= nst; ++s) {
     const Coef cf = rk_coef(nst, s);
     const int last = s == nst;
-    HIPCHK(launch_ctrl_begin_stage(k, last, c->stream));
+    HIPCHK(launch_ctrl_begin_stage(k, last, c->stream, s - 1));
     if (fused) {
       const int rc2 = c->mixed ? enqueue_fused_stage<float>(c, c->f32, cf, last)
                                : enqueue_fused_stage<double>(c, c->f64, cf, last);

@@ -53,6 +53,9 @@ struct Ctrl {
   long long fofc_total;
   unsigned long long rho_bits;   // minimum active density of the last con2prim (parabolic dt)
   long long limiter_total;       // positivity flux-limiter activations
+  int early;                     // this stage flags before the boundary fill
+  int stage;                     // RK stage index of the time loop (-1: a single-kernel entry point)
+  int fired;                     // bit s: stage s of the last step flagged cells
 };
 
 inline __host__ __device__ int div_up(int a, int b) { return (a + b - 1) / b; }

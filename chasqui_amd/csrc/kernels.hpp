@@ -278,7 +278,8 @@ template <class T> hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s);
 hipError_t launch_flag(const C2pArgs &a, hipStream_t s);   // standalone FlagBadCells
 // control-block kernels
 hipError_t launch_ctrl_begin_step(Ctrl *c, hipStream_t s);
-hipError_t launch_ctrl_begin_stage(Ctrl *c, int reset_slots, hipStream_t s);
+// stage: RK stage index of the time loop (selects the adaptive fofc order), -1 otherwise
+hipError_t launch_ctrl_begin_stage(Ctrl *c, int reset_slots, hipStream_t s, int stage = -1);
 hipError_t launch_ctrl_end_stage(Ctrl *c, int last, int fused, hipStream_t s);
 hipError_t launch_ctrl_end_step(Ctrl *c, hipStream_t s);
 

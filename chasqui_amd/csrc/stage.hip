@@ -20,11 +20,15 @@ namespace {
 constexpr int BX = 64, BY = 4;
 
 // only_if_bad: 0 always, 1 only when the stage flagged cells (the fofc branch), 2 only when it
-// flagged none (the fused stage's first boundary fill and con2prim when the flux limiter is on)
+// flagged none (the fused stage's first boundary fill and con2prim when the flux limiter is on),
+// 3 only when the stage flags early (the flag pass before the fill), 4 unless it flagged early
+// and found cells (the first fill and con2prim of the adaptive order)
 __device__ __forceinline__ bool gate(const Ctrl *c, int only_if_bad) {
   if (!c->active) return false;
   if (only_if_bad == 1 && c->nbad == 0) return false;
   if (only_if_bad == 2 && c->nbad != 0) return false;
+  if (only_if_bad == 3 && !c->early) return false;
+  if (only_if_bad == 4 && c->early && c->nbad != 0) return false;
   return true;
 }
 
@@ -473,7 +477,9 @@ __device__ __forceinline__ void c2p_core(const C2pArgsT<T> &a, int k, int j, int
   double *u = a.u + c;
   const double u0 = u[IDN * cs], u1 = u[IM1 * cs], u2 = u[IM2 * cs], u3 = u[IM3 * cs], u4 = u[IEN * cs];
   const bool active = i >= g.is && i <= g.ie && j >= g.js && j <= g.je && k >= g.ks && k <= g.ke;
-  if (a.flag_mode && active) {   // fofc.hpp:24-40 on the tentative state
+  // flag_mode 3: append (as 2) unless this stage already flagged before the fill
+  const int fm = a.flag_mode == 3 ? (a.ctrl->early ? 0 : 2) : a.flag_mode;
+  if (fm && active) {   // fofc.hpp:24-40 on the tentative state
     bool bad = u0 <= e.dfloor;
     if (!bad) {
       const double kin = 0.5 * (sq(u1) + sq(u2) + sq(u3)) / u0;
@@ -482,7 +488,7 @@ __device__ __forceinline__ void c2p_core(const C2pArgsT<T> &a, int k, int j, int
       if (DE && e.dual_energy && ei < e.de_eta * fabs(u4)) p = (u[IENT * cs] / u0) * pow(u0, e.gamma);
       bad = p <= e.pfloor;
     }
-    if (a.flag_mode == 1) {
+    if (fm == 1) {
       a.flag[c] = bad ? 1 : 0;
       nbad_local += bad ? 1 : 0;
     } else if (bad) {
@@ -595,7 +601,7 @@ __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
 // standalone FlagBadCells over the active cells (fofc.hpp:17-44); count into ctrl->nbad, with
 // the flag array (flag_mode 1 or 0) or appending to the flagged-cell list (flag_mode 2)
 __global__ void __launch_bounds__(BX * BY) k_flag(const C2pArgs a) {
-  if (!gate(a.ctrl, 0)) return;
+  if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
   const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
   const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
@@ -644,7 +650,16 @@ __device__ __forceinline__ void ctrl_begin_step(Ctrl *c) {
   if (c->t + dt > c->tlim) dt = c->tlim - c->t;
   c->dt = dt;
 }
-__device__ __forceinline__ void ctrl_begin_stage(Ctrl *c, int reset_slots) {
+// The adaptive fofc order: an RK stage whose counterpart in the previous step flagged cells
+// flags right after the update, before the boundary fill (the reference's order), so when it
+// fires again it fills and converts once; other stages keep the flag inside con2prim and pay a
+// second fill and con2prim only when they fire. Firing clusters by stage (the first stage's
+// full forward step, mostly), not from one stage to the next. Both orders give the same bits:
+// boundaries never change the active cells the flag reads. stage -1 (the single-kernel entry
+// points) never flags early.
+__device__ __forceinline__ void ctrl_begin_stage(Ctrl *c, int reset_slots, int stage) {
+  c->stage = stage;
+  c->early = stage >= 0 && ((c->fired >> stage) & 1);
   c->nbad = 0;
   if (reset_slots) { c->mdt_bits[0] = bits_of(1e300); c->mdt_bits[1] = bits_of(1e300); }
 }
@@ -656,6 +671,7 @@ __device__ __forceinline__ void ctrl_end_stage(Ctrl *c, int last, int fused) {
   if (fused) c->wcur ^= 1;
   c->fofc_total += c->nbad;
   c->nbad_last = c->nbad;
+  if (c->stage >= 0) c->fired = c->nbad ? (c->fired | (1 << c->stage)) : (c->fired & ~(1 << c->stage));
   if (last) c->min_dt = fmin(from_bits(c->mdt_bits[(fused && c->nbad) ? 1 : 0]), 1e300);
 }
 // main.cpp:291-292
@@ -665,7 +681,7 @@ __device__ __forceinline__ void ctrl_end_step(Ctrl *c) {
   c->cycle += 1;
 }
 __global__ void k_begin_step(Ctrl *c) { ctrl_begin_step(c); }
-__global__ void k_begin_stage(Ctrl *c, int reset_slots) { ctrl_begin_stage(c, reset_slots); }
+__global__ void k_begin_stage(Ctrl *c, int reset_slots, int stage) { ctrl_begin_stage(c, reset_slots, stage); }
 __global__ void k_end_stage(Ctrl *c, int last, int fused) { ctrl_end_stage(c, last, fused); }
 __global__ void k_end_step(Ctrl *c) { ctrl_end_step(c); }
 
@@ -877,7 +893,7 @@ __global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *n
   if (r >= *nruns) return;
   Ctrl *c = R[r].ctrl;
   if (op == CTRL_BEGIN_STEP) ctrl_begin_step(c);
-  else if (op == CTRL_BEGIN_STAGE) ctrl_begin_stage(c, arg);
+  else if (op == CTRL_BEGIN_STAGE) ctrl_begin_stage(c, arg, -1);
   else if (op == CTRL_END_STAGE) ctrl_end_stage(c, arg, 1);
   else {
     if (op == CTRL_END_STEP) ctrl_end_step(c);
@@ -998,8 +1014,8 @@ hipError_t launch_ctrl_begin_step(Ctrl *c, hipStream_t s) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, c);
   return hipGetLastError();
 }
-hipError_t launch_ctrl_begin_stage(Ctrl *c, int reset_slots, hipStream_t s) {
-  hipLaunchKernelGGL(k_begin_stage, dim3(1), dim3(1), 0, s, c, reset_slots);
+hipError_t launch_ctrl_begin_stage(Ctrl *c, int reset_slots, hipStream_t s, int stage) {
+  hipLaunchKernelGGL(k_begin_stage, dim3(1), dim3(1), 0, s, c, reset_slots, stage);
   return hipGetLastError();
 }
 hipError_t launch_ctrl_end_stage(Ctrl *c, int last, int fused, hipStream_t s) {
