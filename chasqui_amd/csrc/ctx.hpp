@@ -246,7 +246,7 @@ struct chq_ctx {
 
   template <class T> FluxArgsT<T> flux_args_t(const FieldSet<T> &f, int d, int cur, int wset, const Ctrl *c) const {
     FluxArgs a0 = flux_args(d, cur, wset, c);
-    FluxArgsT<T> a;
+    FluxArgsT<T> a{};
     a.g = a0.g; a.eos = a0.eos; a.flatten = a0.flatten; a.low_mach = a0.low_mach; a.entropy = a0.entropy;
     a.charac = a0.charac;
     a.lhlld_phi = a0.lhlld_phi;
@@ -273,7 +273,7 @@ struct chq_ctx {
   template <class T> UpdateArgsT<T> update_args_t(const FieldSet<T> &f, int out, double g0, double g1, int nvar,
                                                   const Ctrl *c, int oib) const {
     UpdateArgs a0 = update_args(out, g0, g1, nvar, c, oib);
-    UpdateArgsT<T> a;
+    UpdateArgsT<T> a{};
     a.g = a0.g; a.g0 = g0; a.g1 = g1; a.beta = 0.0; a.beta_dt = 0.0; a.use_ctrl_dt = 0; a.nvar = nvar;
     a.f1 = f.flx[0]; a.f2 = f.flx[1]; a.f3 = f.flx[2];
     a.ua = a0.ua; a.ub = a0.ub; a.uo = a0.uo;
@@ -286,7 +286,7 @@ struct chq_ctx {
   }
   template <class T> C2pArgsT<T> c2p_args_t(const FieldSet<T> &f, int buf, int wset, Ctrl *c) const {
     C2pArgs a0 = c2p_args(buf, wset, c);
-    C2pArgsT<T> a;
+    C2pArgsT<T> a{};
     a.g = a0.g; a.eos = a0.eos; a.b1 = a0.b1; a.b2 = a0.b2; a.b3 = a0.b3; a.u = a0.u;
     a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.ecc = f.ecc[wset];
     a.write_went = 1; a.entropy_out = 1; a.compute_dt = 0; a.slot = 0; a.flag_mode = 0;
