@@ -198,3 +198,33 @@ def test_driver_reflecting_box_mass(tmp_path):
           "output.basename=box", "output.hst_dt=0.4"], tmp_path)
     m = (tmp_path / "box.hst").read_text().strip().split("\n")[-1].split()
     assert abs(float(m[0]) - 0.4) < 1e-12 and abs(float(m[2]) - 1.0) < 1e-9, m
+
+
+# 2D oblique CPAW errors (tests/convergence.sh:14-17 runs, README.md:98), total L1 over the eight
+# fields (the entropy slot left out, as the README's tables): the oracle's values, pinned here at
+# 32^2 and 64^2 bitwise-equivalent runs; PLM / WENO-Z is 8.4x and 9.6x, not the README's 16x
+CPAW2D = ["mesh.x1max=1.4142135623730951", "mesh.x2max=1.4142135623730951", "problem.angle=45.0",
+          "output.basename=cp2"]
+
+
+def test_driver_cpaw_2d_errors_match_the_oracle(tmp_path):
+    """The reference's 2D convergence runs through the driver (its _errors.dat lines) equal the
+    oracle's L1 errors for PLM and WENO-Z at N = 32, 64, 128 (printed with 8 digits); the
+    PLM / WENO-Z ratio is the oracle's at every N (8.4, 9.6, 9.6)."""
+    ratios = []
+    for n in (32, 64, 128):
+        errs = {}
+        for r in ("plm", "wenoz"):
+            _run(["-i", os.path.join(DECKS, "cpaw_1d.in"), f"mesh.nx1={n}", f"mesh.nx2={n}", f"mhd.reconstruct={r}",
+                  "time.ndiag=0"] + CPAW2D, tmp_path)
+            row = [float(x) for x in (tmp_path / "cp2_errors.dat").read_text().splitlines()[-1].split()]
+            dev = row[-1] - row[3 + 5]          # total without the entropy slot
+            o = po.OracleSim(deck("cpaw_1d.in"), [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mhd.reconstruct={r}"] + CPAW2D[:3])
+            o.run()
+            e = o.l1_error()
+            ref = e[9] - e[5]
+            assert abs(dev - ref) <= 1e-7 * ref, (n, r, dev, ref)
+            errs[r] = dev
+        ratios.append(errs["plm"] / errs["wenoz"])
+    print("PLM / WENO-Z 2D ratios at N = 32, 64, 128:", ratios, flush=True)
+    assert [round(x, 1) for x in ratios] == [8.4, 9.6, 9.6]
