@@ -201,8 +201,8 @@ def test_driver_reflecting_box_mass(tmp_path):
 
 
 # 2D oblique CPAW errors (tests/convergence.sh:14-17 runs, README.md:98), total L1 over the eight
-# fields (the entropy slot left out, as the README's tables): the oracle's values, pinned here at
-# 32^2 and 64^2 bitwise-equivalent runs; PLM / WENO-Z is 8.4x and 9.6x, not the README's 16x
+# fields (the entropy slot left out, as the README's tables) against the oracle's at N = 32, 64,
+# 128; PLM / WENO-Z is 8.3x and 9.6x, not the README's 16x
 CPAW2D = ["mesh.x1max=1.4142135623730951", "mesh.x2max=1.4142135623730951", "problem.angle=45.0",
           "output.basename=cp2"]
 
@@ -210,7 +210,7 @@ CPAW2D = ["mesh.x1max=1.4142135623730951", "mesh.x2max=1.4142135623730951", "pro
 def test_driver_cpaw_2d_errors_match_the_oracle(tmp_path):
     """The reference's 2D convergence runs through the driver (its _errors.dat lines) equal the
     oracle's L1 errors for PLM and WENO-Z at N = 32, 64, 128 (printed with 8 digits); the
-    PLM / WENO-Z ratio is the oracle's at every N (8.4, 9.6, 9.6)."""
+    PLM / WENO-Z ratio is the oracle's at every N (8.3, 9.6, 9.6)."""
     ratios = []
     for n in (32, 64, 128):
         errs = {}
@@ -227,4 +227,4 @@ def test_driver_cpaw_2d_errors_match_the_oracle(tmp_path):
             errs[r] = dev
         ratios.append(errs["plm"] / errs["wenoz"])
     print("PLM / WENO-Z 2D ratios at N = 32, 64, 128:", ratios, flush=True)
-    assert [round(x, 1) for x in ratios] == [8.4, 9.6, 9.6]
+    assert [round(x, 1) for x in ratios] == [8.3, 9.6, 9.6]
