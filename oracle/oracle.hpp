@@ -12,7 +12,8 @@
 //
 // Every physics option of the reference's Stage() and time loop is restated.
 // Parity unpinned (no golden vector, known answer or invariant of its own): the fofc flag count
-// of a real run beyond its closed-form cases, and the summation order of the history file.
+// of a real run beyond its closed-form cases. The fofc repair (faces and values) and the aliased
+// whole-domain re-update are pinned by numpy closed forms in tests/test_oracle_fofc_pins.py.
 #pragma once
 #include <cstdint>
 #include <cstdio>
