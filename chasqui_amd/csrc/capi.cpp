@@ -691,6 +691,8 @@ int chq_init_resume(chq_ctx *c, double t, int cycle, double tlim, int nlim) {
   h.active = 0;
   h.cycle = cycle;
   h.cycle0 = cycle;   // time.nlim counts from here (main.cpp:241-243)
+  // the epoch restarts at 0: clear the near-marks of the local fofc re-update it is compared with
+  HIPCHK(launch_fill_zero(reinterpret_cast<double *>(c->near_marks()), c->g.cstride / 2, c->stream));
   h.wcur = c->wcur;
   HIPCHK(hipMemcpyAsync(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -725,9 +727,21 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   c->tmark(CHQ_K_CORNER, true);
   HIPCHK(launch_corner<T>(c->corner_args_t(fs, wc, k, 0), c->stream));
   c->tmark(CHQ_K_CORNER, false);
+  const bool strict = fp64 && c->cfg.fofc && (lim || c->eos.dual_energy);
+  // The first stage of RK2 and RK3 (u0 -> u1 with g1 = 0) re-updates only the points around the
+  // flagged cells when it fires (k_mark_near / k_update_near), unless its first pass counted a
+  // point where that could differ from the whole-domain re-update (FP64; the FP32 field set's
+  // update does not read u1 when g1 = 0). Needs the fast fofc order and no source term after
+  // the update.
+  const bool local = c->cfg.fofc && !strict && !(fp64 && c->gravity()) && c->cfg.nstage <= 3 && cf.out == 1 &&
+                     cf.g1 == 0.0;
   UpdateArgsT<T> ua = c->update_args_t(fs, cf.out, cf.g0, cf.g1, nvar, k, 0);
   ua.use_ctrl_dt = 1;
   ua.beta = cf.beta;
+  if (local && fp64) {
+    ua.zwatch = 1;
+    ua.zcount = &k->zsens;
+  }
   c->tmark(CHQ_K_UPDATE, true);
   HIPCHK(launch_update<T>(ua, c->stream));
   c->tmark(CHQ_K_UPDATE, false);
@@ -738,7 +752,6 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   // tentative state as the update left it (main.cpp:78-91): the limiter can leave the two
   // copies of a periodic shared face unequal (the fill would replace one), and dual-energy
   // con2prim rewrites u[IEN] of switched cells (physics.hpp:463-468).
-  const bool strict = fp64 && c->cfg.fofc && (lim || c->eos.dual_energy);
   if constexpr (fp64) {
     if (strict) {
       C2pArgs fl = c->c2p_args(cf.out, wc, k);
@@ -783,6 +796,11 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
     UpdateArgsT<T> ub = c->update_args_t(fs, cf.out, cf.g0, cf.g1, nvar, k, 1);
     ub.use_ctrl_dt = 1;
     ub.beta = cf.beta;
+    if (local) {
+      HIPCHK(launch_mark_near(c->g, k, c->badlist, c->near_marks(), c->stream));   // gate 5
+      HIPCHK(launch_update_near<T>(ub, c->near_marks(), c->stream));               // gate 5
+      ub.only_if_bad = 6;   // the whole domain when the first pass counted a point
+    }
     HIPCHK(launch_update<T>(ub, c->stream));
     if (fp64 && c->gravity()) {   // main.cpp:89
       SrcArgs sb = c->src_args(cf.out, wc, k, cf.beta, true);

@@ -56,6 +56,8 @@ struct Ctrl {
   int early;                     // this stage flags before the boundary fill
   int stage;                     // RK stage index of the time loop (-1: a single-kernel entry point)
   int fired;                     // bit s: stage s of the last step flagged cells
+  int epoch;                     // stages begun since the loop started (near-marks of the local re-update)
+  int zsens;                     // first-pass points the local fofc re-update cannot skip (zero sign, non-finite)
 };
 
 inline __host__ __device__ int div_up(int a, int b) { return (a + b - 1) / b; }

@@ -72,7 +72,9 @@ struct chq_ctx {
   FieldSet<float> f32;       // mixed precision (only one of the two sets is allocated)
   bool mixed = false;        // primitives, fluxes and EMFs are FP32
   int *flag = nullptr;
-  int *badlist = nullptr;    // flagged cells of the current stage (fused fofc path)
+  int *badlist = nullptr;    // flagged cells of the current stage (fused fofc path); the area
+                             // holds 2 cstride ints: the list, then one near-mark per point
+  int *near_marks() const { return badlist + g.cstride; }
   Ctrl *ctrl = nullptr;      // time loop
   Ctrl *ctrl_api = nullptr;  // per-kernel entry points (always active)
   void *staging = nullptr;   // device staging buffer for layout conversion

@@ -58,6 +58,11 @@ struct UpdateArgsT {
   const T *e1, *e2, *e3;
   const Ctrl *ctrl;
   int only_if_bad;
+  // FP64 first pass of a stage that may re-update locally (output = ub, g1 = 0): count into
+  // *zcount the points whose recomputation could differ from this pass, i.e. where g0 * ua is
+  // -0 and the divergence (curl) term is 0, or where the result is not finite
+  int zwatch;
+  int *zcount;
 };
 using UpdateArgs = UpdateArgsT<double>;
 
@@ -226,6 +231,10 @@ hipError_t launch_repair_list(const FluxArgsT<float> a[3], int ndir, const int *
 template <class T> hipError_t launch_corner(const CornerArgsT<T> &a, hipStream_t s);
 template <class T> hipError_t launch_corner_list(const CornerArgsT<T> &a, const int *list, hipStream_t s);
 template <class T> hipError_t launch_update(const UpdateArgsT<T> &a, hipStream_t s);
+// the local fofc re-update: stamp the points of the update box within one cell of each listed
+// (flagged) cell with the control block's epoch, then recompute exactly the stamped points
+hipError_t launch_mark_near(const Geom &g, const Ctrl *ctrl, const int *list, int *marks, hipStream_t s);
+template <class T> hipError_t launch_update_near(const UpdateArgsT<T> &a, const int *marks, hipStream_t s);
 hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s);
 
 // explicit diffusion (FP64 single-mesh path), diffusion.hip

@@ -22,13 +22,16 @@ constexpr int BX = 64, BY = 4;
 // only_if_bad: 0 always, 1 only when the stage flagged cells (the fofc branch), 2 only when it
 // flagged none (the fused stage's first boundary fill and con2prim when the flux limiter is on),
 // 3 only when the stage flags early (the flag pass before the fill), 4 unless it flagged early
-// and found cells (the first fill and con2prim of the adaptive order)
+// and found cells (the first fill and con2prim of the adaptive order), 5 / 6 when it flagged
+// cells and its first pass counted no / some point the local re-update cannot skip
 __device__ __forceinline__ bool gate(const Ctrl *c, int only_if_bad) {
   if (!c->active) return false;
   if (only_if_bad == 1 && c->nbad == 0) return false;
   if (only_if_bad == 2 && c->nbad != 0) return false;
   if (only_if_bad == 3 && !c->early) return false;
   if (only_if_bad == 4 && c->early && c->nbad != 0) return false;
+  if (only_if_bad == 5 && (c->nbad == 0 || c->zsens != 0)) return false;
+  if (only_if_bad == 6 && (c->nbad == 0 || c->zsens == 0)) return false;
   return true;
 }
 
@@ -160,34 +163,96 @@ __device__ __forceinline__ void update_at(const UpdateArgsT<T> &a, const double 
   auto comb = [&](const double *xa, const double *xb, long long o) {
     return skip_b ? g0 * xa[o] : g0 * xa[o] + g1 * xb[o];
   };
+  // the watch of UpdateArgsT::zwatch (FP64, g1 = 0): a recomputation reads this result as its
+  // ub and adds 0 * (this result), which can only change a result whose g0 * ua is -0 with a
+  // zero divergence (curl) term, or a result that is not finite
+  bool zs = false;
+  auto watch = [&](const double *xa, long long o, double z, double r) {
+    if (!std::is_same<T, double>::value || !a.zwatch) return;
+    const double p = g0 * xa[o];
+    zs = zs || (p == 0.0 && signbit(p) && z == 0.0) || !isfinite(r);
+  };
   if (i <= g.ie && j <= g.je && k <= g.ke) {
     for (int n = 0; n < a.nvar; ++n) {
       const long long o = n * cs + c;
       double dv = (double(a.f1[o + 1]) - double(a.f1[o])) * idx1;
       if (g.two_d) dv += (double(a.f2[o + sj]) - double(a.f2[o])) * idx2;
       if (g.three_d) dv += (double(a.f3[o + sk]) - double(a.f3[o])) * idx3;
-      a.uo[o] = comb(a.ua, a.ub, o) - bdt * dv;
+      const double z = bdt * dv;
+      const double r = comb(a.ua, a.ub, o) - z;
+      a.uo[o] = r;
+      watch(a.ua, o, z, r);
     }
   }
   if (j <= g.je && k <= g.ke) {   // B1 on x1-faces
     double cr = 0.0;
     if (g.two_d) cr += (double(a.e3[c + sj]) - double(a.e3[c])) * idx2;
     if (g.three_d) cr -= (double(a.e2[c + sk]) - double(a.e2[c])) * idx3;
-    a.b1o[c] = comb(a.b1a, a.b1b, c) - bdt * cr;
+    const double z = bdt * cr;
+    const double r = comb(a.b1a, a.b1b, c) - z;
+    a.b1o[c] = r;
+    watch(a.b1a, c, z, r);
   }
   if (i <= g.ie && k <= g.ke && (g.two_d || j == g.js)) {   // B2 on x2-faces
     double cr = -(double(a.e3[c + 1]) - double(a.e3[c])) * idx1;
     if (g.three_d) cr += (double(a.e1[c + sk]) - double(a.e1[c])) * idx3;
-    const double v = comb(a.b2a, a.b2b, c) - bdt * cr;
+    const double z = bdt * cr;
+    const double v = comb(a.b2a, a.b2b, c) - z;
     a.b2o[c] = v;
     if (!g.two_d) a.b2o[c + sj] = v;
+    watch(a.b2a, c, z, v);
   }
   if (i <= g.ie && j <= g.je && (g.three_d || k == g.ks)) {   // B3 on x3-faces
     double cr = (double(a.e2[c + 1]) - double(a.e2[c])) * idx1;
     if (g.two_d) cr -= (double(a.e1[c + sj]) - double(a.e1[c])) * idx2;
-    const double v = comb(a.b3a, a.b3b, c) - bdt * cr;
+    const double z = bdt * cr;
+    const double v = comb(a.b3a, a.b3b, c) - z;
     a.b3o[c] = v;
     if (!g.three_d) a.b3o[c + sk] = v;
+    watch(a.b3a, c, z, v);
+  }
+  if (zs) atomicAdd(a.zcount, 1);
+}
+
+// The local fofc re-update (main.cpp:86-88) of an RK stage u0 -> u1 with g1 = 0. The
+// reference re-updates the whole domain from the repaired fluxes; its aliased input u1 (b?f1)
+// is this stage's first-pass output and enters as 0 * u1, so away from the repaired faces and
+// the edges around them the recomputation reproduces the first pass bit for bit, except where
+// the first pass's watch counted a point (then the whole domain is recomputed instead). The
+// repair changes the fluxes of the flagged cells' faces and the EMFs of their edges, and only
+// the points of the update box within one cell of a flagged cell read those, so:
+//   k_mark_near   - stamps those points (3x3x3 around each listed cell) with the stage's epoch;
+//                   several cells stamp a shared point with the same value
+//   k_update_near - one thread per point of the update box recomputes the stamped ones, each
+//                   once, reading the first-pass value of the aliased buffer, as the reference
+// Stamps of earlier stages hold smaller epochs and are ignored.
+__global__ void __launch_bounds__(256) k_mark_near(const Geom g, const Ctrl *ctl, const int *list, int *marks) {
+  if (!gate(ctl, 5)) return;
+  const int epoch = ctl->epoch;
+  const int n = ctl->nbad * 27;
+  const int ku = g.three_d ? g.ke + 1 : g.ks, ju = g.two_d ? g.je + 1 : g.js;
+  for (int t = int(blockIdx.x) * 256 + int(threadIdx.x); t < n; t += int(gridDim.x) * 256) {
+    const long long r = (long long)list[t / 27] - g.ioff;   // k * plane + j * pitch + i
+    const int o = t % 27;
+    const int k = int(r / g.plane) + o / 9 - 1;
+    const int j = int((r % g.plane) / g.pitch) + (o / 3) % 3 - 1;
+    const int i = int(r % g.pitch) + o % 3 - 1;
+    if (k < g.ks || k > ku || j < g.js || j > ju || i < g.is || i > g.ie + 1) continue;
+    marks[g.at(k, j, i)] = epoch;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_update_near(const UpdateArgsT<T> a, const VB v, const int *marks) {
+  if (!gate(a.ctrl, 5)) return;
+  const Geom &g = a.g;
+  const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
+  const int epoch = a.ctrl->epoch;
+  for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
+    int k, j, i;
+    if (!v.at(vb, k, j, i)) continue;
+    k += g.ks; j += g.js; i += g.is;
+    if (marks[g.at(k, j, i)] == epoch) update_at(a, bdt, k, j, i);
   }
 }
 
@@ -658,6 +723,8 @@ __device__ __forceinline__ void ctrl_begin_step(Ctrl *c) {
 // boundaries never change the active cells the flag reads. stage -1 (the single-kernel entry
 // points) never flags early.
 __device__ __forceinline__ void ctrl_begin_stage(Ctrl *c, int reset_slots, int stage) {
+  c->epoch += 1;
+  c->zsens = 0;
   c->stage = stage;
   c->early = stage >= 0 && ((c->fired >> stage) & 1);
   c->nbad = 0;
@@ -951,6 +1018,20 @@ template hipError_t launch_corner_list<double>(const CornerArgsT<double> &, cons
 template hipError_t launch_corner_list<float>(const CornerArgsT<float> &, const int *, hipStream_t);
 template hipError_t launch_update<double>(const UpdateArgsT<double> &, hipStream_t);
 template hipError_t launch_update<float>(const UpdateArgsT<float> &, hipStream_t);
+
+hipError_t launch_mark_near(const Geom &g, const Ctrl *ctrl, const int *list, int *marks, hipStream_t s) {
+  hipLaunchKernelGGL(k_mark_near, dim3(256), dim3(256), 0, s, g, ctrl, list, marks);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_update_near(const UpdateArgsT<T> &a, const int *marks, hipStream_t s) {
+  const VB v = update_vb(a.g);
+  hipLaunchKernelGGL(k_update_near<T>, dim3(v.n), dim3(256), 0, s, a, v, marks);
+  return hipGetLastError();
+}
+template hipError_t launch_update_near<double>(const UpdateArgsT<double> &, const int *, hipStream_t);
+template hipError_t launch_update_near<float>(const UpdateArgsT<float> &, const int *, hipStream_t);
 
 hipError_t launch_bc(const BcArgs &a, int dir, hipStream_t s) {
   const int nb = int((bc_points(a.g, dir) + 255) / 256);

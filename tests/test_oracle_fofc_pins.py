@@ -333,3 +333,71 @@ def test_fofc_reupdate_reads_the_aliased_first_pass(name, ov, stage):
     bb = b1b.copy()
     (ba if out == 0 else bb)[k, j, i] = b1_first
     assert np.array_equal(b1o[k, j, i], _ct_b1(o, E2, ba, bb, g0, g1, bdt)), (name, stage)
+
+
+@pytest.mark.parametrize("target", [0.03, 0.3])
+@pytest.mark.parametrize("name,ov", CASES)
+def test_first_stage_reupdate_changes_only_points_near_flagged_cells(name, ov, target):
+    """The premise of a re-update limited to the flagged cells' neighbourhoods: in the first RK
+    stage (u0 -> u1 with g1 = 0) the reference's whole-domain re-update after the repair
+    (main.cpp:78-91: RepairFluxes, CornerEMF, Update on the aliased buffers) reproduces the first
+    pass bit for bit, zero signs included, at every point of the update box
+    [is..ie+1] x [js..je+1] x [ks..ke+1] farther than one cell from a flagged active cell. The
+    repair changes the fluxes of the flagged cells' faces and the EMFs of their edges; a point
+    farther away reads neither, and its aliased input u1 enters as 0 * u1."""
+    o = po.OracleSim(deck(name), ov)
+    o.step(2)
+    bdt = 0.3 * o.min_dt
+    dirs = [0] + ([1] if o.two_d else []) + ([2] if o.three_d else [])
+    probe = po.OracleSim(deck(name), ov)
+    for nm in ("u0", "u1", "b1f0", "b2f0", "b3f0", "b1f1", "b2f1", "b3f1", "w", "bcc", "ecc"):
+        probe.view(nm)[...] = o.view(nm)
+    for d in dirs:
+        probe.fluxes(d, 0)
+    probe.corner_emf()
+    probe.update(1, 1.0, 0.0, bdt)
+    _, pfloor = _flag_some(probe, 1, target)
+    probe.close()
+    o.set_floors(1e-10, pfloor)
+    for d in dirs:
+        o.fluxes(d, 0)
+    o.corner_emf()
+    o.update(1, 1.0, 0.0, bdt)
+    names = ["u1", "b1f1", "b2f1", "b3f1"]
+    first = {nm: o.get(nm) for nm in names}
+    assert o.fofc_flag(1) > 0
+    flagged = _active(o) & (o.flag() != 0)
+    for d in dirs:
+        o.fofc_repair(d, 0)
+    o.corner_emf()
+    o.update(1, 1.0, 0.0, bdt)
+    whole = {nm: o.get(nm) for nm in names}
+    # points within one cell of a flagged active cell (a 3x3x3 dilation, clipped to the mesh)
+    near = np.zeros_like(flagged)
+    kk, jj, ii = np.nonzero(flagged)
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                k, j, i = kk + dk, jj + dj, ii + di
+                ok = (k >= 0) & (k < o.nk) & (j >= 0) & (j < o.nj) & (i >= 0) & (i < o.ni)
+                near[k[ok], j[ok], i[ok]] = True
+    ku, ju = (o.ke + 1 if o.three_d else o.ks), (o.je + 1 if o.two_d else o.js)
+    box = np.zeros_like(flagged)
+    box[o.ks:ku + 1, o.js:ju + 1, o.is_:o.ie + 2] = True
+    changed_near = 0
+    for nm in names:
+        a, b = first[nm], whole[nm]
+        a = a.reshape((-1,) + a.shape[-3:])
+        b = b.reshape((-1,) + b.shape[-3:])
+        diff = np.any(a.view(np.int64) != b.view(np.int64), axis=0)   # bit patterns: -0 != +0
+        pts = np.zeros(diff.shape, dtype=bool)    # face arrays: one more entry per dimension
+        pts[:o.nk, :o.nj, :o.ni] = near & box
+        # a point's update also writes the copy of B2 one row up in 1D, of B3 one plane up
+        # below 3D (physics.hpp:339, 345)
+        if nm == "b2f1" and not o.two_d:
+            pts[:, o.js + 1] |= pts[:, o.js]
+        if nm == "b3f1" and not o.three_d:
+            pts[o.ks + 1] |= pts[o.ks]
+        assert not np.any(diff & ~pts), (nm, int(np.count_nonzero(diff & ~pts)))
+        changed_near += int(np.count_nonzero(diff))
+    assert changed_near > 0
