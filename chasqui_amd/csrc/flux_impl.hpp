@@ -23,11 +23,11 @@ namespace chq {
 namespace fluxk {
 
 
-// Riemann solve of one face from its two half-states and store (physics.hpp:167-224).
+// Riemann solve of one face from its two half-states (physics.hpp:167-211) into f[NFLX].
 // c = face index (device layout); sd = stride along DIR.
-template <class T, int DIR, RSolver RS, bool ENT, int NV>
-__device__ __forceinline__ void solve_store(const FluxArgsT<T> &a, long long c, long long sd, const T *Ls,
-                                            const T *Rs_) {
+template <class T, int DIR, RSolver RS, int NV>
+__device__ __forceinline__ void solve_face(const FluxArgsT<T> &a, long long c, long long sd, const T *Ls,
+                                           const T *Rs_, T f[NFLX]) {
   using V = Rsv<T>;
   const Geom &g = a.g;
   const T gam = T(a.eos.gamma), gm1 = T(a.eos.gm1);
@@ -43,7 +43,6 @@ __device__ __forceinline__ void solve_store(const FluxArgsT<T> &a, long long c, 
     L.u = um + ud;
     R.u = um - ud;
   }
-  T f[NFLX];
   if constexpr (RS == RSolver::hlld) {
     V::template hlld<false>(L, R, bn, gam, gm1, f);
   } else if constexpr (RS == RSolver::lhlld) {                     // physics.hpp:182-211
@@ -74,6 +73,16 @@ __device__ __forceinline__ void solve_store(const FluxArgsT<T> &a, long long c, 
   } else {
     V::hlle(L, R, bn, gam, gm1, f);
   }
+}
+
+// Riemann solve of one face and store of its fluxes and face EMFs (physics.hpp:167-224).
+template <class T, int DIR, RSolver RS, bool ENT, int NV>
+__device__ __forceinline__ void solve_store(const FluxArgsT<T> &a, long long c, long long sd, const T *Ls,
+                                            const T *Rs_) {
+  const long long cs = a.g.cstride;
+  constexpr int IU = IVX + DIR, IV = IVX + (DIR + 1) % 3, IW = IVX + (DIR + 2) % 3;
+  T f[NFLX];
+  solve_face<T, DIR, RS, NV>(a, c, sd, Ls, Rs_, f);
   T *fx = a.flx + c;
   fx[IDN * cs] = f[FD];
   fx[IU * cs] = f[FMU];
@@ -103,32 +112,15 @@ __device__ __forceinline__ void var_ptrs(const FluxArgsT<T> &a, const T *p[NV]) 
   if constexpr (NV == 8) p[7] = a.w + IENT * cs;
 }
 
-// ---------------------------------------------------------------- x faces: lane = cell
-// One wave's row segment of x faces (lane = cell).
-template <class T, Recon RC, RSolver RS, bool ENT>
-__device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nseg, const int wave) {
-  constexpr int NV = ENT ? 8 : 7;
+// Lane = cell along the face direction: the two half-states of this lane's cell from the
+// cells of the neighbouring lanes (flattening and the donor fallback included) and, in
+// Lface, the left state of the face below this lane's cell (the lower lane's Lf).
+template <class T, Recon RC, int NV>
+__device__ __forceinline__ void recon_lanes(const T q[NV], const bool flatten, const T dfl, const T pfl, T Lface[NV],
+                                            T Rf[NV]) {
   constexpr bool W = wide_v<RC>;
-  constexpr int H = W ? 3 : 2;            // lanes before the first face: cell halo + 1
-  constexpr int FPW = fpw_v<RC>;
-  const int lane = int(threadIdx.x) & 63;
-  const Geom &g = a.g;
-  // this lane's cell in the rows laid end to end (nseg = cells per row); lanes before the
-  // first or past the last cell read any valid cell and compute no face
-  const int nrow_j = a.ju - a.jl + 1;
-  const long long ncell = (long long)nseg * nrow_j * (a.ku - a.kl + 1);
-  const long long v = (long long)wave * FPW - H + lane;
-  const long long vc = v < 0 ? 0 : (v >= ncell ? ncell - 1 : v);
-  const int row = int(vc / nseg), ci = int(vc - (long long)row * nseg);
-  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
-  const long long row0 = g.at(k, j, 0);
-  const T *src[NV];
-  var_ptrs<T, 0, NV>(a, src);
-  T q[NV];
-#pragma unroll
-  for (int n = 0; n < NV; ++n) q[n] = src[n][row0 + ci];
   using R = Rec<T>;
-  T Lf[NV], Rf[NV];
+  T Lf[NV];
   T fco = T(0);
   if constexpr (RC == Recon::ppm) {
     T d[NV], e[NV];
@@ -162,7 +154,7 @@ __device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nse
     }
   }
   if constexpr (W) {
-    if (a.flatten) {
+    if (flatten) {
       const T p = q[4];
       const T pm2 = __shfl_up(p, 2, 64), pm1 = __shfl_up(p, 1, 64);
       const T pp1 = __shfl_down(p, 1, 64), pp2 = __shfl_down(p, 2, 64);
@@ -170,16 +162,44 @@ __device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nse
       fco = R::flat(pm2, pm1, p, pp1, pp2, um1, up1);
     }
   }
-  finish_cell<T, NV>(Lf, Rf, q, fco, W && a.flatten, T(a.eos.dfloor), T(a.eos.pfloor));
+  finish_cell<T, NV>(Lf, Rf, q, fco, W && flatten, dfl, pfl);
   // face i = cell index of this lane: left state = Lf of the lane below
-  T Lface[NV];
 #pragma unroll
   for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
+}
+
+// ---------------------------------------------------------------- x faces: lane = cell
+// One wave's row segment of x faces (lane = cell).
+template <class T, Recon RC, RSolver RS, bool ENT>
+__device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nseg, const int wave) {
+  constexpr int NV = ENT ? 8 : 7;
+  constexpr bool W = wide_v<RC>;
+  constexpr int H = W ? 3 : 2;            // lanes before the first face: cell halo + 1
+  constexpr int FPW = fpw_v<RC>;
+  const int lane = int(threadIdx.x) & 63;
+  const Geom &g = a.g;
+  // this lane's cell in the rows laid end to end (nseg = cells per row); lanes before the
+  // first or past the last cell read any valid cell and compute no face
+  const int nrow_j = a.ju - a.jl + 1;
+  const long long ncell = (long long)nseg * nrow_j * (a.ku - a.kl + 1);
+  const long long v = (long long)wave * FPW - H + lane;
+  const long long vc = v < 0 ? 0 : (v >= ncell ? ncell - 1 : v);
+  const int row = int(vc / nseg), ci = int(vc - (long long)row * nseg);
+  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
+  const long long row0 = g.at(k, j, 0);
+  const T *src[NV];
+  var_ptrs<T, 0, NV>(a, src);
+  T q[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) q[n] = src[n][row0 + ci];
+  T Lface[NV], Rf[NV];
+  recon_lanes<T, RC, NV>(q, a.flatten, T(a.eos.dfloor), T(a.eos.pfloor), Lface, Rf);
   // face ci (between cells ci - 1 and ci): its stencil ci - H .. ci + 2 lies in this row and in
   // this wave's lanes
   if (lane < H || lane >= H + FPW || v != vc || ci < a.il || ci > a.iu) return;
   solve_store<T, 0, RS, ENT, NV>(a, row0 + ci, 1, Lface, Rf);
 }
+
 
 template <class T, Recon RC, RSolver RS, bool ENT>
 __global__ void __launch_bounds__(256) k_flux_x(const FluxArgsT<T> a, const int nseg, const int nwaves) {
@@ -199,8 +219,8 @@ __device__ __forceinline__ void flux_m_tile(const FluxArgsT<T> &a, const int bx)
   int i, t, sg;
   if (!m_item<T, DIR>(a, (long long)bx * (BX * BY) + int(threadIdx.y) * BX + int(threadIdx.x), i, t, sg)) return;
   const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
-  const int F0 = lo_d + sg * SEG;
-  const int F1 = min(F0 + SEG - 1, hi_d);
+  const int F0 = lo_d + sg * seg_v<T>;
+  const int F1 = min(F0 + seg_v<T> - 1, hi_d);
   int j, k;
   if constexpr (DIR == 1) { k = a.kl + t; j = 0; }
   else { j = a.jl + t; k = 0; }

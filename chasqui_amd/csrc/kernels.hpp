@@ -5,6 +5,9 @@
 // variables and face-centred B are always double.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace chq {
@@ -140,6 +143,8 @@ inline __host__ __device__ long long bc_points(const Geom &g, int d) {
 namespace fluxk {
 constexpr int BX = 64, BY = 4;
 constexpr int SEG = 24;   // faces per thread in the y/z march (measured at 256^3: 24 beats 8, 16, 32)
+constexpr int SEG32 = 24;   // the same for the FP32 field set (24 beats 13, 20, 26, 37, 52)
+template <class T> constexpr int seg_v = std::is_same<T, float>::value ? SEG32 : SEG;
 
 template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
 inline __host__ __device__ int fpw_of(Recon rc) {
@@ -165,7 +170,7 @@ __host__ __device__ inline void m_shape(const FluxArgsT<T> &a, int &gx, int &gy,
   const int nx = a.iu - a.il + 1;
   const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
   const int nd = DIR == 1 ? a.ju - a.jl + 1 : a.ku - a.kl + 1;
-  gx = int(((long long)nx * nt * div_up(nd, SEG) + BX * BY - 1) / (BX * BY));
+  gx = int(((long long)nx * nt * div_up(nd, seg_v<T>) + BX * BY - 1) / (BX * BY));
   gy = 1;
   gz = 1;
 }
@@ -179,7 +184,7 @@ __host__ __device__ inline bool m_item(const FluxArgsT<T> &a, long long lin, int
   i = a.il + int(lin - r * nx);
   t = int(r % nt);
   sg = int(r / nt);
-  return sg < div_up(nd, SEG);
+  return sg < div_up(nd, seg_v<T>);
 }
 
 }  // namespace fluxk
