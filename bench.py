@@ -274,6 +274,33 @@ def ensemble(nruns, steps, precision, seed=7, tune=True):
             "speedup_vs_sequential": seq_ms / ms, "tuned_sum_ms": tuned, "grids": grids}
 
 
+def batch_floor(nruns=64, steps=50):
+    """Per-step cost of batches whose work is negligible, where the launches of a batched step
+    set the time: `nruns` 2D Orszag-Tang runs of 32^2 and `nruns` 1D Brio-Wu tubes of 400 cells
+    (PPM + HLLD + RK3, FP64), stepped as a batch with the captured graph; ms per batched step."""
+    from chasqui_amd import Batch
+    from chasqui_amd.deck import Deck
+    res = {}
+    for name, text, ov in (("orszag_tang_32x32", OT_DECK, ["mesh.nx1=32", "mesh.nx2=32", "mhd.reconstruct=ppm"]),
+                           ("brio_wu_400", BW_DECK, ["mesh.nx1=400"])):
+        meshes = []
+        for _ in range(nruns):
+            d = Deck(text, ov + ["time.nlim=-1", "time.tlim=1e9"])
+            meshes.append(d.make_mesh())
+            d.close()
+        b = Batch(nruns)
+        for m in meshes:
+            b.add(m)
+        b.step(2)
+        b.sync()
+        res[name + "_ms_per_step"] = b.step_timed(steps) / steps
+        b.close()
+        for m in meshes:
+            m.close()
+    res["runs"] = nruns
+    return res
+
+
 def serve_jobs(njobs, seed=11):
     """Served stream: seeded mix of 1D Brio-Wu shock tubes (400-4000 cells) run to their
     tlim = 0.1, 2D Orszag-Tang (128^2-1024^2) and 3D blasts (64^3-192^3) capped at 100 and 50
@@ -384,7 +411,9 @@ def main():
         if r.returncode == 0:
             stream = json.loads(r.stdout)
     if args.only == "batch":
-        print(json.dumps({p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}))
+        res = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
+        res["batch_floor"] = batch_floor()
+        print(json.dumps(res))
         return
     if args.only == "serve":
         print(json.dumps(serve(args.serve_jobs, rate_factor=args.serve_rate, chunk=args.serve_chunk)))
@@ -449,6 +478,7 @@ def main():
                         "l1_rel_vs_fp64": l1, "l1_cycles": mst["cycle"]}
     if args.ensemble_runs > 0:
         out["batch"] = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
+        out["batch_floor"] = batch_floor()
     if args.serve_jobs > 0:
         out["serve"] = serve(args.serve_jobs, rate_factor=args.serve_rate, chunk=args.serve_chunk)
     if not args.no_cpu_baseline:

@@ -192,6 +192,18 @@ struct Batch final : BatchBase {
     const int n = int(b->runs.size());
     h_runs.resize(size_t(n));
     for (int r = 0; r < n; ++r) h_runs[r] = describe(b->runs[r]);
+    // the y/z march segment follows the march threads of the whole batch per direction
+    for (int d = 1; d < 3; ++d) {
+      long long threads = 0;
+      for (int r = 0; r < n; ++r) {
+        const FluxArgsT<T> &a = h_runs[r].flux[0][0][d];
+        if (d < h_runs[r].ndir) threads += fluxk::march_threads(a.il, a.iu, a.jl, a.ju, a.kl, a.ku, d, fluxk::SEG);
+      }
+      const int seg = fluxk::march_seg(threads);
+      for (int r = 0; r < n; ++r)
+        for (auto &row : h_runs[r].flux)
+          for (auto &ws : row) ws[d].seg = seg;
+    }
     for (int k = 0; k < K_NLIST; ++k) h_items[k].clear();
     for (int r = 0; r < n; ++r) {
       for (int k = 0; k < K_NLIST; ++k) {
@@ -216,6 +228,7 @@ struct Batch final : BatchBase {
     }
     if (n) HIPCHK(hipMemcpyAsync(d_runs, h_runs.data(), sizeof(RunArgs<T>) * size_t(n), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(d_count, h_count, sizeof(h_count), hipMemcpyHostToDevice, b->stream));
+    if (present_mask() != captured_present) b->graph_ok = false;   // a class appeared or emptied
     // live flags in the new table order, from the runs' control blocks (after the run count)
     if (int e = ctrl(CTRL_LIVE, 0)) return e;
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -239,6 +252,10 @@ struct Batch final : BatchBase {
     return CHQ_OK;
   }
 
+  // The launches of a stage; flux and boundary classes that no run of the batch has (the y and z
+  // directions of 1D runs, z of 2D) are left out (present[], fixed when the step is enqueued or
+  // captured; a change re-captures the graph). The control launches between stages are merged:
+  // begin step + begin stage 1, end stage + begin next, end last stage + end step.
   int enqueue_stage(int s) {
     const Coef cf = rk_coef(b->nstage, s);
     const int last = s == b->nstage;
@@ -246,28 +263,40 @@ struct Batch final : BatchBase {
     const RSolver rs = RSolver(b->rsolver);
     hipStream_t st = b->stream;
     const int *g = b->grid;
-    if (int e = ctrl(CTRL_BEGIN_STAGE, last)) return e;
-    for (int d = 0; d < 3; ++d) HIPCHK(launchb_flux(d_runs, list(K_FLUX1 + d), d, cf.cur, rc, rs, g[K_FLUX1 + d], st));
+    for (int d = 0; d < 3; ++d)
+      if (present[K_FLUX1 + d]) HIPCHK(launchb_flux(d_runs, list(K_FLUX1 + d), d, cf.cur, rc, rs, g[K_FLUX1 + d], st));
     HIPCHK(launchb_corner<T>(d_runs, list(K_CORNER), g[K_CORNER], st));
     HIPCHK(launchb_update<T>(d_runs, list(K_UPDATE), cf.out, cf.g0, cf.g1, cf.beta, g[K_UPDATE], st));
-    for (int d = 0; d < 3; ++d) HIPCHK(launchb_bc<T>(d_runs, list(K_BC1 + d), d, cf.out, g[K_BC1 + d], st));
+    for (int d = 0; d < 3; ++d)
+      if (present[K_BC1 + d]) HIPCHK(launchb_bc<T>(d_runs, list(K_BC1 + d), d, cf.out, g[K_BC1 + d], st));
     HIPCHK(launchb_c2p<T>(d_runs, list(K_C2P), cf.out, last, g[K_C2P], st));
     // fofc branch of the runs that flagged cells
     HIPCHK(launchb_fofc_scan<T>(d_runs, d_count, cond(), st));
     HIPCHK(launchb_repair(d_runs, cond(), cf.cur, g[G_REPAIR], st));
     HIPCHK(launchb_corner_cond<T>(d_runs, cond(), g[G_REPAIR], st));
     HIPCHK(launchb_update_cond<T>(d_runs, cond(), cf.out, cf.g0, cf.g1, cf.beta, g[G_COND], st));
-    for (int d = 0; d < 3; ++d) HIPCHK(launchb_bc_cond<T>(d_runs, cond(), d, cf.out, g[G_COND], st));
+    for (int d = 0; d < 3; ++d)
+      if (present[K_BC1 + d]) HIPCHK(launchb_bc_cond<T>(d_runs, cond(), d, cf.out, g[G_COND], st));
     HIPCHK(launchb_c2p_cond<T>(d_runs, cond(), cf.out, last, g[G_COND], st));
-    if (int e = ctrl(CTRL_END_STAGE, last)) return e;
-    return CHQ_OK;
+    if (last) return ctrl(CTRL_END_LAST, 0);
+    return ctrl(CTRL_TURN, s + 1 == b->nstage);
   }
 
   int enqueue_step() override {
-    if (int e = ctrl(CTRL_BEGIN_STEP, 0)) return e;
+    for (int k = 0; k < K_NLIST; ++k) present[k] = h_count[1 + k] > 0;
+    captured_present = present_mask();
+    if (int e = ctrl(CTRL_STEP_FIRST, b->nstage == 1)) return e;
     for (int s = 1; s <= b->nstage; ++s)
       if (int e = enqueue_stage(s)) return e;
-    return ctrl(CTRL_END_STEP, 0);
+    return CHQ_OK;
+  }
+
+  bool present[K_NLIST]{};
+  unsigned captured_present = ~0u;
+  unsigned present_mask() const {
+    unsigned m = 0;
+    for (int k = 0; k < K_NLIST; ++k) m |= (h_count[1 + k] > 0 ? 1u : 0u) << k;
+    return m;
   }
 
   // one timed launch of class q with grid gsz, in the stage-1 configuration (input buffer 0,

@@ -9,7 +9,7 @@
 // 5-cell window c-2..c+2 its reconstruction reads), so, as for the primitive kernels, each
 // cell is done once and hands its two half-states to its two faces:
 //   * x faces: lane = cell, the window arrives through cross-lane shuffles;
-//   * y and z faces: lane = i, each thread marches SEG faces along the direction with the
+//   * y and z faces: lane = i, each thread marches a.seg faces along the direction with the
 //     raw window in registers.
 // Sums keep the reference's order (ascending variable index for the projection, ascending
 // wave index for the map back); the structurally zero matrix entries are left out. The
@@ -239,8 +239,8 @@ __global__ void __launch_bounds__(BX * BY) k_fluxc_m(const FluxArgsT<double> a) 
   if (!m_item<double, DIR>(a, (long long)blockIdx.x * (BX * BY) + int(threadIdx.y) * BX + int(threadIdx.x), i, t, sg))
     return;
   const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
-  const int F0 = lo_d + sg * SEG;
-  const int F1 = min(F0 + SEG - 1, hi_d);
+  const int F0 = lo_d + sg * a.seg;
+  const int F1 = min(F0 + a.seg - 1, hi_d);
   int j, k;
   if constexpr (DIR == 1) { k = a.kl + t; j = 0; }
   else { j = a.jl + t; k = 0; }

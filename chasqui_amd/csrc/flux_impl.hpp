@@ -10,7 +10,7 @@
 //     move between lanes with cross-lane shuffles; a wave yields 64 - 5 faces (halo 3 + 2)
 //     and may span the end of one row and the start of the next.
 //   * y and z faces (DIR 1, 2): lane = i (coalesced loads), each thread marches along j or
-//     k over a segment of SEG faces with a register window: every primitive is loaded
+//     k over a segment of a.seg faces with a register window: every primitive is loaded
 //     once per segment and each cell is reconstructed once. Threads are numbered with i
 //     fastest over the exact face range, so no lane idles at a row end.
 // For T = double the arithmetic and its order are the reference's: fluxes are bitwise equal.
@@ -219,8 +219,8 @@ __device__ __forceinline__ void flux_m_tile(const FluxArgsT<T> &a, const int bx)
   int i, t, sg;
   if (!m_item<T, DIR>(a, (long long)bx * (BX * BY) + int(threadIdx.y) * BX + int(threadIdx.x), i, t, sg)) return;
   const int lo_d = DIR == 1 ? a.jl : a.kl, hi_d = DIR == 1 ? a.ju : a.ku;
-  const int F0 = lo_d + sg * seg_v<T>;
-  const int F1 = min(F0 + seg_v<T> - 1, hi_d);
+  const int F0 = lo_d + sg * a.seg;
+  const int F1 = min(F0 + a.seg - 1, hi_d);
   int j, k;
   if constexpr (DIR == 1) { k = a.kl + t; j = 0; }
   else { j = a.jl + t; k = 0; }

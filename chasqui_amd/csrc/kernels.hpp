@@ -22,6 +22,7 @@ struct FluxArgsT {
   int charac;         // reconstruct in characteristic variables (FP64 single-mesh launches only)
   double lhlld_phi;
   int il, iu, jl, ju, kl, ku;
+  int seg;            // faces per thread of the y/z march (fluxk::march_seg)
   const T *w;         // primitives, 6 components (IENT used only when entropy is transported)
   const T *bcc;       // cell-centred B, 3 components
   const double *bn;   // face-normal field of this direction (the stage-input face B)
@@ -142,9 +143,19 @@ inline __host__ __device__ long long bc_points(const Geom &g, int d) {
 // of the lanes idle)
 namespace fluxk {
 constexpr int BX = 64, BY = 4;
-constexpr int SEG = 24;   // faces per thread in the y/z march (measured at 256^3: 24 beats 8, 16, 32)
-constexpr int SEG32 = 24;   // the same for the FP32 field set (24 beats 13, 20, 26, 37, 52)
-template <class T> constexpr int seg_v = std::is_same<T, float>::value ? SEG32 : SEG;
+// Faces per thread in the y/z march. SEG where the launch's march threads fill the device
+// (measured at 256^3: 24 beats 8, 16, 32 in FP64 and 13, 20, 26, 37, 52 in FP32); SEG_SMALL
+// where they would leave it idle (small meshes, batches of small runs), since one thread's
+// serial march is then the launch's latency. The choice moves work between threads only: every
+// face is computed by the same expressions either way.
+constexpr int SEG = 24, SEG_SMALL = 8;
+constexpr long long SEG_FILL = 131072;   // march threads of two 256-thread blocks per CU, 256 CUs
+// march threads of direction d (1 or 2) for segments of `seg` faces, from the face ranges
+inline __host__ __device__ long long march_threads(int il, int iu, int jl, int ju, int kl, int ku, int d, int seg) {
+  const long long nx = iu - il + 1, nt = d == 1 ? ku - kl + 1 : ju - jl + 1, nd = d == 1 ? ju - jl + 1 : ku - kl + 1;
+  return nx * nt * ((nd + seg - 1) / seg);
+}
+inline __host__ __device__ int march_seg(long long threads_at_seg) { return threads_at_seg < SEG_FILL ? SEG_SMALL : SEG; }
 
 template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
 inline __host__ __device__ int fpw_of(Recon rc) {
@@ -163,14 +174,14 @@ __host__ __device__ inline void x_shape(const FluxArgsT<T> &a, int fpw, int &nse
   nwaves = int((cells + fpw - 1) / fpw);
 }
 
-// y/z-face work: one thread per (i, transverse row, segment of SEG faces along DIR), i fastest,
+// y/z-face work: one thread per (i, transverse row, segment of a.seg faces along DIR), i fastest,
 // in blocks of BX x BY threads (gx blocks; gy = gz = 1).
 template <class T, int DIR>
 __host__ __device__ inline void m_shape(const FluxArgsT<T> &a, int &gx, int &gy, int &gz) {
   const int nx = a.iu - a.il + 1;
   const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
   const int nd = DIR == 1 ? a.ju - a.jl + 1 : a.ku - a.kl + 1;
-  gx = int(((long long)nx * nt * div_up(nd, seg_v<T>) + BX * BY - 1) / (BX * BY));
+  gx = int(((long long)nx * nt * div_up(nd, a.seg) + BX * BY - 1) / (BX * BY));
   gy = 1;
   gz = 1;
 }
@@ -184,7 +195,7 @@ __host__ __device__ inline bool m_item(const FluxArgsT<T> &a, long long lin, int
   i = a.il + int(lin - r * nx);
   t = int(r % nt);
   sg = int(r / nt);
-  return sg < div_up(nd, seg_v<T>);
+  return sg < div_up(nd, a.seg);
 }
 
 }  // namespace fluxk
@@ -315,7 +326,10 @@ template <class T> hipError_t launchb_c2p_cond(const RunArgs<T> *R, CondList C, 
 // control-block step of every run (one thread per table slot, `cap` slots); the end of a step
 // also writes live[r] = run r has not reached tlim / nlim
 template <class T> hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int cap, int op, int arg, int *live, hipStream_t s);
-enum : int { CTRL_BEGIN_STEP = 0, CTRL_BEGIN_STAGE, CTRL_END_STAGE, CTRL_END_STEP, CTRL_LIVE };
+// CTRL_STEP_FIRST: begin step + begin stage 1; CTRL_TURN: end a stage + begin the next (arg:
+// the next one is the last); CTRL_END_LAST: end the last stage + end step (+ live flags)
+enum : int { CTRL_BEGIN_STEP = 0, CTRL_BEGIN_STAGE, CTRL_END_STAGE, CTRL_END_STEP, CTRL_LIVE, CTRL_STEP_FIRST,
+             CTRL_TURN, CTRL_END_LAST };
 
 // position of `it` in an exclusive prefix pre[0..n] (largest p with pre[p] <= it)
 __device__ __forceinline__ int locate(const int *pre, int n, int it) {

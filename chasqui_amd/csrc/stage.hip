@@ -963,8 +963,15 @@ __global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *n
   if (op == CTRL_BEGIN_STEP) ctrl_begin_step(c);
   else if (op == CTRL_BEGIN_STAGE) ctrl_begin_stage(c, arg, -1);
   else if (op == CTRL_END_STAGE) ctrl_end_stage(c, arg, 1);
-  else {
-    if (op == CTRL_END_STEP) ctrl_end_step(c);
+  else if (op == CTRL_STEP_FIRST) {
+    ctrl_begin_step(c);
+    ctrl_begin_stage(c, arg, -1);
+  } else if (op == CTRL_TURN) {
+    ctrl_end_stage(c, 0, 1);
+    ctrl_begin_stage(c, arg, -1);
+  } else {
+    if (op == CTRL_END_LAST) ctrl_end_stage(c, 1, 1);
+    if (op == CTRL_END_STEP || op == CTRL_END_LAST) ctrl_end_step(c);
     live[r] = ctrl_running(c) ? 1 : 0;
   }
 }

@@ -113,6 +113,39 @@ def test_batch_membership_changes():
         m.close()
 
 
+def test_batch_direction_classes_appear_and_vanish():
+    """The batched step leaves out the flux and boundary launches of directions no run has
+    (1D and 2D batches); a run that brings a new direction re-captures the graph with them, and
+    one that takes the last run of a direction away captures it without them."""
+    from chasqui_amd import Batch
+    print("case direction classes", flush=True)
+    one, two, three = RUNS[5], RUNS[3], RUNS[0]
+    ref = {}
+    for key, (name, ov), n in (("1", one, 10), ("2", two, 6), ("3", three, 3)):
+        m = _mesh(name, ov, "fp64")
+        m.step(n)
+        ref[key] = _state(m)
+        m.close()
+    b = Batch(4)
+    m1, m2, m3 = _mesh(*one, "fp64"), _mesh(*two, "fp64"), _mesh(*three, "fp64")
+    b.add(m1)
+    b.add(m2)
+    b.step(2)        # x and y classes only
+    b.add(m3)        # z appears: re-captured
+    b.step(3)
+    b.remove(m3)     # 3 steps for the 3D run
+    b.step(1)
+    b.remove(m2)     # 6 steps for the 2D run; y disappears
+    b.step(4)        # 10 steps for the 1D run
+    b.sync()
+    _same(_state(m1), ref["1"], "1D")
+    _same(_state(m2), ref["2"], "2D")
+    _same(_state(m3), ref["3"], "3D")
+    b.close()
+    for m in (m1, m2, m3):
+        m.close()
+
+
 def test_batch_autotune_keeps_results():
     """The autotuner times kernel classes on scratch arrays only: tuning between steps must not
     change any run's trajectory."""
