@@ -145,17 +145,20 @@ namespace fluxk {
 constexpr int BX = 64, BY = 4;
 // Faces per thread in the y/z march. SEG where the launch's march threads fill the device
 // (measured at 256^3: 24 beats 8, 16, 32 in FP64 and 13, 20, 26, 37, 52 in FP32); SEG_SMALL
-// where they would leave it idle (small meshes, batches of small runs), since one thread's
-// serial march is then the launch's latency. The choice moves work between threads only: every
+// where they would leave it idle (small meshes, batches of small runs), and SEG_TINY where
+// they would use a sixteenth of it, since one thread's serial march is then the launch's
+// latency. The choice moves work between threads only: every
 // face is computed by the same expressions either way.
-constexpr int SEG = 24, SEG_SMALL = 8;
+constexpr int SEG = 24, SEG_SMALL = 8, SEG_TINY = 4;
 constexpr long long SEG_FILL = 131072;   // march threads of two 256-thread blocks per CU, 256 CUs
 // march threads of direction d (1 or 2) for segments of `seg` faces, from the face ranges
 inline __host__ __device__ long long march_threads(int il, int iu, int jl, int ju, int kl, int ku, int d, int seg) {
   const long long nx = iu - il + 1, nt = d == 1 ? ku - kl + 1 : ju - jl + 1, nd = d == 1 ? ju - jl + 1 : ku - kl + 1;
   return nx * nt * ((nd + seg - 1) / seg);
 }
-inline __host__ __device__ int march_seg(long long threads_at_seg) { return threads_at_seg < SEG_FILL ? SEG_SMALL : SEG; }
+inline __host__ __device__ int march_seg(long long threads_at_seg) {
+  return threads_at_seg < SEG_FILL / 16 ? SEG_TINY : threads_at_seg < SEG_FILL ? SEG_SMALL : SEG;
+}
 
 template <Recon RC> constexpr bool wide_v = !(RC == Recon::dc || RC == Recon::plm);
 inline __host__ __device__ int fpw_of(Recon rc) {
