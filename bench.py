@@ -274,6 +274,27 @@ def ensemble(nruns, steps, precision, seed=7, tune=True):
             "speedup_vs_sequential": seq_ms / ms, "tuned_sum_ms": tuned, "grids": grids}
 
 
+def shard(n=256, ranks=8, steps=20, warmup=2):
+    """One GPU's share of the n^3 blast split into `ranks` slabs along z: an n x n x (n/ranks)
+    mesh with the same cell size, cut through the blast centre, periodic ghosts standing in for
+    the halo exchange a multi-GPU run would do after each update. FP64, the headline scheme;
+    zone-updates/s of the slab."""
+    from chasqui_amd.deck import Deck
+    nz = n // ranks
+    half = 0.5 * nz / n
+    d = Deck(DECK, [f"mesh.nx1={n}", f"mesh.nx2={n}", f"mesh.nx3={nz}", f"mesh.x3min={-half!r}",
+                    f"mesh.x3max={half!r}"])
+    mesh = d.make_mesh()
+    d.close()
+    mesh.step(warmup)
+    mesh.sync()
+    ms = mesh.step_timed(steps)
+    mesh.close()
+    return {"shape": [n, n, nz], "ranks": ranks, "ms_per_step": ms / steps,
+            "zone_updates_per_s": float(n) * n * nz * steps / (ms * 1e-3),
+            "note": "one of the slabs of the headline mesh; the halo exchange is not included"}
+
+
 def batch_floor(nruns=64, steps=50):
     """Per-step cost of batches whose work is negligible, where the launches of a batched step
     set the time: `nruns` 2D Orszag-Tang runs of 32^2 and `nruns` 1D Brio-Wu tubes of 400 cells
@@ -387,7 +408,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision side measurement")
     ap.add_argument("--ensemble-runs", type=int, default=512,
-                    help="runs of the batched Orszag-Tang ensemble measured beside the headline (0: skip)")
+                    help="runs of the batched Orszag-Tang ensemble measured beside the headline (0: skip it and "
+                         "the other side measurements: the batch floor and the one-slab shard)")
     ap.add_argument("--serve-jobs", type=int, default=100,
                     help="jobs of the served stream measured beside the headline (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0,
@@ -477,6 +499,8 @@ def main():
                         "fofc_corrections_in_run": mst["fofc_total"], "kernels": mk,
                         "l1_rel_vs_fp64": l1, "l1_cycles": mst["cycle"]}
     if args.ensemble_runs > 0:
+        if n == 256:
+            out["shard"] = shard(n)
         out["batch"] = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
         out["batch_floor"] = batch_floor()
     if args.serve_jobs > 0:
