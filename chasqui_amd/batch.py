@@ -1,7 +1,8 @@
 """Batches: many independent meshes (`Mesh` objects of one scheme) advanced by shared launches.
 
-Each RK stage is one launch per kernel class over a device list of (run, tile) items, captured
-once in a hipGraph and replayed; every run keeps its own dt, t, tlim, CFL, boundaries and size,
+Each RK stage is one launch per kernel class over a device list of (run, tile) items (classes
+that no run needs, such as the z faces of 2D runs, are left out), captured once in a hipGraph
+and replayed; every run keeps its own dt, t, tlim, CFL, boundaries and size,
 and its result is bitwise the one `Mesh.step` gives it alone. `tune()` is the autotuner of the
 per-class grid sizes.
 """

@@ -138,7 +138,7 @@ inline __host__ __device__ long long bc_points(const Geom &g, int d) {
 }
 
 // face-flux decomposition (flux_impl.hpp): x faces by wave64 windows over the rows' cells laid
-// end to end, y/z faces by threads marching SEG faces along the direction; both linearised so
+// end to end, y/z faces by threads marching a.seg faces along the direction; both linearised so
 // that no lane idles at the end of a row (a 258-wide row in 64-wide tiles would leave a fifth
 // of the lanes idle)
 namespace fluxk {
