@@ -270,20 +270,27 @@ struct Batch final : BatchBase {
     for (int d = 0; d < 3; ++d)
       if (present[K_BC1 + d]) HIPCHK(launchb_bc<T>(d_runs, list(K_BC1 + d), d, cf.out, g[K_BC1 + d], st));
     HIPCHK(launchb_c2p<T>(d_runs, list(K_C2P), cf.out, last, g[K_C2P], st));
-    // fofc branch of the runs that flagged cells
-    HIPCHK(launchb_fofc_scan<T>(d_runs, d_count, cond(), st));
-    HIPCHK(launchb_repair(d_runs, cond(), cf.cur, g[G_REPAIR], st));
-    HIPCHK(launchb_corner_cond<T>(d_runs, cond(), g[G_REPAIR], st));
-    HIPCHK(launchb_update_cond<T>(d_runs, cond(), cf.out, cf.g0, cf.g1, cf.beta, g[G_COND], st));
-    for (int d = 0; d < 3; ++d)
-      if (present[K_BC1 + d]) HIPCHK(launchb_bc_cond<T>(d_runs, cond(), d, cf.out, g[G_COND], st));
-    HIPCHK(launchb_c2p_cond<T>(d_runs, cond(), cf.out, last, g[G_COND], st));
+    // fofc branch of the runs that flagged cells: for batches of small runs, one block per run
+    // in two launches; otherwise compacted over the flagged runs, one launch per class
+    if (small_branch) {
+      HIPCHK(launchb_repair_runs(d_runs, d_count, b->capacity, cf.cur, st));
+      HIPCHK(launchb_fofc_runs<T>(d_runs, d_count, b->capacity, cf.out, cf.g0, cf.g1, cf.beta, last, st));
+    } else {
+      HIPCHK(launchb_fofc_scan<T>(d_runs, d_count, cond(), st));
+      HIPCHK(launchb_repair(d_runs, cond(), cf.cur, g[G_REPAIR], st));
+      HIPCHK(launchb_corner_cond<T>(d_runs, cond(), g[G_REPAIR], st));
+      HIPCHK(launchb_update_cond<T>(d_runs, cond(), cf.out, cf.g0, cf.g1, cf.beta, g[G_COND], st));
+      for (int d = 0; d < 3; ++d)
+        if (present[K_BC1 + d]) HIPCHK(launchb_bc_cond<T>(d_runs, cond(), d, cf.out, g[G_COND], st));
+      HIPCHK(launchb_c2p_cond<T>(d_runs, cond(), cf.out, last, g[G_COND], st));
+    }
     if (last) return ctrl(CTRL_END_LAST, 0);
     return ctrl(CTRL_TURN, s + 1 == b->nstage);
   }
 
   int enqueue_step() override {
     for (int k = 0; k < K_NLIST; ++k) present[k] = h_count[1 + k] > 0;
+    small_branch = small_runs();
     captured_present = present_mask();
     if (int e = ctrl(CTRL_STEP_FIRST, b->nstage == 1)) return e;
     for (int s = 1; s <= b->nstage; ++s)
@@ -292,11 +299,25 @@ struct Batch final : BatchBase {
   }
 
   bool present[K_NLIST]{};
+  bool small_branch = false;
   unsigned captured_present = ~0u;
+  // the runs' fofc branch fits one block each: at most SMALL_BRANCH_TILES tiles of re-update,
+  // boundaries and con2prim per run (a few thousand cells), where launches cost more than the
+  // block's serial walk over its tiles
+  static constexpr int SMALL_BRANCH_TILES = 48;
+  bool small_runs() const {
+    if (h_runs.empty()) return false;
+    for (const RunArgs<T> &r : h_runs) {
+      int t = 0;
+      for (int q = 0; q < 5; ++q) t += r.cond_tiles[q];
+      if (t > SMALL_BRANCH_TILES) return false;
+    }
+    return true;
+  }
   unsigned present_mask() const {
     unsigned m = 0;
     for (int k = 0; k < K_NLIST; ++k) m |= (h_count[1 + k] > 0 ? 1u : 0u) << k;
-    return m;
+    return m | (small_runs() ? 1u << K_NLIST : 0u);
   }
 
   // one timed launch of class q with grid gsz, in the stage-1 configuration (input buffer 0,

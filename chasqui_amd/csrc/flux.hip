@@ -58,5 +58,8 @@ hipError_t launchb_flux(const RunArgs<double> *R, ItemList L, int dir, int cur, 
 hipError_t launchb_repair(const RunArgs<double> *R, CondList C, int cur, int grid, hipStream_t s) {
   return fluxk::launchb_repair_t<double>(R, C, cur, grid, s);
 }
+hipError_t launchb_repair_runs(const RunArgs<double> *R, const int *nruns, int cap, int cur, hipStream_t s) {
+  return fluxk::launchb_repair_runs_t<double>(R, nruns, cap, cur, s);
+}
 
 }  // namespace chq

@@ -28,5 +28,8 @@ hipError_t launchb_flux(const RunArgs<float> *R, ItemList L, int dir, int cur, R
 hipError_t launchb_repair(const RunArgs<float> *R, CondList C, int cur, int grid, hipStream_t s) {
   return fluxk::launchb_repair_t<float>(R, C, cur, grid, s);
 }
+hipError_t launchb_repair_runs(const RunArgs<float> *R, const int *nruns, int cap, int cur, hipStream_t s) {
+  return fluxk::launchb_repair_runs_t<float>(R, nruns, cap, cur, s);
+}
 
 }  // namespace chq

@@ -465,6 +465,20 @@ __global__ void __launch_bounds__(BX * BY) kb_flux_m(const RunArgs<T> *R, const 
 }
 
 // fofc repair of the flagged runs (compacted list, tiles of 256 (cell, face) entries)
+// entry t of run r's repair: face `side` of direction `dir` of listed cell t / (2 ndir)
+template <class T>
+__device__ __forceinline__ void repair_entry(const RunArgs<T> &run, const Ctrl *ctl, int t, int cur) {
+  const int per = 2 * run.ndir;
+  if (t >= ctl->nbad * per) return;
+  const FluxArgsT<T> *fa = run.flux[cur][ctl->wcur];
+  const Geom &g = fa[0].g;
+  const long long c0 = run.badlist[t / per];
+  const int q = t % per, dir = q >> 1, side = q & 1;
+  if (dir == 0) repair_face<T, 0>(fa[0], c0 + side, 1);
+  else if (dir == 1) repair_face<T, 1>(fa[1], c0 + side * (long long)g.pitch, g.pitch);
+  else repair_face<T, 2>(fa[2], c0 + side * g.plane, g.plane);
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) kb_repair(const RunArgs<T> *R, const CondList C, const int cur) {
   const int nrun = *C.nrun();
@@ -474,18 +488,20 @@ __global__ void __launch_bounds__(256) kb_repair(const RunArgs<T> *R, const Cond
   for (int it = int(blockIdx.x); it < total; it += int(gridDim.x)) {
     const int p = locate(pre, nrun, it);
     const int r = C.run()[p];
-    const Ctrl *ctl = R[r].ctrl;
-    const int wc = ctl->wcur, per = 2 * R[r].ndir;
-    const int t = (it - pre[p]) * 256 + int(threadIdx.x);
-    if (t >= ctl->nbad * per) continue;
-    const FluxArgsT<T> *fa = R[r].flux[cur][wc];
-    const Geom &g = fa[0].g;
-    const long long c0 = R[r].badlist[t / per];
-    const int q = t % per, dir = q >> 1, side = q & 1;
-    if (dir == 0) repair_face<T, 0>(fa[0], c0 + side, 1);
-    else if (dir == 1) repair_face<T, 1>(fa[1], c0 + side * (long long)g.pitch, g.pitch);
-    else repair_face<T, 2>(fa[2], c0 + side * g.plane, g.plane);
+    repair_entry(R[r], R[r].ctrl, (it - pre[p]) * 256 + int(threadIdx.x), cur);
   }
+}
+
+// The repair of a batch of small runs: one block per run, which walks its run's entries
+// (runs that flagged nothing exit at once; no compaction pass is needed).
+template <class T>
+__global__ void __launch_bounds__(256) kb_repair_runs(const RunArgs<T> *R, const int *nruns, const int cur) {
+  const int r = int(blockIdx.x);
+  if (r >= *nruns) return;
+  const Ctrl *ctl = R[r].ctrl;
+  if (!ctl->active || ctl->nbad == 0) return;
+  const int n = ctl->nbad * 2 * R[r].ndir;
+  for (int t = int(threadIdx.x); t < n; t += 256) repair_entry(R[r], ctl, t, cur);
 }
 
 template <class T, int DIR, Recon RC>
@@ -524,6 +540,12 @@ hipError_t launchb_flux_t(const RunArgs<T> *R, ItemList L, int dir, int cur, Rec
 template <class T>
 hipError_t launchb_repair_t(const RunArgs<T> *R, CondList C, int cur, int grid, hipStream_t s) {
   hipLaunchKernelGGL(kb_repair<T>, dim3(grid), dim3(256), 0, s, R, C, cur);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launchb_repair_runs_t(const RunArgs<T> *R, const int *nruns, int cap, int cur, hipStream_t s) {
+  hipLaunchKernelGGL(kb_repair_runs<T>, dim3(cap > 0 ? cap : 1), dim3(256), 0, s, R, nruns, cur);
   return hipGetLastError();
 }
 

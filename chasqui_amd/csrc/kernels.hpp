@@ -317,6 +317,12 @@ hipError_t launchb_flux(const RunArgs<double> *R, ItemList L, int dir, int cur, 
 hipError_t launchb_flux(const RunArgs<float> *R, ItemList L, int dir, int cur, Recon rc, RSolver rs, int grid, hipStream_t s);
 hipError_t launchb_repair(const RunArgs<double> *R, CondList C, int cur, int grid, hipStream_t s);
 hipError_t launchb_repair(const RunArgs<float> *R, CondList C, int cur, int grid, hipStream_t s);
+// the fofc branch of a batch of small runs: one block per run (cap blocks), the repair, then the
+// edge EMFs, re-update, boundaries and con2prim of the run
+hipError_t launchb_repair_runs(const RunArgs<double> *R, const int *nruns, int cap, int cur, hipStream_t s);
+hipError_t launchb_repair_runs(const RunArgs<float> *R, const int *nruns, int cap, int cur, hipStream_t s);
+template <class T> hipError_t launchb_fofc_runs(const RunArgs<T> *R, const int *nruns, int cap, int out, double g0,
+                                                double g1, double beta, int last, hipStream_t s);
 template <class T> hipError_t launchb_corner(const RunArgs<T> *R, ItemList L, int grid, hipStream_t s);
 template <class T> hipError_t launchb_update(const RunArgs<T> *R, ItemList L, int out, double g0, double g1, double beta, int grid, hipStream_t s);
 template <class T> hipError_t launchb_bc(const RunArgs<T> *R, ItemList L, int dir, int buf, int grid, hipStream_t s);

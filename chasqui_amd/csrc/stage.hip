@@ -954,6 +954,38 @@ __global__ void __launch_bounds__(256) kb_c2p_cond(const RunArgs<T> *R, const Co
   cond_walk(C, CL_C2P, [&](int r, int tile) { c2p_tile<T, LAST>(R[r], R[r].ctrl, tile, buf, true, smin, ssum); });
 }
 
+// The rest of the fofc branch of a batch of small runs, after kb_repair_runs: one block per
+// run walks the tiles of its run phase by phase (edge EMFs around the listed cells, the
+// whole-run re-update, boundaries x1 -> x2 -> x3, con2prim), with block barriers between the
+// phases; a run that flagged nothing exits at once. The tile bodies are those of the
+// multi-launch branch, so every value is the same.
+template <class T, bool LAST>
+__global__ void __launch_bounds__(256) kb_fofc_runs(const RunArgs<T> *R, const int *nruns, int out, double g0,
+                                                    double g1, double beta) {
+  __shared__ double smin[BX * BY / 64];
+  __shared__ int ssum[BX * BY / 64];
+  const int r = int(blockIdx.x);
+  if (r >= *nruns) return;
+  const RunArgs<T> &run = R[r];
+  const Ctrl *ctl = run.ctrl;
+  if (!ctl->active || ctl->nbad == 0) return;   // the same for every thread of the block
+  const int nbad = ctl->nbad;
+  {
+    const CornerArgsT<T> a = run.corner[ctl->wcur];
+    for (int t = int(threadIdx.x); t < nbad * 8; t += 256) corner_list_entry(a, run.badlist, t);
+  }
+  __syncthreads();
+  for (int tile = 0; tile < run.cond_tiles[0]; ++tile) update_tile(run, ctl, tile, out, g0, g1, beta);
+  __syncthreads();
+  for (int tile = 0; tile < run.cond_tiles[1]; ++tile) bc_at<0>(run.bc[out][0], (long long)tile * 256 + threadIdx.x);
+  __syncthreads();
+  for (int tile = 0; tile < run.cond_tiles[2]; ++tile) bc_at<1>(run.bc[out][1], (long long)tile * 256 + threadIdx.x);
+  __syncthreads();
+  for (int tile = 0; tile < run.cond_tiles[3]; ++tile) bc_at<2>(run.bc[out][2], (long long)tile * 256 + threadIdx.x);
+  __syncthreads();
+  for (int tile = 0; tile < run.cond_tiles[4]; ++tile) c2p_tile<T, LAST>(run, ctl, tile, out, true, smin, ssum);
+}
+
 // live[r] (end of step) = 1 while run r has not reached tlim / nlim
 template <class T>
 __global__ void __launch_bounds__(256) kb_ctrl(const RunArgs<T> *R, const int *nruns, int op, int arg, int *live) {
@@ -1165,6 +1197,15 @@ hipError_t launchb_bc_cond(const RunArgs<T> *R, CondList C, int dir, int buf, in
   return hipGetLastError();
 }
 template <class T>
+hipError_t launchb_fofc_runs(const RunArgs<T> *R, const int *nruns, int cap, int out, double g0, double g1,
+                             double beta, int last, hipStream_t s) {
+  const dim3 grid(cap > 0 ? cap : 1), block(256);
+  if (last) hipLaunchKernelGGL((kb_fofc_runs<T, true>), grid, block, 0, s, R, nruns, out, g0, g1, beta);
+  else hipLaunchKernelGGL((kb_fofc_runs<T, false>), grid, block, 0, s, R, nruns, out, g0, g1, beta);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launchb_c2p_cond(const RunArgs<T> *R, CondList C, int buf, int last, int grid, hipStream_t s) {
   if (last) hipLaunchKernelGGL((kb_c2p_cond<T, true>), dim3(grid), dim3(256), 0, s, R, C, buf);
   else hipLaunchKernelGGL((kb_c2p_cond<T, false>), dim3(grid), dim3(256), 0, s, R, C, buf);
@@ -1188,6 +1229,8 @@ hipError_t launchb_ctrl(const RunArgs<T> *R, const int *nruns, int cap, int op, 
                                              hipStream_t);                                                    \
   template hipError_t launchb_bc_cond<T>(const RunArgs<T> *, CondList, int, int, int, hipStream_t);          \
   template hipError_t launchb_c2p_cond<T>(const RunArgs<T> *, CondList, int, int, int, hipStream_t);         \
+  template hipError_t launchb_fofc_runs<T>(const RunArgs<T> *, const int *, int, int, double, double, double, int, \
+                                           hipStream_t);         \
   template hipError_t launchb_ctrl<T>(const RunArgs<T> *, const int *, int, int, int, int *, hipStream_t);
 CHQ_BATCH_INST(double)
 CHQ_BATCH_INST(float)

@@ -146,6 +146,43 @@ def test_batch_direction_classes_appear_and_vanish():
         m.close()
 
 
+SMALL_RUNS = [
+    ("blast_3d.in", ["mesh.nx1=10", "mesh.nx2=10", "mesh.nx3=10", "mhd.pfloor=0.0999"]),   # fofc fires
+    ("blast_3d.in", ["mesh.nx1=8", "mesh.nx2=10", "mesh.nx3=6"]),
+    ("orszag_tang.in", ["mesh.nx1=24", "mesh.nx2=20"]),
+    ("brio_wu.in", ["mesh.nx1=200", "mhd.dfloor=0.2"]),                                      # fofc fires
+    ("cpaw_1d.in", ["mesh.nx1=64"]),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_small_runs_fofc_equals_solo(prec):
+    """A batch whose runs are all small takes the fofc branch with one block per run (two
+    launches per stage instead of one per kernel class); two of its runs flag cells in every
+    stage. Each run is bitwise its solo run."""
+    from chasqui_amd import Batch
+    print(f"case small runs {prec}", flush=True)
+    steps = 6
+    ref = []
+    for name, ov in SMALL_RUNS:
+        m = _mesh(name, ov, prec)
+        m.step(steps)
+        ref.append(_state(m))
+        m.close()
+    assert ref[0]["status"][3] > 0 and ref[3]["status"][3] > 0   # fofc fired in the solo runs
+    b = Batch(len(SMALL_RUNS))
+    meshes = [_mesh(name, ov, prec) for name, ov in SMALL_RUNS]
+    for m in meshes:
+        b.add(m)
+    b.step(steps)
+    b.sync()
+    for q, m in enumerate(meshes):
+        _same(_state(m), ref[q], f"small run {q}")
+    b.close()
+    for m in meshes:
+        m.close()
+
+
 def test_batch_autotune_keeps_results():
     """The autotuner times kernel classes on scratch arrays only: tuning between steps must not
     change any run's trajectory."""
