@@ -5,7 +5,10 @@ Device code is compiled with -ffp-contract=off (no FMA contraction: the FP64 pat
 bitwise equal to the reference method, which is built the same way) and without packed
 FP32 instructions. The FP32 flux kernels of the mixed-precision path (flux32.hip) are built
 with FMA contraction, the fast FP32 divide / square root and FP32 denormals flushed to zero
-(the division then needs no denormal range scaling: 12% fewer instructions per face).
+(the division then needs no denormal range scaling: 12% fewer instructions per face), finite-math
+and no-signed-zero semantics, and reciprocal division (a / b as a * (1 / b), with 1 / b shared
+where several quotients have the same divisor: 6-9% fewer instructions per face again, mixed
+step 0.9% faster). The flux templates hold no NaN, infinity or signed-zero test.
 """
 import os
 import subprocess
@@ -31,7 +34,8 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 HIP_SOURCES = ["flux.hip", "flux32.hip", "flux_char.hip", "stage.hip", "layout.hip", "diffusion.hip", "srcterms.hip",
                "history.hip"]
 EXTRA = {"flux32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
-                        "-fgpu-flush-denormals-to-zero"]}
+                        "-fgpu-flush-denormals-to-zero", "-fno-honor-nans", "-fno-honor-infinities",
+                        "-fno-signed-zeros", "-freciprocal-math"]}
 HOST_SOURCES = ["capi.cpp", "batch.cpp", "server.cpp", "host/params.cpp", "host/pgen.cpp", "host/output.cpp"]
 
 

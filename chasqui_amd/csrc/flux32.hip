@@ -1,7 +1,8 @@
 // FP32 face-flux kernels of the mixed-precision path: reconstruction, flattening, Riemann
 // solve and face EMFs in FP32 registers, from FP32 primitives (written by the FP64
 // con2prim) to FP32 fluxes and EMFs (widened to FP64 by the update). This file is compiled
-// with fast FP32 division/square root and FMA contraction; the kernel bodies are the same
+// with fast FP32 division/square root, FMA contraction, flushed denormals, finite-math and
+// no-signed-zero semantics and reciprocal division; the kernel bodies are the same
 // templates as the FP64 path (flux_impl.hpp) with the reference's single-precision literals
 // (recon.hpp:145-149 WENO epsilon; Real(1e-300) -> 0).
 #include "flux_impl.hpp"
