@@ -272,6 +272,7 @@ struct chq_ctx {
     a.e1 = f.ee[0]; a.e2 = f.ee[1]; a.e3 = f.ee[2];
     a.ctrl = c;
     a.only_if_bad = oib;
+    a.xg = XG_CORNER;
     return a;
   }
   template <class T> UpdateArgsT<T> update_args_t(const FieldSet<T> &f, int out, double g0, double g1, int nvar,
@@ -286,6 +287,7 @@ struct chq_ctx {
     a.e1 = f.ee[0]; a.e2 = f.ee[1]; a.e3 = f.ee[2];
     a.ctrl = c;
     a.only_if_bad = oib;
+    a.xg = std::is_same<T, float>::value ? XG_UPDATE_F32 : 0;
     return a;
   }
   template <class T> C2pArgsT<T> c2p_args_t(const FieldSet<T> &f, int buf, int wset, Ctrl *c) const {
@@ -295,6 +297,7 @@ struct chq_ctx {
     a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.ecc = f.ecc[wset];
     a.write_went = 1; a.entropy_out = 1; a.compute_dt = 0; a.slot = 0; a.flag_mode = 0;
     a.flag = flag; a.badlist = badlist; a.ctrl = c; a.only_if_bad = 0;
+    a.xg = XG_C2P;
     return a;
   }
 };

@@ -42,6 +42,7 @@ struct CornerArgsT {
   T *e1, *e2, *e3;
   const Ctrl *ctrl;
   int only_if_bad;             // skip unless the current stage flagged cells (fofc re-run)
+  int xg;                      // XCD tile grouping (xcd_tile)
 };
 using CornerArgs = CornerArgsT<double>;
 
@@ -67,6 +68,7 @@ struct UpdateArgsT {
   // -0 and the divergence (curl) term is 0, or where the result is not finite
   int zwatch;
   int *zcount;
+  int xg;                      // XCD tile grouping (xcd_tile)
 };
 using UpdateArgs = UpdateArgsT<double>;
 
@@ -95,6 +97,7 @@ struct C2pArgsT {
   int *badlist;
   Ctrl *ctrl;
   int only_if_bad;
+  int xg;                      // XCD tile grouping (xcd_tile)
 };
 using C2pArgs = C2pArgsT<double>;
 
@@ -102,6 +105,24 @@ using C2pArgs = C2pArgsT<double>;
 // The memory-bound kernels walk "virtual blocks": chunks of 256 consecutive points of one
 // (j, i) plane of an nx x ny box (rows are linearised, so rows of 257 or 262 points waste no
 // lanes), times nz planes.
+// XCD-grouped tile order: the dispatcher deals consecutive workgroups of a launch to the 8 XCDs
+// in turn, so neighbouring tiles (rows that read each other's lines) land in different L2s.
+// Block b of n takes tile xcd_tile(b, n, grp): each XCD gets runs of grp consecutive tiles,
+// groups of 8 * grp tiles in launch order (the tail past the last whole group keeps b). A
+// bijection on [0, n); grp <= 1 is the identity.
+__device__ __forceinline__ int xcd_tile(int b, int n, int grp) {
+  if (grp <= 1) return b;
+  const int span = 8 * grp, full = (n / span) * span;
+  if (b >= full) return b;
+  const int l = b >> 3;
+  return (l / grp) * span + (b & 7) * grp + l % grp;
+}
+// Group sizes of the fused stage's kernels, measured at 256^3 (ms per RK3 step, FP64 / mixed):
+// con2prim 32 (1.65 -> 1.58 / 1.28 -> 1.16; 4, 8, 16 and 64 gain less), corner EMF 4 (1.47 ->
+// 1.44 / 0.90 -> 0.88; 8 and 16 the same), the float update march 8 (2.62 -> 2.60); the FP64
+// update keeps the launch order (grouping 4-16 made it 1-1.2% slower).
+constexpr int XG_C2P = 32, XG_CORNER = 4, XG_UPDATE_F32 = 8;
+
 struct VB {
   int nx, ny, cpp, n;   // plane size, chunks per plane, total chunks
   __device__ __forceinline__ bool at(int vb, int &k, int &j, int &i) const {

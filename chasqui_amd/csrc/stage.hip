@@ -261,9 +261,10 @@ __global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
   const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
+  const int grp = int(gridDim.x) == v.n ? a.xg : 0;   // one tile per block: XCD-grouped order
   for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
     int k, j, i;
-    if (!v.at(vb, k, j, i)) continue;
+    if (!v.at(xcd_tile(vb, v.n, grp), k, j, i)) continue;
     update_at(a, bdt, g.ks + k, g.js + j, g.is + i);
   }
 }
@@ -283,10 +284,13 @@ template <class T>
 __global__ void __launch_bounds__(256) k_corner_march(const CornerArgsT<T> a, const VB v, const int nk, const int kseg) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
+  const int nb = int(gridDim.x * gridDim.y);
+  const int t = xcd_tile(int(blockIdx.y * gridDim.x + blockIdx.x), nb, a.xg);
+  const int bx = t % int(gridDim.x), by = t / int(gridDim.x);
   int kk, jj, ii;
-  if (!v.at(int(blockIdx.x), kk, jj, ii)) return;   // v spans one plane
+  if (!v.at(bx, kk, jj, ii)) return;   // v spans one plane
   const int i = g.is + ii, j = g.js + jj;
-  const int k0 = g.ks + int(blockIdx.y) * kseg;
+  const int k0 = g.ks + by * kseg;
   const int k1 = min(k0 + kseg - 1, g.ks + nk - 1);
   const long long sj = g.pitch, sk = g.plane, cs = g.cstride;
   const T *ex = a.ecc, *ey = a.ecc + cs, *ez = a.ecc + 2 * cs;
@@ -335,10 +339,13 @@ template <class T>
 __global__ void __launch_bounds__(256) k_update_march(const UpdateArgsT<T> a, const VB v, const int nk, const int kseg) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   const Geom &g = a.g;
+  const int nb = int(gridDim.x * gridDim.y);
+  const int t = xcd_tile(int(blockIdx.y * gridDim.x + blockIdx.x), nb, a.xg);
+  const int bx = t % int(gridDim.x), by = t / int(gridDim.x);
   int kk, jj, ii;
-  if (!v.at(int(blockIdx.x), kk, jj, ii)) return;
+  if (!v.at(bx, kk, jj, ii)) return;
   const int i = g.is + ii, j = g.js + jj;
-  const int k0 = g.ks + int(blockIdx.y) * kseg;
+  const int k0 = g.ks + by * kseg;
   const int k1 = min(k0 + kseg - 1, g.ks + nk - 1);
   const double bdt = a.use_ctrl_dt ? a.beta * a.ctrl->dt : a.beta_dt;
   const double g0 = a.g0, g1 = a.g1;
@@ -649,14 +656,14 @@ __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   double mdt = 1e300;
   int nbad_local = 0;
-  if (LOOP) {
+  if (LOOP) {   // (the grid is a multiple of 8 blocks, so vb & 7 is still the block's XCD)
     for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
       int k, j, i;
-      if (v.at(vb, k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
+      if (v.at(xcd_tile(vb, v.n, a.xg), k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
     }
   } else {
     int k, j, i;
-    if (v.at(int(blockIdx.x), k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
+    if (v.at(xcd_tile(int(blockIdx.x), int(gridDim.x), a.xg), k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
   }
   if (!DT && a.flag_mode != 1) return;
   __shared__ double smin[BX * BY / 64];
