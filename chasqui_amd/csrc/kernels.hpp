@@ -120,7 +120,7 @@ __device__ __forceinline__ int xcd_tile(int b, int n, int grp) {
 // Group sizes of the fused stage's kernels, measured at 256^3 (ms per RK3 step, FP64 / mixed):
 // con2prim 32 (1.65 -> 1.58 / 1.28 -> 1.16; 4, 8, 16 and 64 gain less), corner EMF 4 (1.47 ->
 // 1.44 / 0.90 -> 0.88; 8 and 16 the same), the float update march 8 (2.62 -> 2.60); the FP64
-// update keeps the launch order (grouping 4-16 made it 1-1.2% slower).
+// update keeps the launch order (runs of 2, 3, 4, 8 or 16 made it 0.4-1.2% slower).
 constexpr int XG_C2P = 32, XG_CORNER = 4, XG_UPDATE_F32 = 8;
 
 struct VB {
