@@ -769,7 +769,7 @@ template <class T>
 __global__ void __launch_bounds__(256) kb_corner(const RunArgs<T> *R, const ItemList L) {
   const int n = *L.n;
   for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
-    const Item m = L.item[it];
+    const Item m = L.item[xcd_tile(it, n, XG_CORNER)];   // (the grids are multiples of 8 blocks)
     const Ctrl *ctl = R[m.run].ctrl;
     if (!ctl->active) continue;
     const CornerArgsT<T> a = R[m.run].corner[ctl->wcur];
@@ -797,8 +797,9 @@ template <class T>
 __global__ void __launch_bounds__(256) kb_update(const RunArgs<T> *R, const ItemList L, int out, double g0, double g1,
                                                  double beta) {
   const int n = *L.n;
+  const int grp = std::is_same<T, float>::value ? XG_UPDATE_F32 : 0;
   for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
-    const Item m = L.item[it];
+    const Item m = L.item[xcd_tile(it, n, grp)];
     const Ctrl *ctl = R[m.run].ctrl;
     if (!ctl->active) continue;
     update_tile(R[m.run], ctl, m.tile, out, g0, g1, beta);
@@ -839,7 +840,7 @@ __global__ void __launch_bounds__(256) kb_c2p(const RunArgs<T> *R, const ItemLis
   __shared__ int ssum[BX * BY / 64];
   const int n = *L.n;
   for (int it = int(blockIdx.x); it < n; it += int(gridDim.x)) {
-    const Item m = L.item[it];
+    const Item m = L.item[xcd_tile(it, n, XG_C2P)];
     const Ctrl *ctl = R[m.run].ctrl;
     if (!ctl->active) continue;
     c2p_tile<T, LAST>(R[m.run], ctl, m.tile, buf, false, smin, ssum);
