@@ -649,22 +649,18 @@ __device__ __forceinline__ void c2p_block_merge(const C2pArgsT<T> &a, double mdt
 // cons -> prim over all cells, physics.hpp:437-494. The CFL minimum is reduced with wave64
 // shuffles and one LDS pass per block, then merged with a 64-bit atomicMin on the bit
 // pattern, issued only when the block's minimum beats the current one.
-// LOOP: grid-stride over the tiles (the conditional launches); otherwise one tile per block,
-// which keeps the loop-invariant constants of pow() out of the registers of the whole kernel.
-template <class T, bool DT, bool ENT, bool DE, bool LOOP>
+// One tile per block, also for the fofc-branch re-run (gate 1): a grid-stride form with a capped
+// grid exits sooner when the gate is closed (about 5 vs 29 us at 256^3) but ran the re-run of a
+// firing stage 13% slower (749 vs 660 us: blocks of four or five tiles leave a long tail), and
+// in the benchmark stage 1 fires every step. One tile per block also keeps the loop-invariant
+// constants of pow() out of the registers of the whole kernel.
+template <class T, bool DT, bool ENT, bool DE>
 __global__ void __launch_bounds__(256) k_c2p(const C2pArgsT<T> a, const VB v) {
   if (!gate(a.ctrl, a.only_if_bad)) return;
   double mdt = 1e300;
   int nbad_local = 0;
-  if (LOOP) {   // (the grid is a multiple of 8 blocks, so vb & 7 is still the block's XCD)
-    for (int vb = blockIdx.x; vb < v.n; vb += gridDim.x) {
-      int k, j, i;
-      if (v.at(xcd_tile(vb, v.n, a.xg), k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
-    }
-  } else {
-    int k, j, i;
-    if (v.at(xcd_tile(int(blockIdx.x), int(gridDim.x), a.xg), k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
-  }
+  int k, j, i;
+  if (v.at(xcd_tile(int(blockIdx.x), int(gridDim.x), a.xg), k, j, i)) c2p_at<T, DT, ENT, DE>(a, k, j, i, mdt, nbad_local);
   if (!DT && a.flag_mode != 1) return;
   __shared__ double smin[BX * BY / 64];
   __shared__ int ssum[BX * BY / 64];
@@ -678,9 +674,10 @@ __global__ void __launch_bounds__(BX * BY) k_flag(const C2pArgs a) {
   const Geom &g = a.g;
   const int i = g.is + int(blockIdx.x) * BX + int(threadIdx.x);
   const int j = g.js + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int k = g.ks + int(blockIdx.z);
   int bad_count = 0;
-  if (i <= g.ie && j <= g.je) {
+  // planes ks + blockIdx.z, + gridDim.z, ...: the grid is capped (launch_flag), so a launch
+  // whose gate is closed costs a few microseconds
+  for (int k = g.ks + int(blockIdx.z); k <= g.ke && i <= g.ie && j <= g.je; k += int(gridDim.z)) {
     const Eos e = a.eos;
     const long long c = g.at(k, j, i), sj = g.pitch, sk = g.plane, cs = g.cstride;
     const double bx = 0.5 * (a.b1[c] + a.b1[c + 1]);
@@ -700,7 +697,7 @@ __global__ void __launch_bounds__(BX * BY) k_flag(const C2pArgs a) {
       if (bad) a.badlist[atomicAdd(&a.ctrl->nbad, 1)] = int(c);
     } else {
       a.flag[c] = bad ? 1 : 0;
-      bad_count = bad;
+      bad_count += bad;
     }
   }
   if (a.flag_mode == 2) return;
@@ -1074,8 +1071,8 @@ hipError_t launch_mark_near(const Geom &g, const Ctrl *ctrl, const int *list, in
 
 template <class T>
 hipError_t launch_update_near(const UpdateArgsT<T> &a, const int *marks, hipStream_t s) {
-  const VB v = update_vb(a.g);
-  hipLaunchKernelGGL(k_update_near<T>, dim3(v.n), dim3(256), 0, s, a, v, marks);
+  const VB v = update_vb(a.g);   // a capped grid-stride grid: most points only read their mark
+  hipLaunchKernelGGL(k_update_near<T>, dim3(grid_of(v, true)), dim3(256), 0, s, a, v, marks);
   return hipGetLastError();
 }
 template hipError_t launch_update_near<double>(const UpdateArgsT<double> &, const int *, hipStream_t);
@@ -1095,23 +1092,13 @@ void launch_c2p_de(const C2pArgsT<T> &a, hipStream_t s) {
   const Geom &g = a.g;
   const bool ent = a.entropy_out || a.eos.dual_energy;
   const VB v = c2p_vb(g);
-  const dim3 grid(grid_of(v, a.only_if_bad == 1)), block(256);
-  if (a.only_if_bad == 1) {
-    if (a.compute_dt) {
-      if (ent) hipLaunchKernelGGL((k_c2p<T, true, true, DE, true>), grid, block, 0, s, a, v);
-      else hipLaunchKernelGGL((k_c2p<T, true, false, DE, true>), grid, block, 0, s, a, v);
-    } else {
-      if (ent) hipLaunchKernelGGL((k_c2p<T, false, true, DE, true>), grid, block, 0, s, a, v);
-      else hipLaunchKernelGGL((k_c2p<T, false, false, DE, true>), grid, block, 0, s, a, v);
-    }
+  const dim3 grid(v.n), block(256);
+  if (a.compute_dt) {
+    if (ent) hipLaunchKernelGGL((k_c2p<T, true, true, DE>), grid, block, 0, s, a, v);
+    else hipLaunchKernelGGL((k_c2p<T, true, false, DE>), grid, block, 0, s, a, v);
   } else {
-    if (a.compute_dt) {
-      if (ent) hipLaunchKernelGGL((k_c2p<T, true, true, DE, false>), grid, block, 0, s, a, v);
-      else hipLaunchKernelGGL((k_c2p<T, true, false, DE, false>), grid, block, 0, s, a, v);
-    } else {
-      if (ent) hipLaunchKernelGGL((k_c2p<T, false, true, DE, false>), grid, block, 0, s, a, v);
-      else hipLaunchKernelGGL((k_c2p<T, false, false, DE, false>), grid, block, 0, s, a, v);
-    }
+    if (ent) hipLaunchKernelGGL((k_c2p<T, false, true, DE>), grid, block, 0, s, a, v);
+    else hipLaunchKernelGGL((k_c2p<T, false, false, DE>), grid, block, 0, s, a, v);
   }
 }
 }  // namespace
@@ -1134,8 +1121,9 @@ template hipError_t launch_c2p<float>(const C2pArgsT<float> &, hipStream_t);
 
 hipError_t launch_flag(const C2pArgs &a, hipStream_t s) {
   const Geom &g = a.g;
-  dim3 grid(div_up(g.ie - g.is + 1, BX), div_up(g.je - g.js + 1, BY), g.ke - g.ks + 1), block(BX, BY, 1);
-  hipLaunchKernelGGL(k_flag, grid, block, 0, s, a);
+  const int gx = div_up(g.ie - g.is + 1, BX), gy = div_up(g.je - g.js + 1, BY), nk = g.ke - g.ks + 1;
+  const int gz = std::max(1, std::min(nk, COND_BLOCKS / (gx * gy)));   // planes strided by gz
+  hipLaunchKernelGGL(k_flag, dim3(gx, gy, gz), dim3(BX, BY, 1), 0, s, a);
   return hipGetLastError();
 }
 
