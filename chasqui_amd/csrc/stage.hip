@@ -36,8 +36,9 @@ __device__ __forceinline__ bool gate(const Ctrl *c, int only_if_bad) {
 }
 
 // Blocks iterate over virtual blocks (VB, kernels.hpp): a normal launch has one block per
-// tile; the launches of the fofc branch, which usually exit at once, use a small
-// grid-stride grid so an empty branch costs a few microseconds.
+// tile; most launches of the fofc branch, which usually exit at once, use a small
+// grid-stride grid so an empty branch costs a few microseconds (con2prim keeps one tile per
+// block: the re-run of a firing stage is a full pass, and a grid-stride form slows it 13%).
 constexpr int COND_BLOCKS = 16384;   // grid of the conditional (fofc-branch) launches
 
 // ---------------------------------------------------------------- corner EMF
