@@ -417,25 +417,39 @@ __device__ __forceinline__ void bc1_at(const BcArgs &a, const long long t) {
   const int sr = bc == 0 ? ir - nx : bc == 1 ? ie : 2 * ie + 1 - ir;
   const double sg = bc == 2 ? -1.0 : 1.0;
   const long long row = g.at(k, j, 0), cs = g.cstride;
-  if (k < g.nk && j < g.nj) {
-    for (int n = 0; n < a.nvar; ++n) {
-      const double f = n == IM1 ? sg : 1.0;
-      double *u = a.u + n * cs + row;
-      const double vl = u[sl], vr = u[sr];
-      u[il] = f * vl;
-      u[ir] = f * vr;
-    }
-    a.b2[row + il] = a.b2[row + sl];
-    a.b2[row + ir] = a.b2[row + sr];
-    a.b3[row + il] = a.b3[row + sl];
-    a.b3[row + ir] = a.b3[row + sr];
+  // every load of this thread first, then every store: none of its stores feeds one of its
+  // loads, and its two ghosts of each row lie in lines of their own, so the loads overlap
+  const bool cells = k < g.nk && j < g.nj;
+  double vl[NVAR], vr[NVAR], b2l = 0.0, b2r = 0.0, b3l = 0.0, b3r = 0.0;
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n)
+      if (n < a.nvar) { vl[n] = a.u[n * cs + row + sl]; vr[n] = a.u[n * cs + row + sr]; }
+    b2l = a.b2[row + sl]; b2r = a.b2[row + sr];
+    b3l = a.b3[row + sl]; b3r = a.b3[row + sr];
   }
   double *b1 = a.b1 + row;
   const int fl = gg, fr = ie + 2 + gg;
-  if (bc == 0) { b1[fl] = b1[fl + nx]; b1[fr] = b1[fr - nx]; }
-  else if (bc == 1) { b1[fl] = b1[is]; b1[fr] = b1[ie + 1]; }
-  else { b1[fl] = -b1[2 * is - fl]; b1[fr] = -b1[2 * (ie + 1) - fr]; }
-  if (bc == 0 && gg == 0) b1[ie + 1] = b1[is];
+  double f1l, f1r;
+  if (bc == 0) { f1l = b1[fl + nx]; f1r = b1[fr - nx]; }
+  else if (bc == 1) { f1l = b1[is]; f1r = b1[ie + 1]; }
+  else { f1l = -b1[2 * is - fl]; f1r = -b1[2 * (ie + 1) - fr]; }
+  const bool shared = bc == 0 && gg == 0;   // the periodic shared face takes the lower copy
+  const double f1s = shared ? b1[is] : 0.0;
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n)
+      if (n < a.nvar) {
+        const double f = n == IM1 ? sg : 1.0;
+        a.u[n * cs + row + il] = f * vl[n];
+        a.u[n * cs + row + ir] = f * vr[n];
+      }
+    a.b2[row + il] = b2l; a.b2[row + ir] = b2r;
+    a.b3[row + il] = b3l; a.b3[row + ir] = b3r;
+  }
+  b1[fl] = f1l;
+  b1[fr] = f1r;
+  if (shared) b1[ie + 1] = f1s;
 }
 
 __device__ __forceinline__ void bc2_at(const BcArgs &a, const long long t) {
@@ -451,27 +465,38 @@ __device__ __forceinline__ void bc2_at(const BcArgs &a, const long long t) {
   const double sg = bc == 2 ? -1.0 : 1.0;
   const long long sj = g.pitch, cs = g.cstride;
   const long long col = g.at(k, 0, i);
-  if (k < g.nk && i < g.ni) {
-    for (int n = 0; n < a.nvar; ++n) {
-      const double f = n == IM2 ? sg : 1.0;
-      double *u = a.u + n * cs + col;
-      const double vl = u[sl * sj], vr = u[sr * sj];
-      u[jl * sj] = f * vl;
-      u[jr * sj] = f * vr;
-    }
-    a.b3[col + jl * sj] = a.b3[col + sl * sj];
-    a.b3[col + jr * sj] = a.b3[col + sr * sj];
+  // loads first, then stores (as in bc1_at)
+  const bool cells = k < g.nk && i < g.ni, faces1 = k < g.nk;
+  double vl[NVAR], vr[NVAR], b3l = 0.0, b3r = 0.0, b1l = 0.0, b1r = 0.0;
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n)
+      if (n < a.nvar) { vl[n] = a.u[n * cs + col + sl * sj]; vr[n] = a.u[n * cs + col + sr * sj]; }
+    b3l = a.b3[col + sl * sj]; b3r = a.b3[col + sr * sj];
   }
-  if (k < g.nk) {
-    a.b1[col + jl * sj] = a.b1[col + sl * sj];
-    a.b1[col + jr * sj] = a.b1[col + sr * sj];
-  }
+  if (faces1) { b1l = a.b1[col + sl * sj]; b1r = a.b1[col + sr * sj]; }
   double *b2 = a.b2 + col;
   const int fl = gg, fr = je + 2 + gg;
-  if (bc == 0) { b2[fl * sj] = b2[(fl + ny) * sj]; b2[fr * sj] = b2[(fr - ny) * sj]; }
-  else if (bc == 1) { b2[fl * sj] = b2[js * sj]; b2[fr * sj] = b2[(je + 1) * sj]; }
-  else { b2[fl * sj] = -b2[(2 * js - fl) * sj]; b2[fr * sj] = -b2[(2 * (je + 1) - fr) * sj]; }
-  if (bc == 0 && gg == 0) b2[(je + 1) * sj] = b2[js * sj];
+  double f2l, f2r;
+  if (bc == 0) { f2l = b2[(fl + ny) * sj]; f2r = b2[(fr - ny) * sj]; }
+  else if (bc == 1) { f2l = b2[js * sj]; f2r = b2[(je + 1) * sj]; }
+  else { f2l = -b2[(2 * js - fl) * sj]; f2r = -b2[(2 * (je + 1) - fr) * sj]; }
+  const bool shared = bc == 0 && gg == 0;
+  const double f2s = shared ? b2[js * sj] : 0.0;
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n)
+      if (n < a.nvar) {
+        const double f = n == IM2 ? sg : 1.0;
+        a.u[n * cs + col + jl * sj] = f * vl[n];
+        a.u[n * cs + col + jr * sj] = f * vr[n];
+      }
+    a.b3[col + jl * sj] = b3l; a.b3[col + jr * sj] = b3r;
+  }
+  if (faces1) { a.b1[col + jl * sj] = b1l; a.b1[col + jr * sj] = b1r; }
+  b2[fl * sj] = f2l;
+  b2[fr * sj] = f2r;
+  if (shared) b2[(je + 1) * sj] = f2s;
 }
 
 __device__ __forceinline__ void bc3_at(const BcArgs &a, const long long t) {
@@ -487,25 +512,38 @@ __device__ __forceinline__ void bc3_at(const BcArgs &a, const long long t) {
   const double sg = bc == 2 ? -1.0 : 1.0;
   const long long sk = g.plane, cs = g.cstride;
   const long long col = g.at(0, j, i);
-  if (j < g.nj && i < g.ni) {
-    for (int n = 0; n < a.nvar; ++n) {
-      const double f = n == IM3 ? sg : 1.0;
-      double *u = a.u + n * cs + col;
-      const double vl = u[sl * sk], vr = u[sr * sk];
-      u[kl * sk] = f * vl;
-      u[kr * sk] = f * vr;
-    }
+  // loads first, then stores (as in bc1_at)
+  const bool cells = j < g.nj && i < g.ni;
+  double vl[NVAR], vr[NVAR];
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n)
+      if (n < a.nvar) { vl[n] = a.u[n * cs + col + sl * sk]; vr[n] = a.u[n * cs + col + sr * sk]; }
   }
-  a.b1[col + kl * sk] = a.b1[col + sl * sk];
-  a.b1[col + kr * sk] = a.b1[col + sr * sk];
-  a.b2[col + kl * sk] = a.b2[col + sl * sk];
-  a.b2[col + kr * sk] = a.b2[col + sr * sk];
+  const double b1l = a.b1[col + sl * sk], b1r = a.b1[col + sr * sk];
+  const double b2l = a.b2[col + sl * sk], b2r = a.b2[col + sr * sk];
   double *b3 = a.b3 + col;
   const int fl = gg, fr = ke + 2 + gg;
-  if (bc == 0) { b3[fl * sk] = b3[(fl + nz) * sk]; b3[fr * sk] = b3[(fr - nz) * sk]; }
-  else if (bc == 1) { b3[fl * sk] = b3[ks * sk]; b3[fr * sk] = b3[(ke + 1) * sk]; }
-  else { b3[fl * sk] = -b3[(2 * ks - fl) * sk]; b3[fr * sk] = -b3[(2 * (ke + 1) - fr) * sk]; }
-  if (bc == 0 && gg == 0) b3[(ke + 1) * sk] = b3[ks * sk];
+  double f3l, f3r;
+  if (bc == 0) { f3l = b3[(fl + nz) * sk]; f3r = b3[(fr - nz) * sk]; }
+  else if (bc == 1) { f3l = b3[ks * sk]; f3r = b3[(ke + 1) * sk]; }
+  else { f3l = -b3[(2 * ks - fl) * sk]; f3r = -b3[(2 * (ke + 1) - fr) * sk]; }
+  const bool shared = bc == 0 && gg == 0;
+  const double f3s = shared ? b3[ks * sk] : 0.0;
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n)
+      if (n < a.nvar) {
+        const double f = n == IM3 ? sg : 1.0;
+        a.u[n * cs + col + kl * sk] = f * vl[n];
+        a.u[n * cs + col + kr * sk] = f * vr[n];
+      }
+  }
+  a.b1[col + kl * sk] = b1l; a.b1[col + kr * sk] = b1r;
+  a.b2[col + kl * sk] = b2l; a.b2[col + kr * sk] = b2r;
+  b3[fl * sk] = f3l;
+  b3[fr * sk] = f3r;
+  if (shared) b3[(ke + 1) * sk] = f3s;
 }
 
 template <int DIR>
