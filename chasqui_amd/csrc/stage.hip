@@ -277,9 +277,10 @@ __global__ void __launch_bounds__(256) k_update(const UpdateArgsT<T> a, const VB
 // same thread in the previous iteration, so every input plane is read once per segment instead
 // of being fetched again by the blocks of the next plane. Same expressions, same order as
 // corner3_at / update_at.
-// planes per thread, measured at 256^3 (corner: 4 beats 8, 16, 32, 64 and ties with 2; mixed update: 2
+// planes per thread, measured at 256^3 (corner: 4 beat 8, 16, 32, 64 and tied with 2 in launch
+// order; with the XCD-grouped order 2 is 1% faster than 4 and 1.3% faster than 1; mixed update: 2
 // beats 3, 4, 8, 16 and ties with 1)
-constexpr int KSEG_CORNER = 4, KSEG_UPDATE = 2;
+constexpr int KSEG_CORNER = 2, KSEG_UPDATE = 2;
 
 template <class T>
 __global__ void __launch_bounds__(256) k_corner_march(const CornerArgsT<T> a, const VB v, const int nk, const int kseg) {
