@@ -344,7 +344,7 @@ struct Batch final : BatchBase {
   // `reps` launches) and keep the fastest. Con2prim is timed with the fofc flags of the
   // tentative state disabled, so repeated launches do not grow the flag lists.
   int tune(int reps, double *best_ms) override {
-    static const int cand[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 32768};
+    static const int cand[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072, 262144};
     if (int e = ctrl(CTRL_BEGIN_STEP, 0)) return e;
     if (int e = ctrl(CTRL_BEGIN_STAGE, 0)) return e;
     // con2prim without flags while tuning
