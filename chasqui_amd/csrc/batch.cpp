@@ -513,7 +513,10 @@ int chq_batch_step(chq_batch *b, int nsteps) {
     if (b->use_graph) HIPCHK(hipGraphLaunch(b->graph, b->stream));
     else if (int e = b->impl->enqueue_step()) return e;
   }
-  for (chq_ctx *c : b->runs) c->wcur_stale = true;
+  for (chq_ctx *c : b->runs) {
+    c->wcur_stale = true;
+    c->ient_stale = true;   // batches transport no entropy; the last stage leaves u[IENT] to sync_ient
+  }
   return CHQ_OK;
 }
 

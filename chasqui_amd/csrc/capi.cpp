@@ -77,9 +77,27 @@ int sync_wcur(chq_ctx *c) {
   return CHQ_OK;
 }
 
+// The fast path's last RK stage leaves u[IENT] of the final state (buffer 0) to be
+// resynchronised here, before the host reads the state or a per-kernel entry point runs on it.
+int sync_ient(chq_ctx *c) {
+  if (!c->ient_stale) return CHQ_OK;
+  if (c->batch_stream) HIPCHK(hipStreamSynchronize(c->batch_stream));
+  HIPCHK(launch_ient_sync(c->c2p_args(0, c->wcur, c->ctrl_api), c->stream));
+  c->ient_stale = false;
+  return CHQ_OK;
+}
+int sync_state(chq_ctx *c) {
+  if (int rc = sync_wcur(c)) return rc;
+  return sync_ient(c);
+}
+
 int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
   if (c->batch_stream) HIPCHK(hipStreamSynchronize(c->batch_stream));
   if (int rc = sync_wcur(c)) return rc;
+  if (id == CHQ_U0) {   // an upload replaces every component
+    if (to_dev) c->ient_stale = false;
+    else if (int rc = sync_ient(c)) return rc;
+  }
   void *dev;
   int nc, face, eb, wide;
   if (!array_desc(c, id, &dev, &nc, &face, &eb, &wide)) return fail(CHQ_EINVAL, "unknown array id");
@@ -304,7 +322,7 @@ int chq_download(chq_ctx *c, int id, void *host, size_t count) {
 // ---------------------------------------------------------------- per-kernel entry points
 int chq_fluxes(chq_ctx *c, int dir, int cur) {
   if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
   if (c->mixed) {   // FP32 fluxes; the entropy component is not computed on this path
     FluxArgsT<float> a = c->flux_args_t(c->f32, dir, cur, c->wcur, c->ctrl_api);
@@ -319,7 +337,7 @@ int chq_fluxes(chq_ctx *c, int dir, int cur) {
 
 int chq_corner_emf(chq_ctx *c) {
   if (!c) return fail(CHQ_EINVAL, "null context");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) HIPCHK(launch_corner<float>(c->corner_args_t(c->f32, c->wcur, c->ctrl_api, 0), c->stream));
   else HIPCHK(launch_corner<double>(c->corner_args(c->wcur, c->ctrl_api, 0), c->stream));
   return CHQ_OK;
@@ -349,7 +367,7 @@ int chq_boundaries(chq_ctx *c, int buf) {
 
 int chq_cons_to_prim(chq_ctx *c, int buf, int compute_dt, double *min_dt) {
   if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 1, c->stream));
   if (c->mixed) {
     C2pArgsT<float> a = c->c2p_args_t(c->f32, buf, c->wcur, c->ctrl_api);
@@ -375,7 +393,7 @@ int chq_cons_to_prim(chq_ctx *c, int buf, int compute_dt, double *min_dt) {
 
 int chq_fofc_flag(chq_ctx *c, int buf, int *count) {
   if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_fofc_flag: FP64 contexts only");
   C2pArgs a = c->c2p_args(buf, c->wcur, c->ctrl_api);
   HIPCHK(launch_ctrl_begin_stage(c->ctrl_api, 0, c->stream));
@@ -389,7 +407,7 @@ int chq_fofc_flag(chq_ctx *c, int buf, int *count) {
 
 int chq_fofc_repair(chq_ctx *c, int dir, int cur) {
   if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_fofc_repair: FP64 contexts only");
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
   FluxArgs a = c->flux_args(dir, cur, c->wcur, c->ctrl_api);
@@ -553,7 +571,7 @@ int sts_advance(chq_ctx *c, Ctrl *k, int wset, double tau, double dt_par, int *s
 
 int chq_diffusion_fluxes(chq_ctx *c, int dir, int cur) {
   if (!c || dir < 0 || dir > 2 || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_diffusion_fluxes: FP64 contexts only");
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
   const DiffArgs a = c->diff_args(cur, c->wcur, c->ctrl_api, false);
@@ -563,7 +581,7 @@ int chq_diffusion_fluxes(chq_ctx *c, int dir, int cur) {
 
 int chq_resistive_emf(chq_ctx *c, int cur) {
   if (!c || (cur & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_resistive_emf: FP64 contexts only");
   const DiffArgs a = c->diff_args(cur, c->wcur, c->ctrl_api, false);
   if (a.eta > 0.0) HIPCHK(launch_resistive_emf(a, c->stream));
@@ -572,7 +590,7 @@ int chq_resistive_emf(chq_ctx *c, int cur) {
 
 int chq_diffusion_dt(chq_ctx *c, double *dt) {
   if (!c || !dt) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_diffusion_dt: FP64 contexts only");
   double rho = 1e300;
   if (c->diffusive())
@@ -583,7 +601,7 @@ int chq_diffusion_dt(chq_ctx *c, double *dt) {
 
 int chq_limit_fluxes(chq_ctx *c, int dir, double beta_dt, long *limited) {
   if (!c || dir < 0 || dir > 2) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_limit_fluxes: FP64 contexts only");
   if ((dir == 1 && !c->g.two_d) || (dir == 2 && !c->g.three_d)) return fail(CHQ_EINVAL, "direction not present in this mesh");
   long long before = 0, after = 0;
@@ -602,7 +620,7 @@ int chq_limit_fluxes(chq_ctx *c, int dir, double beta_dt, long *limited) {
 
 int chq_apply_gravity(chq_ctx *c, int out, double beta_dt) {
   if (!c || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_apply_gravity: FP64 contexts only");
   if (c->gravity()) HIPCHK(launch_gravity(c->src_args(out, c->wcur, c->ctrl_api, beta_dt, false), c->stream));
   return CHQ_OK;
@@ -610,7 +628,7 @@ int chq_apply_gravity(chq_ctx *c, int out, double beta_dt) {
 
 int chq_apply_cooling(chq_ctx *c, int buf, double dt) {
   if (!c || (buf & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_apply_cooling: FP64 contexts only");
   if (c->cfg.cooling) HIPCHK(launch_cooling(c->src_args(buf, c->wcur, c->ctrl_api, dt, false), c->stream));
   return CHQ_OK;
@@ -620,7 +638,7 @@ int chq_apply_cooling(chq_ctx *c, int buf, double dt) {
 // device when the flag kernel counted cells)
 int chq_stage(chq_ctx *c, int cur, int out, double g0, double g1, double beta_dt, long *nbad) {
   if (!c || (cur & ~1) || (out & ~1)) return fail(CHQ_EINVAL, "bad argument");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   if (c->mixed) return fail(CHQ_EUNSUP, "chq_stage: FP64 contexts only");
   const Ctrl *k = c->ctrl_api;
   const Recon rc = Recon(c->cfg.recon);
@@ -672,7 +690,7 @@ int chq_init(chq_ctx *c, double t, double tlim, int nlim) { return chq_init_resu
 
 int chq_init_resume(chq_ctx *c, double t, int cycle, double tlim, int nlim) {
   if (!c) return fail(CHQ_EINVAL, "null context");
-  if (int rc_ = sync_wcur(c)) return rc_;
+  if (int rc_ = sync_state(c)) return rc_;
   HIPCHK(c->boundaries(0, 6, c->ctrl_api, 0));
   double mdt = 0.0;
   int rc = chq_cons_to_prim(c, 0, 1, &mdt);
@@ -777,7 +795,11 @@ static int enqueue_fused_stage(chq_ctx *c, const FieldSet<T> &fs, const Coef &cf
   C2pArgsT<T> ca = c->c2p_args_t(fs, cf.out, wn, k);
   ca.only_if_bad = first;
   ca.write_went = c->entropy;
-  ca.entropy_out = c->entropy || last;   // u[IENT] of the intermediate RK states is never read
+  // u[IENT] is read by nothing in this path unless entropy is transported: the intermediate RK
+  // states never resynchronise it, and the last stage leaves it to sync_ient (the first host
+  // read of the state or per-kernel entry point), which stores what this con2prim would have
+  ca.entropy_out = c->entropy;
+  if (last && !c->entropy) c->ient_stale = true;   // the last stage writes buffer 0
   ca.compute_dt = last && !(c->cfg.sts || c->cfg.cooling);   // split steps take dt after the splits
   ca.slot = 0;
   ca.flag_mode = adaptive ? 3 : 0;

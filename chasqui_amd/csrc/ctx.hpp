@@ -84,6 +84,8 @@ struct chq_ctx {
   double *hst_scratch = nullptr;   // this device's static history scratch (chq_history_noalloc)
   int wcur = 0;              // which primitive set (w, bcc, ecc) is current
   bool wcur_stale = false;   // the time loop may have stopped early: the device copy (Ctrl::wcur) rules
+  bool ient_stale = false;   // u[0]'s u[IENT] awaits k_ient_sync: the fast path (entropy not
+                             // transported) leaves it to the first host read of the state
   void *batch = nullptr;     // the batch advancing this context, if any
   hipStream_t batch_stream = nullptr;   // its stream (host reads wait for it)
   bool entropy = false;      // transport IENT in the time loop (only needed with dual energy)

@@ -271,6 +271,9 @@ hipError_t launch_repair_list(const FluxArgsT<float> a[3], int ndir, const int *
 template <class T> hipError_t launch_corner(const CornerArgsT<T> &a, hipStream_t s);
 template <class T> hipError_t launch_corner_list(const CornerArgsT<T> &a, const int *list, hipStream_t s);
 template <class T> hipError_t launch_update(const UpdateArgsT<T> &a, hipStream_t s);
+// u[IENT] = rho p / rho^gamma over every cell of buffer a.u with con2prim's expressions (the
+// deferred resynchronisation of the fast path's last stage); writes nothing else
+hipError_t launch_ient_sync(const C2pArgsT<double> &a, hipStream_t s);
 // the local fofc re-update: stamp the points of the update box within one cell of each listed
 // (flagged) cell with the control block's epoch, then recompute exactly the stamped points
 hipError_t launch_mark_near(const Geom &g, const Ctrl *ctrl, const int *list, int *marks, hipStream_t s);

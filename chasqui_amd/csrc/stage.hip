@@ -579,7 +579,8 @@ __device__ __forceinline__ int wave_sum(int v) {
 // bz: the cell-centred x3 field (the caller averages the two x3 faces). DE: the dual
 // energy branches are compiled in (without them the kernel holds no pow() for the flag and
 // needs far fewer registers).
-template <class T, bool DT, bool ENT, bool DE>
+// IENT_ONLY: store u[IENT] and nothing else (the deferred resynchronisation, k_ient_sync).
+template <class T, bool DT, bool ENT, bool DE, bool IENT_ONLY = false>
 __device__ __forceinline__ void c2p_core(const C2pArgsT<T> &a, int k, int j, int i, const double bz, double &mdt,
                                          int &nbad_local) {
   const Geom &g = a.g;
@@ -591,7 +592,7 @@ __device__ __forceinline__ void c2p_core(const C2pArgsT<T> &a, int k, int j, int
   const double u0 = u[IDN * cs], u1 = u[IM1 * cs], u2 = u[IM2 * cs], u3 = u[IM3 * cs], u4 = u[IEN * cs];
   const bool active = i >= g.is && i <= g.ie && j >= g.js && j <= g.je && k >= g.ks && k <= g.ke;
   // flag_mode 3: append (as 2) unless this stage already flagged before the fill
-  const int fm = a.flag_mode == 3 ? (a.ctrl->early ? 0 : 2) : a.flag_mode;
+  const int fm = IENT_ONLY ? 0 : a.flag_mode == 3 ? (a.ctrl->early ? 0 : 2) : a.flag_mode;
   if (fm && active) {   // fofc.hpp:24-40 on the tentative state
     bool bad = u0 <= e.dfloor;
     if (!bad) {
@@ -629,6 +630,7 @@ __device__ __forceinline__ void c2p_core(const C2pArgsT<T> &a, int k, int j, int
       u[IENT * cs] = d * se;
     }
   }
+  if constexpr (IENT_ONLY) return;
   T *w = a.w + c;
   w[IDN * cs] = T(d);
   w[IVX * cs] = T(vx);
@@ -656,12 +658,23 @@ __device__ __forceinline__ void c2p_core(const C2pArgsT<T> &a, int k, int j, int
   }
 }
 
-template <class T, bool DT, bool ENT, bool DE>
+template <class T, bool DT, bool ENT, bool DE, bool IENT_ONLY = false>
 __device__ __forceinline__ void c2p_at(const C2pArgsT<T> &a, int k, int j, int i, double &mdt, int &nbad_local) {
   const Geom &g = a.g;
   const long long c = g.at(k, j, i);
   const double bz = g.three_d ? 0.5 * (a.b3[c] + a.b3[c + g.plane]) : a.b3[c];
-  c2p_core<T, DT, ENT, DE>(a, k, j, i, bz, mdt, nbad_local);
+  c2p_core<T, DT, ENT, DE, IENT_ONLY>(a, k, j, i, bz, mdt, nbad_local);
+}
+
+// The deferred resynchronisation of u[IENT] = rho p / rho^gamma over every cell, with the
+// expressions of con2prim: the fast path's last RK stage leaves it to the first host read of
+// the state (u[IENT] is read by nothing in that path when entropy is not transported), and this
+// computes the value that con2prim would have stored from the same u and face B.
+__global__ void __launch_bounds__(256) k_ient_sync(const C2pArgsT<double> a, const VB v) {
+  double mdt = 1e300;
+  int nb = 0;
+  int k, j, i;
+  if (v.at(int(blockIdx.x), k, j, i)) c2p_at<double, false, true, false, true>(a, k, j, i, mdt, nb);
 }
 
 // Block merge of the CFL minimum and the flag count into the control block (the block's
@@ -861,13 +874,13 @@ __device__ __forceinline__ void c2p_tile(const RunArgs<T> &run, const Ctrl *ctl,
                                          double *smin, int *ssum) {
   C2pArgsT<T> a = run.c2p[buf][ctl->wcur ^ 1];
   a.compute_dt = LAST;
-  a.entropy_out = LAST;
+  a.entropy_out = 0;   // u[IENT]: deferred to the first host read (k_ient_sync)
   a.slot = cond ? 1 : 0;
   if (cond) a.flag_mode = 0;
   double mdt = 1e300;
   int nb = 0;
   int k, j, i;
-  if (c2p_vb(a.g).at(tile, k, j, i)) c2p_at<T, LAST, LAST, false>(a, k, j, i, mdt, nb);   // batches: no dual energy
+  if (c2p_vb(a.g).at(tile, k, j, i)) c2p_at<T, LAST, false, false>(a, k, j, i, mdt, nb);   // batches: no dual energy
   if (LAST) c2p_block_merge<LAST>(a, mdt, nb, smin, ssum);
 }
 
@@ -1142,6 +1155,12 @@ void launch_c2p_de(const C2pArgsT<T> &a, hipStream_t s) {
   }
 }
 }  // namespace
+
+hipError_t launch_ient_sync(const C2pArgsT<double> &a, hipStream_t s) {
+  const VB v = c2p_vb(a.g);
+  hipLaunchKernelGGL(k_ient_sync, dim3(v.n), dim3(256), 0, s, a, v);
+  return hipGetLastError();
+}
 
 template <class T>
 hipError_t launch_c2p(const C2pArgsT<T> &a, hipStream_t s) {
