@@ -161,7 +161,9 @@ int chq_init(chq_ctx *ctx, double t, double tlim, int nlim);
 /* chq_init from a restart: the loop continues at (t, cycle) (main.cpp:196-201) */
 int chq_init_resume(chq_ctx *ctx, double t, int cycle, double tlim, int nlim);
 /* enqueue `nsteps` RK steps (main.cpp:246-292); asynchronous, no host round trip. Steps
- * after t reaches tlim (or cycle reaches nlim) are no-ops on the device. */
+ * after t reaches tlim (or cycle reaches nlim) are no-ops on the device. Without dual energy,
+ * u[IENT] = rho p / rho^gamma of the final state is stored by the next chq_download of
+ * CHQ_U0 (or per-kernel entry point) rather than by every step. */
 int chq_step(chq_ctx *ctx, int nsteps);
 /* enqueue nsteps and time them with HIP events on the context stream; *ms = elapsed */
 int chq_step_timed(chq_ctx *ctx, int nsteps, double *ms);
