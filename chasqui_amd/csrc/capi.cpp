@@ -94,10 +94,11 @@ int sync_state(chq_ctx *c) {
 int relayout(chq_ctx *c, int id, void *host, size_t count, bool to_dev) {
   if (c->batch_stream) HIPCHK(hipStreamSynchronize(c->batch_stream));
   if (int rc = sync_wcur(c)) return rc;
-  if (id == CHQ_U0) {   // an upload replaces every component
-    if (to_dev) c->ient_stale = false;
-    else if (int rc = sync_ient(c)) return rc;
-  }
+  // the deferred u[IENT] of buffer 0 is stored before its state is read or its face B is
+  // replaced (it is computed from both); an upload of u0 replaces every component
+  if (id == CHQ_U0 && to_dev) c->ient_stale = false;
+  else if (id == CHQ_U0 || (to_dev && (id == CHQ_B1F0 || id == CHQ_B2F0 || id == CHQ_B3F0)))
+    if (int rc = sync_ient(c)) return rc;
   void *dev;
   int nc, face, eb, wide;
   if (!array_desc(c, id, &dev, &nc, &face, &eb, &wide)) return fail(CHQ_EINVAL, "unknown array id");
