@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(256) k_corner_march(const CornerArgsT<T> a, co
   const int i = g.is + ii, j = g.js + jj;
   const int k0 = g.ks + by * kseg;
   const int k1 = min(k0 + kseg - 1, g.ks + nk - 1);
-  const long long sj = g.pitch, sk = g.plane, cs = g.cstride;
+  const long long sj = g.pitch, cs = g.cstride;
   const T *ex = a.ecc, *ey = a.ecc + cs, *ez = a.ecc + 2 * cs;
   const bool do1 = i <= g.ie, do2 = j <= g.je;
   // plane k-1 values carried through the march
