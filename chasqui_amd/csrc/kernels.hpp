@@ -110,7 +110,7 @@ using C2pArgs = C2pArgsT<double>;
 // Block b of n takes tile xcd_tile(b, n, grp): each XCD gets runs of grp consecutive tiles,
 // groups of 8 * grp tiles in launch order (the tail past the last whole group keeps b). A
 // bijection on [0, n); grp <= 1 is the identity.
-__device__ __forceinline__ int xcd_tile(int b, int n, int grp) {
+__host__ __device__ __forceinline__ int xcd_tile(int b, int n, int grp) {
   if (grp <= 1) return b;
   const int span = 8 * grp, full = (n / span) * span;
   if (b >= full) return b;
