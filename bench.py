@@ -231,10 +231,30 @@ def ensemble_overrides(nruns, seed):
     return out
 
 
-def ensemble(nruns, steps, precision, seed=7, tune=True):
+def active_rho_e(m):
+    """Density and total energy of a mesh's active cells (host copies)."""
+    u = m.download("u0")
+    box = (slice(m.ks, m.ke + 1), slice(m.js, m.je + 1), slice(m.is_, m.ie + 1))
+    return u[0][box].copy(), u[4][box].copy()
+
+
+def rel_l1(states, ref):
+    """Relative L1 distance of density and total energy, summed over runs, from `ref`."""
+    import numpy as np
+    out = {}
+    for q, name in ((0, "rho"), (1, "E")):
+        num = sum(float(np.abs(a[q] - b[q]).sum()) for a, b in zip(states, ref))
+        den = sum(float(np.abs(b[q]).sum()) for b in ref)
+        out[name] = num / den
+    return out
+
+
+def ensemble(nruns, steps, precision, seed=7, tune=True, ref=None):
     """Config: `nruns` independent 2D Orszag-Tang runs (PPMX + HLLD + RK3) advanced as one batch
     (one launch per kernel class and stage, hipGraph replay), against the same runs stepped one
-    after another on their own. Returns zone-updates/s of both and the speedup."""
+    after another on their own. Returns zone-updates/s of both and the speedup, and the runs'
+    final density and energy; with `ref` (those of the FP64 batch) also the relative L1
+    distance of this batch's final state from it (the same runs, the same number of steps)."""
     from chasqui_amd import Batch
     from chasqui_amd.deck import Deck
     ovs = ensemble_overrides(nruns, seed)
@@ -265,13 +285,25 @@ def ensemble(nruns, steps, precision, seed=7, tune=True):
     ms = b.step_timed(steps)
     grids = b.grids()
     b.close()
+    fin = [active_rho_e(m) for m in meshes]   # after the timed steps
     for m in meshes:
         m.close()
     zu = float(cells) * steps
-    return {"runs": nruns, "cells": cells, "steps": steps, "precision": precision,
-            "batched_zone_updates_per_s": zu / (ms * 1e-3), "batched_ms_per_step": ms / steps,
-            "sequential_zone_updates_per_s": zu / (seq_ms * 1e-3), "sequential_ms_per_step": seq_ms / steps,
-            "speedup_vs_sequential": seq_ms / ms, "tuned_sum_ms": tuned, "grids": grids}
+    res = {"runs": nruns, "cells": cells, "steps": steps, "precision": precision,
+           "batched_zone_updates_per_s": zu / (ms * 1e-3), "batched_ms_per_step": ms / steps,
+           "sequential_zone_updates_per_s": zu / (seq_ms * 1e-3), "sequential_ms_per_step": seq_ms / steps,
+           "speedup_vs_sequential": seq_ms / ms, "tuned_sum_ms": tuned, "grids": grids}
+    if ref is not None:
+        res["l1_rel_vs_fp64"] = rel_l1(fin, ref)
+    return res, fin
+
+
+def ensembles(nruns, steps):
+    """The FP64 and mixed batches of the same runs; the mixed one carries its L1 distance from
+    the FP64 one."""
+    r64, fin64 = ensemble(nruns, steps, "fp64")
+    r32, _ = ensemble(nruns, steps, "mixed", ref=fin64)
+    return {"fp64": r64, "mixed": r32}
 
 
 def shard(n=256, ranks=8, steps=20, warmup=2):
@@ -354,14 +386,23 @@ def serve(njobs, rate_factor=8.0, seed=11, budget_gb=200.0, slots=64, chunk=8):
     from chasqui_amd.deck import Deck
     from chasqui_amd.serve import Server
     jobs = serve_jobs(njobs, seed)
+    # every fifth job's FP64 result is kept (density and energy; the copy is not timed) for the
+    # accuracy of the mixed-precision runs
+    sample = set(range(0, njobs, 5))
+    ref = {}
+    copy_s = 0.0
     t0 = time.perf_counter()
-    for text, ov in jobs:
+    for i, (text, ov) in enumerate(jobs):
         d = Deck(text, ov + ["mhd.precision=fp64"])
         m = d.make_mesh()
         d.close()
         m.run(chunk=chunk)
+        if i in sample:
+            t1 = time.perf_counter()
+            ref[i] = active_rho_e(m)
+            copy_s += time.perf_counter() - t1
         m.close()
-    seq_s = time.perf_counter() - t0
+    seq_s = time.perf_counter() - t0 - copy_s
     seq_rate = njobs / seq_s
     rng = np.random.default_rng(seed + 1)
     gaps = rng.exponential(1.0 / (rate_factor * seq_rate), size=njobs)
@@ -382,6 +423,23 @@ def serve(njobs, rate_factor=8.0, seed=11, budget_gb=200.0, slots=64, chunk=8):
     recs, span, phases = play("mixed")
     tts = np.array([r["tts"] for r in recs])
     tts64 = np.array([r["tts"] for r in recs64])
+    # accuracy of the served mixed-precision runs: a served run's result is bitwise the same run
+    # stepped alone (tests/test_gpu_serve.py), so the sampled jobs are rerun alone in mixed
+    # precision and compared with their FP64 results
+    errs = []
+    for i in sorted(ref):
+        text, ov = jobs[i]
+        d = Deck(text, ov + ["mhd.precision=mixed"])
+        m = d.make_mesh()
+        d.close()
+        m.run(chunk=chunk)
+        errs.append(rel_l1([active_rho_e(m)], [ref[i]]))
+        m.close()
+    acc = {"jobs": len(errs)}
+    for q in ("rho", "E"):
+        v = np.array([e[q] for e in errs])
+        acc[q + "_median"] = float(np.median(v))
+        acc[q + "_max"] = float(v.max())
     return {"jobs": njobs, "sequential_fp64_runs_per_s": seq_rate, "sequential_s": seq_s,
             "served_runs_per_s": njobs / span, "served_makespan_s": span,
             "speedup_vs_sequential_fp64": (njobs / span) / seq_rate,
@@ -391,6 +449,7 @@ def serve(njobs, rate_factor=8.0, seed=11, budget_gb=200.0, slots=64, chunk=8):
             "served_fp64_tts_p99_s": float(np.percentile(tts64, 99)),
             "arrival_rate_per_s": rate_factor * seq_rate,
             "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
+            "l1_rel_vs_fp64": acc,
             "fell_back": int(sum(r["fell_back"] for r in recs)),
             "failed": int(sum(r["state"] != "done" for r in recs)),
             "budget_gb": budget_gb, "slots_per_batch": slots, "chunk_steps": chunk,
@@ -433,7 +492,7 @@ def main():
         if r.returncode == 0:
             stream = json.loads(r.stdout)
     if args.only == "batch":
-        res = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
+        res = ensembles(args.ensemble_runs, 10)
         res["batch_floor"] = batch_floor()
         print(json.dumps(res))
         return
@@ -501,7 +560,7 @@ def main():
     if args.ensemble_runs > 0:
         if n == 256:
             out["shard"] = shard(n)
-        out["batch"] = {p: ensemble(args.ensemble_runs, 10, p) for p in ("fp64", "mixed")}
+        out["batch"] = ensembles(args.ensemble_runs, 10)
         out["batch_floor"] = batch_floor()
     if args.serve_jobs > 0:
         out["serve"] = serve(args.serve_jobs, rate_factor=args.serve_rate, chunk=args.serve_chunk)
