@@ -8,7 +8,7 @@
 // the right eigenvectors. A side's value depends only on its own cell (eigenvectors and the
 // 5-cell window c-2..c+2 its reconstruction reads), so, as for the primitive kernels, each
 // cell is done once and hands its two half-states to its two faces:
-//   * x faces: lane = cell, the window arrives through cross-lane shuffles;
+//   * x faces: lane = cell, the window arrives through DPP whole-wave shifts;
 //   * y and z faces: lane = i, each thread marches a.seg faces along the direction with the
 //     raw window in registers.
 // Sums keep the reference's order (ascending variable index for the projection, ascending
@@ -201,10 +201,10 @@ __global__ void __launch_bounds__(256) k_fluxc_x(const FluxArgsT<double> a, cons
   for (int n = 0; n < NV; ++n) {
     const double q = src[n][row0 + ci];
     r[2][n] = q;
-    r[1][n] = __shfl_up(q, 1, 64);
-    r[3][n] = __shfl_down(q, 1, 64);
-    r[0][n] = W ? __shfl_up(q, 2, 64) : r[1][n];
-    r[4][n] = W ? __shfl_down(q, 2, 64) : r[3][n];
+    r[1][n] = lane_up(q);
+    r[3][n] = lane_dn(q);
+    r[0][n] = W ? lane_up<2>(q) : r[1][n];
+    r[4][n] = W ? lane_dn<2>(q) : r[3][n];
   }
   const double bxn = a.bcc[row0 + ci];   // component 0 = the x1 (normal) field
   double Lf[NV], Rf[NV];
@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) k_fluxc_x(const FluxArgsT<double> a, cons
   finish_cell<double, NV>(Lf, Rf, qc, fco, W && a.flatten, a.eos.dfloor, a.eos.pfloor);
   double Lface[NV];
 #pragma unroll
-  for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
+  for (int n = 0; n < NV; ++n) Lface[n] = lane_up(Lf[n]);
   if (lane < H || lane >= H + FPW || v != vc || ci < a.il || ci > a.iu) return;
   solve_store<double, 0, RS, ENT, NV>(a, row0 + ci, 1, Lface, Rf);
 }

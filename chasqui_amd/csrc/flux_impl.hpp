@@ -7,7 +7,7 @@
 // (fofc.hpp:46-96). L/R states never leave registers. Each cell's two half-states are
 // computed once (recon_cell.hpp) and shared by its two faces:
 //   * x faces (DIR 0): lane = cell over the rows' cells laid end to end; neighbour values
-//     move between lanes with cross-lane shuffles; a wave yields 64 - 5 faces (halo 3 + 2)
+//     move between lanes (DPP whole-wave shifts in FP64); a wave yields 64 - 5 faces (halo 3 + 2)
 //     and may span the end of one row and the start of the next.
 //   * y and z faces (DIR 1, 2): lane = i (coalesced loads), each thread marches along j or
 //     k over a segment of a.seg faces with a register window: every primitive is loaded
@@ -21,6 +21,33 @@
 
 namespace chq {
 namespace fluxk {
+
+// The value of lane l - S (lane_up) or l + S (lane_dn) across the wave, S = 1 or 2. FP64: DPP
+// whole-wave shifts (wave_shr:1 / wave_shl:1, one VALU move per 32-bit half instead of two LDS
+// permutes and their wait; the FP64 x kernels ran 4% faster). Lanes whose source lies outside
+// the wave (the first S lanes for lane_up, the last S for lane_dn) receive 0; the x-face kernels
+// take no result from those halo lanes (their faces need the shifted values of lanes
+// H - 1 .. H + FPW only). Every lane of the wave must be active. FP32 keeps __shfl_up /
+// __shfl_down: at eight waves per SIMD the FP32 x kernel is issue-bound, and the extra VALU
+// moves made it slower.
+template <bool UP>
+__device__ __forceinline__ double wave_shift1(double v) {
+  constexpr int ctrl = UP ? 0x138 : 0x130;
+  const long long b = __double_as_longlong(v);
+  const int l2 = __builtin_amdgcn_mov_dpp(int(b), ctrl, 0xF, 0xF, true);
+  const int h2 = __builtin_amdgcn_mov_dpp(int(b >> 32), ctrl, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2));
+}
+template <int S = 1, class T>
+__device__ __forceinline__ T lane_up(T v) {
+  if constexpr (sizeof(T) == 8) return S == 1 ? wave_shift1<true>(v) : wave_shift1<true>(wave_shift1<true>(v));
+  else return __shfl_up(v, S, 64);
+}
+template <int S = 1, class T>
+__device__ __forceinline__ T lane_dn(T v) {
+  if constexpr (sizeof(T) == 8) return S == 1 ? wave_shift1<false>(v) : wave_shift1<false>(wave_shift1<false>(v));
+  else return __shfl_down(v, S, 64);
+}
 
 
 // Riemann solve of one face from its two half-states (physics.hpp:167-211) into f[NFLX].
@@ -126,17 +153,17 @@ __device__ __forceinline__ void recon_lanes(const T q[NV], const bool flatten, c
     T d[NV], e[NV];
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
-      const T qm = __shfl_up(q[n], 1, 64), qp = __shfl_down(q[n], 1, 64);
+      const T qm = lane_up(q[n]), qp = lane_dn(q[n]);
       d[n] = R::mc(qm, q[n], qp);
     }
 #pragma unroll
     for (int n = 0; n < NV; ++n) {   // e[n] = edge at c+1/2
-      const T qp = __shfl_down(q[n], 1, 64), dp = __shfl_down(d[n], 1, 64);
+      const T qp = lane_dn(q[n]), dp = lane_dn(d[n]);
       e[n] = R::edge(q[n], qp, d[n], dp);
     }
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
-      T lo = __shfl_up(e[n], 1, 64), hi = e[n];
+      T lo = lane_up(e[n]), hi = e[n];
       R::mono(q[n], lo, hi);
       Rf[n] = lo;
       Lf[n] = hi;
@@ -144,11 +171,11 @@ __device__ __forceinline__ void recon_lanes(const T q[NV], const bool flatten, c
   } else {
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
-      const T qm1 = __shfl_up(q[n], 1, 64), qp1 = __shfl_down(q[n], 1, 64);
+      const T qm1 = lane_up(q[n]), qp1 = lane_dn(q[n]);
       T qm2 = qm1, qp2 = qp1;
       if constexpr (W) {
-        qm2 = __shfl_up(q[n], 2, 64);
-        qp2 = __shfl_down(q[n], 2, 64);
+        qm2 = lane_up<2>(q[n]);
+        qp2 = lane_dn<2>(q[n]);
       }
       cell_halves<T, RC>(qm2, qm1, q[n], qp1, qp2, Rf[n], Lf[n]);
     }
@@ -156,16 +183,16 @@ __device__ __forceinline__ void recon_lanes(const T q[NV], const bool flatten, c
   if constexpr (W) {
     if (flatten) {
       const T p = q[4];
-      const T pm2 = __shfl_up(p, 2, 64), pm1 = __shfl_up(p, 1, 64);
-      const T pp1 = __shfl_down(p, 1, 64), pp2 = __shfl_down(p, 2, 64);
-      const T um1 = __shfl_up(q[1], 1, 64), up1 = __shfl_down(q[1], 1, 64);
+      const T pm2 = lane_up<2>(p), pm1 = lane_up(p);
+      const T pp1 = lane_dn(p), pp2 = lane_dn<2>(p);
+      const T um1 = lane_up(q[1]), up1 = lane_dn(q[1]);
       fco = R::flat(pm2, pm1, p, pp1, pp2, um1, up1);
     }
   }
   finish_cell<T, NV>(Lf, Rf, q, fco, W && flatten, dfl, pfl);
   // face i = cell index of this lane: left state = Lf of the lane below
 #pragma unroll
-  for (int n = 0; n < NV; ++n) Lface[n] = __shfl_up(Lf[n], 1, 64);
+  for (int n = 0; n < NV; ++n) Lface[n] = lane_up(Lf[n]);
 }
 
 // ---------------------------------------------------------------- x faces: lane = cell
