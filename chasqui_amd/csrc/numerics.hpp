@@ -302,63 +302,55 @@ template <class T> struct Rsv {
     const T abn = vabs(bn);
     const T sls = sm - abn / qdl;
     const T srs = sm + abn / qdr;
-    const T uls[NFLX] = {dls, dls * sm, dls * vls, dls * wls, els, byls, bzls};
-    const T urs[NFLX] = {drs, drs * sm, drs * vrs, drs * wrs, ers, byrs, bzrs};
-    auto flux_ls = [&](T o[NFLX]) {
-      if constexpr (LH) sflux(dls, sm, vls, wls, els, byls, bzls, bn, pts, o);
-      else fstar_hll(L, bn, gm1, sl, uls, o);
-    };
-    auto flux_rs = [&](T o[NFLX]) {
-      if constexpr (LH) sflux(drs, sm, vrs, wrs, ers, byrs, bzrs, bn, pts, o);
-      else fstar_hll(R, bn, gm1, sr, urs, o);
-    };
-    if (sls > T(0)) { flux_ls(f); return; }
-    if (srs < T(0)) { flux_rs(f); return; }
-    if (bn2 <= Lit<T>::rel * (ptl + ptr)) {                      // riemann.hpp:199-207
-      if (sm > T(0)) flux_ls(f);
-      else if (sm < T(0)) flux_rs(f);
-      else {
-        T fls[NFLX], frs[NFLX];
-        flux_ls(fls);
-        flux_rs(frs);
-        for (int n = 0; n < NFLX; ++n) f[n] = T(0.5) * (fls[n] + frs[n]);
-      }
-      return;
+    // The region of the fan that holds the face (riemann.hpp:193-233): the L* or R* state,
+    // the degenerate case (either star state), or the L** or R** state; S_M = 0 (or NaN)
+    // averages the two sides. Each side's flux is the same expression with L and R exchanged,
+    // so a lane of one side selects its side's inputs and runs one flux evaluation: a wave whose
+    // lanes lie on both sides of the contact evaluates one flux for them instead of two.
+    const bool lreg = sls > T(0), rreg = !lreg && srs < T(0);
+    const bool dd = !lreg && !rreg && !(bn2 <= Lit<T>::rel * (ptl + ptr));   // double-star state
+    const bool two = !lreg && !rreg && !(sm > T(0)) && !(sm < T(0));
+    const bool left = lreg || (!rreg && !(sm < T(0)));
+    T vss = T(0), wss = T(0), byss = T(0), bzss = T(0), elss = T(0), erss = T(0);
+    if (dd) {
+      const T s = sgn(bn);                                         // riemann.hpp:208-233
+      const T isum = T(1) / (qdl + qdr);
+      vss = ((qdl * vls + qdr * vrs) + (byrs - byls) * s) * isum;
+      wss = ((qdl * wls + qdr * wrs) + (bzrs - bzls) * s) * isum;
+      byss = ((qdl * byrs + qdr * byls) + (qdl * qdr) * ((vrs - vls) * s)) * isum;
+      bzss = ((qdl * bzrs + qdr * bzls) + (qdl * qdr) * ((wrs - wls) * s)) * isum;
+      const T vbss = (sm * bn + vss * byss) + wss * bzss;
+      elss = els - qdl * ((vbls - vbss) * s);
+      erss = ers + qdr * ((vbrs - vbss) * s);
     }
-    const T s = sgn(bn);                                           // riemann.hpp:208-233
-    const T isum = T(1) / (qdl + qdr);
-    const T vss = ((qdl * vls + qdr * vrs) + (byrs - byls) * s) * isum;
-    const T wss = ((qdl * wls + qdr * wrs) + (bzrs - bzls) * s) * isum;
-    const T byss = ((qdl * byrs + qdr * byls) + (qdl * qdr) * ((vrs - vls) * s)) * isum;
-    const T bzss = ((qdl * bzrs + qdr * bzls) + (qdl * qdr) * ((wrs - wls) * s)) * isum;
-    const T vbss = (sm * bn + vss * byss) + wss * bzss;
-    const T elss = els - qdl * ((vbls - vbss) * s);
-    const T erss = ers + qdr * ((vbrs - vbss) * s);
-    auto flux_lss = [&](T o[NFLX]) {
-      if constexpr (LH) sflux(dls, sm, vss, wss, elss, byss, bzss, bn, pts, o);
-      else {
-        T fls[NFLX];
-        flux_ls(fls);
-        const T ulss[NFLX] = {dls, dls * sm, dls * vss, dls * wss, elss, byss, bzss};
-        for (int n = 0; n < NFLX; ++n) o[n] = fls[n] + sls * (ulss[n] - uls[n]);
+    // one side's flux: F* = F(S) + S_S (U* - U) and, in the double-star region,
+    // F** = F* + S*_S (U** - U*) (riemann.hpp:170-183, 219-229); LHLLD takes the flux of the
+    // intermediate state itself (sflux)
+    auto side = [&](const bool lf, T o[NFLX]) {
+      const T ds = lf ? dls : drs, vs = lf ? vls : vrs, ws = lf ? wls : wrs;
+      const T bys = lf ? byls : byrs, bzs = lf ? bzls : bzrs, es = lf ? els : ers;
+      if constexpr (LH) {
+        if (dd) sflux(ds, sm, vss, wss, lf ? elss : erss, byss, bzss, bn, pts, o);
+        else sflux(ds, sm, vs, ws, es, bys, bzs, bn, pts, o);
+      } else {
+        const St<T> S{lf ? L.d : R.d, lf ? L.u : R.u, lf ? L.v : R.v, lf ? L.w : R.w,
+                      lf ? L.p : R.p, lf ? L.by : R.by, lf ? L.bz : R.bz};
+        const T us[NFLX] = {ds, ds * sm, ds * vs, ds * ws, es, bys, bzs};
+        fstar_hll(S, bn, gm1, lf ? sl : sr, us, o);
+        if (dd) {
+          const T sw = lf ? sls : srs;
+          const T uss[NFLX] = {ds, ds * sm, ds * vss, ds * wss, lf ? elss : erss, byss, bzss};
+          for (int n = 0; n < NFLX; ++n) o[n] = o[n] + sw * (uss[n] - us[n]);
+        }
       }
     };
-    auto flux_rss = [&](T o[NFLX]) {
-      if constexpr (LH) sflux(drs, sm, vss, wss, erss, byss, bzss, bn, pts, o);
-      else {
-        T frs[NFLX];
-        flux_rs(frs);
-        const T urss[NFLX] = {drs, drs * sm, drs * vss, drs * wss, erss, byss, bzss};
-        for (int n = 0; n < NFLX; ++n) o[n] = frs[n] + srs * (urss[n] - urs[n]);
-      }
-    };
-    if (sm > T(0)) flux_lss(f);
-    else if (sm < T(0)) flux_rss(f);
-    else {
+    if (two) {   // both sides in one straight run, so their arithmetic interleaves
       T a[NFLX], b[NFLX];
-      flux_lss(a);
-      flux_rss(b);
+      side(true, a);
+      side(false, b);
       for (int n = 0; n < NFLX; ++n) f[n] = T(0.5) * (a[n] + b[n]);
+    } else {
+      side(left, f);
     }
   }
 };
