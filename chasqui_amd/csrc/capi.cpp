@@ -186,6 +186,10 @@ int check_cfg(const chq_mesh_desc *m, const chq_config *c) {
   if (m->ng < ngmin) return fail(CHQ_EINVAL, "nghost too small for the reconstruction method");
   for (int d = 0; d < 3; ++d)
     if (c->bc[d] < 0 || c->bc[d] > 2) return fail(CHQ_EINVAL, "unknown boundary condition");
+  // cell and thread indices of the kernels are 32-bit (far above what fits in one GPU's HBM)
+  const long long padded = (long long)(m->nx1 + 2 * m->ng) * (m->nx2 > 1 ? m->nx2 + 2 * m->ng : 1) *
+                           (m->nx3 > 1 ? m->nx3 + 2 * m->ng : 1);
+  if (padded >= (1LL << 30)) return fail(CHQ_EINVAL, "mesh too large: 2^30 cells or more, ghosts included");
   // a periodic ghost fill reads nx cells back from the ghost: it needs nx >= ng
   if (m->nx1 < m->ng || (m->nx2 > 1 && m->nx2 < m->ng) || (m->nx3 > 1 && m->nx3 < m->ng))
     return fail(CHQ_EINVAL, "every multi-cell dimension needs at least nghost active cells");

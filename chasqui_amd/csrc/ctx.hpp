@@ -135,6 +135,8 @@ struct chq_ctx {
     if (d == 2) ku = g.ke + 1; else if (g.three_d) { kl = g.ks - 1; ku = g.ke + 1; }
     a.il = il; a.iu = iu; a.jl = jl; a.ju = ju; a.kl = kl; a.ku = ku;
     a.seg = d == 0 ? fluxk::SEG : fluxk::march_seg(fluxk::march_threads(il, iu, jl, ju, kl, ku, d, fluxk::SEG));
+    if (d == 0) { a.q1 = fast_div(g.ni); a.q2 = fast_div(ju - jl + 1); }
+    else { a.q1 = fast_div(iu - il + 1); a.q2 = fast_div(d == 1 ? ku - kl + 1 : ju - jl + 1); }
     a.w = f64.w[wset];
     a.bcc = f64.bcc[wset];
     a.bn = bf[cur][d];
@@ -257,6 +259,7 @@ struct chq_ctx {
     a.lhlld_phi = a0.lhlld_phi;
     a.il = a0.il; a.iu = a0.iu; a.jl = a0.jl; a.ju = a0.ju; a.kl = a0.kl; a.ku = a0.ku;
     a.seg = a0.seg;
+    a.q1 = a0.q1; a.q2 = a0.q2;
     a.w = f.w[wset]; a.bcc = f.bcc[wset]; a.bn = a0.bn; a.flx = f.flx[d];
     static const int et2_id[3] = {1, 2, 5}, et1_id[3] = {0, 3, 4};
     a.et2 = f.ef[et2_id[d]];

@@ -188,11 +188,12 @@ __global__ void __launch_bounds__(256) k_fluxc_x(const FluxArgsT<double> a, cons
   const Geom &g = a.g;
   // the same lane -> cell map as the primitive x kernel (rows laid end to end)
   const int nrow_j = a.ju - a.jl + 1;
-  const long long ncell = (long long)nseg * nrow_j * (a.ku - a.kl + 1);
-  const long long v = (long long)wave * FPW - H + lane;
-  const long long vc = v < 0 ? 0 : (v >= ncell ? ncell - 1 : v);
-  const int row = int(vc / nseg), ci = int(vc - (long long)row * nseg);
-  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
+  const int ncell = nseg * nrow_j * (a.ku - a.kl + 1);   // below 2^30 (the mesh size limit)
+  const int v = wave * FPW - H + lane;
+  const unsigned vc = unsigned(v < 0 ? 0 : (v >= ncell ? ncell - 1 : v));
+  const unsigned row = fdiv(vc, a.q1), kr = fdiv(row, a.q2);   // vc / nseg, row / nrow_j
+  const int ci = int(vc - row * unsigned(nseg));
+  const int j = a.jl + int(row - kr * unsigned(nrow_j)), k = a.kl + int(kr);
   const long long row0 = g.at(k, j, 0);
   const double *src[NV];
   var_ptrs<double, 0, NV>(a, src);
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(256) k_fluxc_x(const FluxArgsT<double> a, cons
   double Lface[NV];
 #pragma unroll
   for (int n = 0; n < NV; ++n) Lface[n] = lane_up(Lf[n]);
-  if (lane < H || lane >= H + FPW || v != vc || ci < a.il || ci > a.iu) return;
+  if (lane < H || lane >= H + FPW || v != int(vc) || ci < a.il || ci > a.iu) return;
   solve_store<double, 0, RS, ENT, NV>(a, row0 + ci, 1, Lface, Rf);
 }
 

@@ -208,11 +208,12 @@ __device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nse
   // this lane's cell in the rows laid end to end (nseg = cells per row); lanes before the
   // first or past the last cell read any valid cell and compute no face
   const int nrow_j = a.ju - a.jl + 1;
-  const long long ncell = (long long)nseg * nrow_j * (a.ku - a.kl + 1);
-  const long long v = (long long)wave * FPW - H + lane;
-  const long long vc = v < 0 ? 0 : (v >= ncell ? ncell - 1 : v);
-  const int row = int(vc / nseg), ci = int(vc - (long long)row * nseg);
-  const int j = a.jl + row % nrow_j, k = a.kl + row / nrow_j;
+  const int ncell = nseg * nrow_j * (a.ku - a.kl + 1);   // below 2^30 (the mesh size limit)
+  const int v = wave * FPW - H + lane;
+  const unsigned vc = unsigned(v < 0 ? 0 : (v >= ncell ? ncell - 1 : v));
+  const unsigned row = fdiv(vc, a.q1), kr = fdiv(row, a.q2);   // vc / nseg, row / nrow_j
+  const int ci = int(vc - row * unsigned(nseg));
+  const int j = a.jl + int(row - kr * unsigned(nrow_j)), k = a.kl + int(kr);
   const long long row0 = g.at(k, j, 0);
   const T *src[NV];
   var_ptrs<T, 0, NV>(a, src);
@@ -223,7 +224,7 @@ __device__ __forceinline__ void flux_x_wave(const FluxArgsT<T> &a, const int nse
   recon_lanes<T, RC, NV>(q, a.flatten, T(a.eos.dfloor), T(a.eos.pfloor), Lface, Rf);
   // face ci (between cells ci - 1 and ci): its stencil ci - H .. ci + 2 lies in this row and in
   // this wave's lanes
-  if (lane < H || lane >= H + FPW || v != vc || ci < a.il || ci > a.iu) return;
+  if (lane < H || lane >= H + FPW || v != int(vc) || ci < a.il || ci > a.iu) return;
   solve_store<T, 0, RS, ENT, NV>(a, row0 + ci, 1, Lface, Rf);
 }
 

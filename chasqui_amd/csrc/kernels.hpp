@@ -13,6 +13,26 @@
 namespace chq {
 
 // A face-flux launch: one direction, faces [il..iu] x [jl..ju] x [kl..ku] (inclusive).
+// Division by a divisor fixed before launch, for numerators below 2^31 (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994, Theorem 4.2): with
+// l = ceil(log2 d) and m = ceil(2^(31+l) / d), 0 <= m d - 2^(31+l) < d <= 2^l, so
+// floor(n / d) = floor(n m / 2^(31+l)) for every 0 <= n < 2^31, and m < 2^32. A 32 x 32 -> 64-bit
+// multiply and a shift per lane instead of an integer division (about 100 instructions for a
+// 64-bit one). 1 <= d < 2^31.
+struct FastDiv {
+  unsigned m;
+  int s;
+};
+__host__ __device__ inline FastDiv fast_div(int d) {
+  int l = 0;
+  while ((1LL << l) < d) ++l;
+  const unsigned long long p = 1ULL << (31 + l);
+  return FastDiv{unsigned((p + (unsigned long long)d - 1) / (unsigned long long)d), 31 + l};
+}
+__host__ __device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f) {
+  return unsigned(((unsigned long long)n * f.m) >> f.s);
+}
+
 template <class T>
 struct FluxArgsT {
   Geom g;
@@ -23,6 +43,8 @@ struct FluxArgsT {
   double lhlld_phi;
   int il, iu, jl, ju, kl, ku;
   int seg;            // faces per thread of the y/z march (fluxk::march_seg)
+  FastDiv q1, q2;     // x faces: by the cells per row and by the rows per plane of the face
+                      // range (x_shape); y/z faces: by nx and by nt (m_item)
   const T *w;         // primitives, 6 components (IENT used only when entropy is transported)
   const T *bcc;       // cell-centred B, 3 components
   const double *bn;   // face-normal field of this direction (the stage-input face B)
@@ -215,10 +237,10 @@ __host__ __device__ inline bool m_item(const FluxArgsT<T> &a, long long lin, int
   const int nx = a.iu - a.il + 1;
   const int nt = DIR == 1 ? a.ku - a.kl + 1 : a.ju - a.jl + 1;
   const int nd = DIR == 1 ? a.ju - a.jl + 1 : a.ku - a.kl + 1;
-  const long long r = lin / nx;
-  i = a.il + int(lin - r * nx);
-  t = int(r % nt);
-  sg = int(r / nt);
+  const unsigned r = fdiv(unsigned(lin), a.q1), rs = fdiv(r, a.q2);   // lin < 2^31 (mesh limit)
+  i = a.il + int(unsigned(lin) - r * unsigned(nx));
+  t = int(r - rs * unsigned(nt));
+  sg = int(rs);
   return sg < div_up(nd, a.seg);
 }
 
