@@ -22,14 +22,14 @@
 namespace chq {
 namespace fluxk {
 
-// The value of lane l - S (lane_up) or l + S (lane_dn) across the wave, S = 1 or 2. FP64: DPP
-// whole-wave shifts (wave_shr:1 / wave_shl:1, one VALU move per 32-bit half instead of two LDS
-// permutes and their wait; the FP64 x kernels ran 4% faster). Lanes whose source lies outside
-// the wave (the first S lanes for lane_up, the last S for lane_dn) receive 0; the x-face kernels
-// take no result from those halo lanes (their faces need the shifted values of lanes
-// H - 1 .. H + FPW only). Every lane of the wave must be active. FP32 keeps __shfl_up /
-// __shfl_down: at eight waves per SIMD the FP32 x kernel is issue-bound, and the extra VALU
-// moves made it slower.
+// The value of lane l - S (lane_up) or l + S (lane_dn) across the wave, S = 1 or 2. FP64, S = 1:
+// DPP whole-wave shifts (wave_shr:1 / wave_shl:1, one VALU move per 32-bit half instead of two
+// LDS permutes and their wait; the FP64 x kernels ran 4% faster). The first lane (lane_up) or
+// the last (lane_dn) receives 0; the x-face kernels take no result from those halo lanes (their
+// faces need the shifted values of lanes H - 1 .. H + FPW only). Every lane of the wave must be
+// active. A shift by 2 (the wide stencils' outer cells) stays an LDS permute: as two DPP
+// shifts it made the PPMX x kernel slower. FP32 keeps __shfl_up / __shfl_down: at eight waves per
+// SIMD the FP32 x kernel is issue-bound, and the extra VALU moves made it slower.
 template <bool UP>
 __device__ __forceinline__ double wave_shift1(double v) {
   constexpr int ctrl = UP ? 0x138 : 0x130;
@@ -40,12 +40,12 @@ __device__ __forceinline__ double wave_shift1(double v) {
 }
 template <int S = 1, class T>
 __device__ __forceinline__ T lane_up(T v) {
-  if constexpr (sizeof(T) == 8) return S == 1 ? wave_shift1<true>(v) : wave_shift1<true>(wave_shift1<true>(v));
+  if constexpr (sizeof(T) == 8 && S == 1) return wave_shift1<true>(v);
   else return __shfl_up(v, S, 64);
 }
 template <int S = 1, class T>
 __device__ __forceinline__ T lane_dn(T v) {
-  if constexpr (sizeof(T) == 8) return S == 1 ? wave_shift1<false>(v) : wave_shift1<false>(wave_shift1<false>(v));
+  if constexpr (sizeof(T) == 8 && S == 1) return wave_shift1<false>(v);
   else return __shfl_down(v, S, 64);
 }
 
