@@ -18,12 +18,13 @@ namespace chq {
 // l = ceil(log2 d) and m = ceil(2^(31+l) / d), 0 <= m d - 2^(31+l) < d <= 2^l, so
 // floor(n / d) = floor(n m / 2^(31+l)) for every 0 <= n < 2^31, and m < 2^32. A 32 x 32 -> 64-bit
 // multiply and a shift per lane instead of an integer division (about 100 instructions for a
-// 64-bit one). 1 <= d < 2^31.
+// 64-bit one). 1 <= d < 2^31 (a d below 1 is taken as 1).
 struct FastDiv {
   unsigned m;
   int s;
 };
 __host__ __device__ inline FastDiv fast_div(int d) {
+  if (d < 1) d = 1;
   int l = 0;
   while ((1LL << l) < d) ++l;
   const unsigned long long p = 1ULL << (31 + l);
